@@ -1,0 +1,155 @@
+"""Benchmark of the perturbation-stage hot path on one MI355X.
+
+A step is one pass of the hot path over one batch: every k-mode of the workload integrated with
+Rodas5P from τ_ini(k) to τ0 in one persistent launch, followed by the P(k) reduction.  The
+default workload is the 1024-mode massive-neutrino configuration ("cfg1": photon and massless-ν
+lmax 16, one massive ν of 0.06 eV with 4 momentum bins and lmax 10, n = 100 unknowns per mode,
+k log-spaced in [1e-5, 10]/Mpc, rtol = atol = 1e-5).  Inputs (background tables, instances) are
+resident in device memory before the timed region.
+
+Prints ONE JSON line with the metric, the roofline of the dominant kernel (algorithmic FP64
+flops counted by the emitter × the solver's own step counts, over the kernel's HIP-event time)
+and a CPU baseline from the oracle on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "symboltz.jl_amd"))
+sys.path.insert(0, str(ROOT))
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(name):
+    from symboltz import models
+    from symboltz.constants import k_dimensionless
+    base = {"h": 0.67, "Omega_b0": 0.0224 / 0.67**2, "Omega_c0": 0.120 / 0.67**2, "T0": 2.7255, "YHe": 0.245,
+            "ln_As1e10": math.log(2.1e-9 * 1e10), "ns": 0.966, "z_re": 7.7}
+    if name == "cfg1":
+        M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
+        pars = dict(base, Neff=2.046, N=1, m_eV=0.06)
+        ks = k_dimensionless(np.logspace(-5, 1, 1024), base["h"])
+        desc = ("LCDM + 1 massive nu (0.06 eV, 4 bins, lmax 10), photon/massless-nu lmax 16, n=100; "
+                "1024 k log-spaced in [1e-5, 10]/Mpc; rtol=atol=1e-5; P(k) at z=0")
+    elif name == "cfg0":
+        M = models.LCDM(lmax=10, h=False)
+        pars = dict(base, Neff=3.046)
+        ks = k_dimensionless(np.logspace(-4, 0, 100), base["h"])
+        desc = "LCDM massless nu lmax 10, n=38; 100 k log-spaced in [1e-4, 1]/Mpc; rtol=atol=1e-5"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return M, pars, ks, desc
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg1")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=32, help="oracle baseline: every N-th mode")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described in "
+              f"DESIGN.md and not measured here.", file=sys.stderr)
+        raise SystemExit(2)
+
+    from symboltz import _lib
+    from symboltz.problem import CosmologyProblem, solvebg
+    from symboltz.solver import GpuModel
+
+    if _lib.device_count() < 1:
+        raise SystemExit("bench.py: no GPU visible")
+    M, pars, ks, desc = workload(args.workload)
+    prob = CosmologyProblem(M, pars)
+    t_bg = time.perf_counter()
+    bg = solvebg(prob)
+    t_bg = time.perf_counter() - t_bg
+    gm = GpuModel(M)
+    gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), bg.t[0], bg.t[-1])
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes)
+    log(f"workload {args.workload}: n={gm.n}, {len(ks)} modes, background {t_bg*1e3:.1f} ms ({len(bg.t)} knots)")
+    for w in range(args.warmup):
+        ms = plan.run()
+        plan.pk()
+        log(f"warmup {w}: kernel {ms:.1f} ms")
+    kernel_ms = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        kernel_ms.append(plan.run())
+        pk = plan.pk()
+    elapsed = time.perf_counter() - t_start
+    for s, ms in enumerate(kernel_ms):
+        log(f"step {s}: kernel {ms:.1f} ms")
+    res = plan.fetch()
+    ms_per_step = elapsed / args.steps * 1e3
+    n_modes = len(ks)
+    value = n_modes / (elapsed / args.steps)
+    steps_total = int(res["stats"][:, 0].sum() + res["stats"][:, 1].sum())
+    flops = steps_total * gm.em.flops_step
+    kern_s = float(np.mean(kernel_ms)) * 1e-3
+    achieved = flops / kern_s / 1e12
+    ok = bool(np.all(res["retcode"] == 0))
+    log(f"retcodes ok={ok}; steps/mode min {res['stats'][:,0].min()} max {res['stats'][:,0].max()} "
+        f"total {steps_total}; flops/step {gm.em.flops_step}; {achieved:.3f} TFLOP/s")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle
+        O = oracle.Oracle(M, bg.P, bg.t, bg.u, bg.du)
+        sel = np.arange(0, n_modes, args.cpu_sample)
+        cores = os.cpu_count() or 1
+        cores = min(cores, 16)
+        tc = time.perf_counter()
+        o = O.solve(ks[sel], t0[sel], nthreads=cores)
+        tc = time.perf_counter() - tc
+        po = np.array([O.p0(k) * O.delta_m(k, bg.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks[sel])])
+        parity = float(np.max(np.abs(pk[sel] / po - 1)))
+        cpu = {"value": len(sel) / tc, "unit": "k-mode solves/s", "cores": cores, "kind": "port",
+               "sample": f"every {args.cpu_sample}th mode of the workload ({len(sel)} modes, same k spread), "
+                         f"{tc:.1f} s on {cores} host threads; max |P_gpu/P_cpu - 1| = {parity:.2e}"}
+        log(f"cpu baseline: {cpu['value']:.1f} solves/s, parity {parity:.2e}")
+
+    out = {
+        "metric": "k-mode solves/s at rtol 1e-5 (plain GPU Rodas5P, P(k) at z=0)",
+        "value": value,
+        "unit": "k-mode solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "description": desc, "n_modes": n_modes, "n_unknowns": gm.n,
+                   "background_ms": t_bg * 1e3, "all_success": ok},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "sbz_rodas5p_kernel", "kernel_ms": float(np.mean(kernel_ms)),
+                     "flops_per_launch": flops, "flops_per_step": gm.em.flops_step, "steps_per_launch": steps_total},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,6 +50,71 @@ int sbz_device_count(int* count);
 int sbz_background_solve(double* params, double rtol, double atol, int max_knots, double* t,
                          double* u, double* du, int* n_knots, int* n_rejected);
 
+/* Background right-hand side at one state (host; used by tests of the background stage). */
+int sbz_background_rhs(const double* params, const double* u, double* du);
+
+/* ---- perturbation solver ------------------------------------------------------------------ */
+
+typedef struct sbz_model sbz_model;
+typedef struct sbz_plan sbz_plan;
+
+/* Model descriptor.  n: number of perturbation unknowns (layout: Φ, photon F_0..F_l, G_0..G_l,
+ * massless ν F_0..F_l, massive ν ψ_i,0..l for each momentum bin, δ_c, θ_c, δ_b, θ_b[, δ_X, θ_X]);
+ * nnz_lu: slots of the static LU pattern; nw: workspace doubles per lane; cache_path: code-object
+ * file loaded if present and written after a hiprtc compile otherwise. */
+typedef struct {
+    int n;
+    int nnz_lu;
+    int nw;
+    const char* cache_path;
+} sbz_model_desc;
+
+/* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number
+ * of concurrent modes in flight on the device (0: one per instance). */
+typedef struct {
+    double rtol;
+    double atol;
+    long long maxiters;
+    int lanes;
+} sbz_opts;
+
+/* Compile emitted HIP source with hiprtc for gfx950 into a code-object file. */
+int sbz_compile(const char* hip_src, const char* out_path);
+
+/* Create a model handle from the emitted source (sbz_model_create in place of MTK's generated
+ * f!/jac!, src/solve.jl:215-227).  The code object comes from desc->cache_path when it exists,
+ * else from hiprtc.  Fails with SBZ_ERR_NODEV when no GPU is visible. */
+int sbz_model_create(const char* hip_src, const sbz_model_desc* desc, sbz_model** out);
+int sbz_model_destroy(sbz_model* m);
+
+/* Upload the background spline of parameter set `set`: knot times t[n], unknowns u[n][5] and
+ * their derivatives du[n][5] (spline(bgsol), src/utils.jl:118-127) and the SBZ_NPAR parameters
+ * including τ0 and κ0 (setuppt copies them into the perturbation problem, src/solve.jl:496-513). */
+int sbz_bg_upload(sbz_model* m, int set, const double* t, const double* u, const double* du, int n_knots,
+                  const double* params);
+
+/* Solve plan: instances (set, k, τ_start) resident on the device.  Replaces the per-mode task
+ * loop of solvept (src/solve.jl:543-569; ptprobgen clamps τ_start, :524-531).  n_save output times
+ * shared by all instances (saveat). */
+int sbz_plan_create(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0, int n_inst,
+                    const sbz_opts* opts, const double* saveat, int n_save, sbz_plan** out);
+/* Launch the persistent solver kernel and wait; *kernel_ms receives its duration (HIP events on
+ * the model's stream). */
+int sbz_plan_run(sbz_plan* p, float* kernel_ms);
+/* Copy results to host: final states [n_inst][n], saved states [n_inst][n_save][n] (may be
+ * null), per-mode return codes, and stats [n_inst][6] = accepted steps, rejected steps, f evals,
+ * J evals, LU factorizations, linear solves (DEStats, src/solve.jl:885-891). */
+int sbz_plan_fetch(sbz_plan* p, double* out_u, double* out_save, int* retcode, long long* stats);
+/* Matter power spectrum P(k, τ0) = P0(k) Δ_m² per instance from the final states
+ * (spectrum_matter, src/observables/fourier.jl:79-101). */
+int sbz_plan_pk(sbz_plan* p, double* pk);
+int sbz_plan_destroy(sbz_plan* p);
+
+/* One-shot: create a plan, run, fetch, destroy. */
+int sbz_solve_batch(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0, int n_inst,
+                    const sbz_opts* opts, const double* saveat, int n_save, double* out_u, double* out_save,
+                    int* retcode, long long* stats);
+
 #ifdef __cplusplus
 }
 #endif

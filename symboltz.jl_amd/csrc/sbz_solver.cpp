@@ -1,0 +1,415 @@
+// Host runtime of the perturbation solver: model handles (code objects compiled by hiprtc or
+// ahead of time by hipcc, cached on disk), background tables, and solve plans whose inputs stay
+// resident in device memory between launches.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "sbz_internal.h"
+
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            sbz_set_error("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return SBZ_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+struct SbzSetTable {
+    std::vector<double> knots;  // t[nk], u[nk][5], du[nk][5]
+    int nk = 0;
+    double params[SBZ_NPAR];
+    bool valid = false;
+};
+
+struct sbz_model {
+    int n = 0, nnz_lu = 0, nw = 0;
+    hipModule_t module = nullptr;
+    hipFunction_t k_solve = nullptr, k_pk = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::vector<SbzSetTable> sets;
+};
+
+// Mirrors struct SbzSolveArgs of sbz_kernel.hip (passed by value as the kernel argument).
+struct SbzSolveArgsHost {
+    const double* params;
+    const double* knots;
+    const long long* knot_off;
+    const int* knot_n;
+    const int* inst_set;
+    const double* inst_k;
+    const double* inst_t0;
+    const int* order;
+    int n_inst;
+    double rtol, atol;
+    long long maxiters;
+    const double* saveat;
+    int n_save;
+    double* out_u;
+    double* out_save;
+    int* retcode;
+    long long* stats;
+    unsigned int* queue;
+    double* ws;
+    int S;
+};
+
+struct sbz_plan {
+    sbz_model* m = nullptr;
+    int n_inst = 0, n_sets = 0, n_save = 0, lanes = 0, block = 64;
+    double rtol = 1e-5, atol = 1e-5;
+    long long maxiters = 100000;
+    // device buffers
+    double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
+    double *d_out = nullptr, *d_save = nullptr, *d_ws = nullptr, *d_pk = nullptr;
+    long long *d_koff = nullptr, *d_stats = nullptr;
+    int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
+    unsigned int* d_queue = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+};
+
+static bool read_file(const char* path, std::vector<char>& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !out.empty();
+}
+
+static int compile_hiprtc(const char* src, std::vector<char>& code) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src, "sbz_model.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        sbz_set_error("hiprtcCreateProgram failed");
+        return SBZ_ERR_COMPILE;
+    }
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (r != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        sbz_set_error("hiprtc compile failed: %.900s", log.c_str());
+        hiprtcDestroyProgram(&prog);
+        return SBZ_ERR_COMPILE;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return SBZ_OK;
+}
+
+extern "C" int sbz_compile(const char* hip_src, const char* out_path) {
+    sbz_clear_error();
+    if (!hip_src || !out_path) {
+        sbz_set_error("sbz_compile: null argument");
+        return SBZ_ERR_ARG;
+    }
+    std::vector<char> code;
+    int st = compile_hiprtc(hip_src, code);
+    if (st != SBZ_OK) return st;
+    std::ofstream f(out_path, std::ios::binary);
+    f.write(code.data(), (std::streamsize)code.size());
+    return f ? SBZ_OK : SBZ_ERR_ARG;
+}
+
+extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc, sbz_model** out) {
+    sbz_clear_error();
+    if (!desc || !out || desc->n <= 0 || desc->nw <= 0) {
+        sbz_set_error("sbz_model_create: invalid descriptor");
+        return SBZ_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        sbz_set_error("sbz_model_create: no GPU visible");
+        return SBZ_ERR_NODEV;
+    }
+    std::vector<char> code;
+    bool have = desc->cache_path && read_file(desc->cache_path, code);
+    if (!have) {
+        if (!hip_src) {
+            sbz_set_error("sbz_model_create: no cached code object at %s and no source",
+                          desc->cache_path ? desc->cache_path : "(null)");
+            return SBZ_ERR_COMPILE;
+        }
+        int st = compile_hiprtc(hip_src, code);
+        if (st != SBZ_OK) return st;
+        if (desc->cache_path) {
+            std::ofstream f(desc->cache_path, std::ios::binary);
+            f.write(code.data(), (std::streamsize)code.size());
+        }
+    }
+    sbz_model* m = new sbz_model();
+    m->n = desc->n;
+    m->nnz_lu = desc->nnz_lu;
+    m->nw = desc->nw;
+    (void)hipGetDevice(&m->device);
+    hipError_t e = hipModuleLoadData(&m->module, code.data());
+    if (e != hipSuccess) {
+        sbz_set_error("hipModuleLoadData: %s", hipGetErrorString(e));
+        delete m;
+        return SBZ_ERR_COMPILE;
+    }
+    if (hipModuleGetFunction(&m->k_solve, m->module, "sbz_rodas5p_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_pk, m->module, "sbz_pk_kernel") != hipSuccess) {
+        sbz_set_error("code object lacks the solver kernels");
+        (void)hipModuleUnload(m->module);
+        delete m;
+        return SBZ_ERR_COMPILE;
+    }
+    if (hipStreamCreate(&m->stream) != hipSuccess) {
+        sbz_set_error("hipStreamCreate failed");
+        (void)hipModuleUnload(m->module);
+        delete m;
+        return SBZ_ERR_HIP;
+    }
+    *out = m;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_model_destroy(sbz_model* m) {
+    if (!m) return SBZ_OK;
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    if (m->module) (void)hipModuleUnload(m->module);
+    delete m;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const double* u, const double* du,
+                             int n_knots, const double* params) {
+    sbz_clear_error();
+    if (!m || set < 0 || !t || !u || !du || !params || n_knots < 2) {
+        sbz_set_error("sbz_bg_upload: invalid argument");
+        return SBZ_ERR_ARG;
+    }
+    for (int i = 1; i < n_knots; ++i)
+        if (!(t[i] > t[i - 1])) {
+            sbz_set_error("sbz_bg_upload: knot times are not strictly increasing at %d", i);
+            return SBZ_ERR_ARG;
+        }
+    if ((int)m->sets.size() <= set) m->sets.resize(set + 1);
+    SbzSetTable& s = m->sets[set];
+    s.nk = n_knots;
+    s.knots.assign(11 * (size_t)n_knots, 0.0);
+    std::memcpy(s.knots.data(), t, sizeof(double) * n_knots);
+    std::memcpy(s.knots.data() + n_knots, u, sizeof(double) * 5 * n_knots);
+    std::memcpy(s.knots.data() + 6 * (size_t)n_knots, du, sizeof(double) * 5 * n_knots);
+    std::memcpy(s.params, params, sizeof(s.params));
+    s.valid = true;
+    return SBZ_OK;
+}
+
+static void plan_free(sbz_plan* p) {
+    void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_ws,
+                    p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+}
+
+template <class T>
+static hipError_t upload(T** dst, const T* src, size_t n) {
+    hipError_t e = hipMalloc((void**)dst, sizeof(T) * std::max<size_t>(n, 1));
+    if (e != hipSuccess || n == 0) return e;
+    return hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
+}
+
+extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0,
+                               int n_inst, const sbz_opts* opts, const double* saveat, int n_save, sbz_plan** out) {
+    sbz_clear_error();
+    if (!m || !inst_set || !inst_k || !inst_t0 || !opts || !out || n_inst < 0 || n_save < 0 ||
+        (n_save > 0 && !saveat)) {
+        sbz_set_error("sbz_plan_create: invalid argument");
+        return SBZ_ERR_ARG;
+    }
+    if (!(opts->rtol > 0) || !(opts->atol > 0)) {
+        sbz_set_error("sbz_plan_create: tolerances must be positive");
+        return SBZ_ERR_ARG;
+    }
+    int n_sets = (int)m->sets.size();
+    for (int i = 0; i < n_inst; ++i) {
+        if (inst_set[i] < 0 || inst_set[i] >= n_sets || !m->sets[inst_set[i]].valid) {
+            sbz_set_error("sbz_plan_create: instance %d refers to parameter set %d without a background table", i,
+                          inst_set[i]);
+            return SBZ_ERR_ARG;
+        }
+    }
+    for (int i = 1; i < n_save; ++i)
+        if (saveat[i] < saveat[i - 1]) {
+            sbz_set_error("sbz_plan_create: saveat must be ascending");
+            return SBZ_ERR_ARG;
+        }
+    (void)hipSetDevice(m->device);
+    sbz_plan* p = new sbz_plan();
+    p->m = m;
+    p->n_inst = n_inst;
+    p->n_sets = n_sets;
+    p->n_save = n_save;
+    p->rtol = opts->rtol;
+    p->atol = opts->atol;
+    p->maxiters = opts->maxiters > 0 ? opts->maxiters : 100000;
+    p->block = 64;
+    int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
+    lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
+    lanes = (lanes + p->block - 1) / p->block * p->block;
+    p->lanes = lanes;
+    // concatenated background tables
+    std::vector<double> params((size_t)n_sets * SBZ_NPAR), knots;
+    std::vector<long long> koff(n_sets);
+    std::vector<int> kn(n_sets);
+    for (int s = 0; s < n_sets; ++s) {
+        std::memcpy(params.data() + (size_t)s * SBZ_NPAR, m->sets[s].params, sizeof(double) * SBZ_NPAR);
+        koff[s] = (long long)knots.size();
+        kn[s] = m->sets[s].nk;
+        knots.insert(knots.end(), m->sets[s].knots.begin(), m->sets[s].knots.end());
+    }
+    // longest-first order: estimated cost grows with k (step count ∝ k at high k)
+    std::vector<int> order(n_inst);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst_k[a] > inst_k[b]; });
+    hipError_t e = hipSuccess;
+#define UP(dst, src, n) \
+    if (e == hipSuccess) e = upload(dst, src, n)
+    UP(&p->d_params, params.data(), params.size());
+    UP(&p->d_knots, knots.data(), knots.size());
+    UP(&p->d_koff, koff.data(), koff.size());
+    UP(&p->d_knotn, kn.data(), kn.size());
+    UP(&p->d_set, inst_set, (size_t)n_inst);
+    UP(&p->d_k, inst_k, (size_t)n_inst);
+    UP(&p->d_t0, inst_t0, (size_t)n_inst);
+    UP(&p->d_order, order.data(), (size_t)n_inst);
+    UP(&p->d_saveat, saveat, (size_t)n_save);
+#undef UP
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_out, sizeof(double) * std::max<size_t>((size_t)n_inst * m->n, 1));
+    if (e == hipSuccess && n_save > 0)
+        e = hipMalloc((void**)&p->d_save, sizeof(double) * (size_t)n_inst * n_save * m->n);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_ret, sizeof(int) * std::max(n_inst, 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, sizeof(long long) * 6 * std::max(n_inst, 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_pk, sizeof(double) * std::max(n_inst, 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_queue, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_ws, sizeof(double) * (size_t)m->nw * lanes);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&p->ev1);
+    if (e != hipSuccess) {
+        sbz_set_error("sbz_plan_create: %s", hipGetErrorString(e));
+        plan_free(p);
+        delete p;
+        return SBZ_ERR_HIP;
+    }
+    *out = p;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
+    sbz_clear_error();
+    if (!p) return SBZ_ERR_ARG;
+    sbz_model* m = p->m;
+    if (p->n_inst == 0) {
+        if (kernel_ms) *kernel_ms = 0.0f;
+        return SBZ_OK;
+    }
+    HIPCHK(hipMemsetAsync(p->d_queue, 0, sizeof(unsigned int), m->stream));
+    SbzSolveArgsHost a;
+    a.params = p->d_params;
+    a.knots = p->d_knots;
+    a.knot_off = p->d_koff;
+    a.knot_n = p->d_knotn;
+    a.inst_set = p->d_set;
+    a.inst_k = p->d_k;
+    a.inst_t0 = p->d_t0;
+    a.order = p->d_order;
+    a.n_inst = p->n_inst;
+    a.rtol = p->rtol;
+    a.atol = p->atol;
+    a.maxiters = p->maxiters;
+    a.saveat = p->n_save ? p->d_saveat : nullptr;
+    a.n_save = p->n_save;
+    a.out_u = p->d_out;
+    a.out_save = p->d_save;
+    a.retcode = p->d_ret;
+    a.stats = p->d_stats;
+    a.queue = p->d_queue;
+    a.ws = p->d_ws;
+    a.S = p->lanes;
+    void* args[] = {&a};
+    unsigned grid = (unsigned)(p->lanes / p->block);
+    HIPCHK(hipEventRecord(p->ev0, m->stream));
+    HIPCHK(hipModuleLaunchKernel(m->k_solve, grid, 1, 1, (unsigned)p->block, 1, 1, 0, m->stream, args, nullptr));
+    HIPCHK(hipEventRecord(p->ev1, m->stream));
+    HIPCHK(hipEventSynchronize(p->ev1));
+    HIPCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));
+    if (kernel_ms) *kernel_ms = p->last_ms;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_pk(sbz_plan* p, double* pk) {
+    sbz_clear_error();
+    if (!p || !pk) return SBZ_ERR_ARG;
+    if (p->n_inst == 0) return SBZ_OK;
+    sbz_model* m = p->m;
+    const double* params = p->d_params;
+    const double* knots = p->d_knots;
+    const long long* koff = p->d_koff;
+    const int* kn = p->d_knotn;
+    const int* set = p->d_set;
+    const double* kk = p->d_k;
+    const double* uf = p->d_out;
+    int n = p->n_inst;
+    double* dpk = p->d_pk;
+    void* args[] = {&params, &knots, &koff, &kn, &set, &kk, &uf, &n, &dpk};
+    unsigned grid = (unsigned)((n + 255) / 256);
+    HIPCHK(hipModuleLaunchKernel(m->k_pk, grid, 1, 1, 256, 1, 1, 0, m->stream, args, nullptr));
+    HIPCHK(hipMemcpyAsync(pk, p->d_pk, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_fetch(sbz_plan* p, double* out_u, double* out_save, int* retcode, long long* stats) {
+    sbz_clear_error();
+    if (!p) return SBZ_ERR_ARG;
+    sbz_model* m = p->m;
+    size_t n = (size_t)p->n_inst;
+    if (n == 0) return SBZ_OK;
+    if (out_u) HIPCHK(hipMemcpyAsync(out_u, p->d_out, sizeof(double) * n * m->n, hipMemcpyDeviceToHost, m->stream));
+    if (out_save && p->n_save)
+        HIPCHK(hipMemcpyAsync(out_save, p->d_save, sizeof(double) * n * p->n_save * m->n, hipMemcpyDeviceToHost,
+                              m->stream));
+    if (retcode) HIPCHK(hipMemcpyAsync(retcode, p->d_ret, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
+    if (stats) HIPCHK(hipMemcpyAsync(stats, p->d_stats, sizeof(long long) * 6 * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_destroy(sbz_plan* p) {
+    if (!p) return SBZ_OK;
+    plan_free(p);
+    delete p;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_solve_batch(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0,
+                               int n_inst, const sbz_opts* opts, const double* saveat, int n_save, double* out_u,
+                               double* out_save, int* retcode, long long* stats) {
+    sbz_plan* p = nullptr;
+    int st = sbz_plan_create(m, inst_set, inst_k, inst_t0, n_inst, opts, saveat, n_save, &p);
+    if (st != SBZ_OK) return st;
+    st = sbz_plan_run(p, nullptr);
+    if (st == SBZ_OK) st = sbz_plan_fetch(p, out_u, out_save, retcode, stats);
+    sbz_plan_destroy(p);
+    return st;
+}

@@ -9,7 +9,7 @@
 // spline pass (src/utils.jl:184-236; paper/aa57450-25.tex:463-466).
 #pragma once
 #include "sbz_dual.h"
-#include "sbz_params.h"
+#include "../../include/sbz_params.h"
 
 namespace sbzc {
 // CODATA 2018 (src/constants.jl:3-10) and astronomical units (:11-19).

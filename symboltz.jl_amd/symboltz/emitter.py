@@ -1,0 +1,560 @@
+"""HIP code emitter for the perturbation system of a model.
+
+The reference builds its perturbation ODE symbolically (ModelingToolkit) and generates `f!`,
+the analytic sparse Jacobian `jac!` and the initialization system (src/solve.jl:215-227,
+src/utils.jl:184-236).  This module does the MI355X counterpart: it states the model's
+perturbation equations symbolically (sympy), in terms of the per-mode unknowns u, the wavenumber
+k and a small vector B(τ) of background quantities that the device recomputes from the splined
+background unknowns, and emits straight-line device code for
+
+  sbz_f      f(τ, u)                                   (the RHS, with common subexpressions)
+  sbz_w      W = I/(γh) - J(τ) on the filled sparse pattern of a static LU
+  sbz_lu     the numeric LU of W without pivoting in a min-degree elimination order
+  sbz_solve  the forward/back substitution with that LU
+  sbz_dfdt   ∂f/∂τ = Σ_b (∂f/∂B_b) dB_b/dτ + explicit 1/τ terms
+  sbz_ic     adiabatic initial conditions
+  sbz_dm     the gauge-invariant matter overdensity Δ_m used by P(k)
+
+together with exact flop counts of each routine (add, sub, mul count 1, fma 2; divisions and
+special functions are counted separately).
+
+Equations (component code of the reference): gravity.jl:38-39; photons.jl:30-42,47-53,56-67;
+neutrinos.jl:23-26,33-39 (massless) and :109-139 (massive); generic_species.jl:46-57;
+baryons via cosmologies.jl:95; dark_energy.jl:50-64; couplings cosmologies.jl:85-97;
+P(k) overdensity fourier.jl:39-52.
+"""
+import hashlib
+import re
+from pathlib import Path
+
+import sympy as sp
+from sympy.printing.c import C99CodePrinter
+
+CSRC = Path(__file__).resolve().parent.parent / "csrc"
+
+# Background vector layout B[] (device: sbz_pt_background in csrc/sbz_kernel.h)
+B_NAMES = ["tau", "a", "Hc", "kd", "cs2", "rg", "rn", "rc", "rb", "rh", "ph", "Irho", "IP", "rx", "wx", "wdx"] + \
+          [f"E{i}" for i in range(8)]
+NB = len(B_NAMES)
+BIDX = {n: i for i, n in enumerate(B_NAMES)}
+
+# Per-set constants vector C[] (device: computed once per mode from the parameter vector)
+C_NAMES = [f"x{i}" for i in range(8)] + [f"w{i}" for i in range(8)] + [f"dl{i}" for i in range(8)] + ["cs2X"]
+CIDX = {n: i for i, n in enumerate(C_NAMES)}
+
+
+def layout(M):
+    """Index layout of the unknowns (identical to the oracle's and to include/sbz.h)."""
+    L = {}
+    i = 0
+    L["Phi"] = i; i += 1
+    L["F"] = i; i += M.lmax_g + 1
+    L["G"] = i; i += M.lmax_g + 1
+    L["N"] = i
+    if M.has_nu:
+        i += M.lmax_nu + 1
+    L["H"] = i
+    if M.has_h:
+        i += M.nx * (M.lmax_h + 1)
+    L["C"] = i; i += 2
+    L["B"] = i; i += 2
+    L["X"] = i
+    if M.de == "w0wa":
+        i += 2
+    L["n"] = i
+    return L
+
+
+class System:
+    """The symbolic perturbation system of one model."""
+
+    def __init__(self, M):
+        self.M = M
+        self.L = layout(M)
+        n = self.L["n"]
+        self.n = n
+        self.u = sp.symbols(f"u0:{n}", real=True)
+        self.B = {nm: sp.Symbol(f"B_{nm}", real=True) for nm in B_NAMES}
+        self.dB = {nm: sp.Symbol(f"dB_{nm}", real=True) for nm in B_NAMES}
+        self.C = {nm: sp.Symbol(f"C_{nm}", real=True) for nm in C_NAMES}
+        self.k = sp.Symbol("k", positive=True)
+        self.f = self._rhs()
+
+    # -- equations ---------------------------------------------------------------------------
+    def _rhs(self):
+        M, L, u, B, C, k = self.M, self.L, self.u, self.B, self.C, self.k
+        R = sp.Rational
+        pi = sp.pi
+        n = self.n
+        f = [None] * n
+        tau, a, H, kd = B["tau"], B["a"], B["Hc"], B["kd"]
+        F = [u[L["F"] + l] for l in range(M.lmax_g + 1)]
+        G = [u[L["G"] + l] for l in range(M.lmax_g + 1)]
+        Phi = u[L["Phi"]]
+        dc, thc, db, thb = u[L["C"]], u[L["C"] + 1], u[L["B"]], u[L["B"] + 1]
+        # δρ and Π = Σ (1 + w) ρ σ                                        cosmologies.jl:92-94
+        drho = F[0] * B["rg"] + dc * B["rc"] + db * B["rb"]
+        Pi = R(4, 3) * B["rg"] * F[2] / 2
+        if M.has_nu:
+            N = [u[L["N"] + l] for l in range(M.lmax_nu + 1)]
+            drho += N[0] * B["rn"]
+            Pi += R(4, 3) * B["rn"] * N[2] / 2
+        if M.has_h:
+            Lh = M.lmax_h
+            psi = [[u[L["H"] + i * (Lh + 1) + l] for l in range(Lh + 1)] for i in range(M.nx)]
+            E = [B[f"E{i}"] for i in range(M.nx)]
+            xs = [C[f"x{i}"] for i in range(M.nx)]
+            ws = [C[f"w{i}"] for i in range(M.nx)]
+            dls = [C[f"dl{i}"] for i in range(M.nx)]
+            Idr = sum(ws[i] * E[i] * psi[i][0] for i in range(M.nx))         # neutrinos.jl:116
+            sig_h = R(2, 3) * sum(ws[i] * xs[i] ** 2 / E[i] * psi[i][2] for i in range(M.nx)) / \
+                (B["Irho"] + B["IP"] / 3)                                    # neutrinos.jl:121
+            drho += Idr / B["Irho"] * B["rh"]
+            Pi += (B["rh"] + B["ph"]) * sig_h
+        if M.de == "w0wa":
+            dX, thX = u[L["X"]], u[L["X"] + 1]
+            drho += dX * B["rx"]
+        Psi = Phi - 12 * pi * a**2 * Pi / k**2                               # gravity.jl:39
+        dPhi = -R(4, 3) * pi * a**2 / H * drho - k**2 / (3 * H) * Phi - H * Psi  # gravity.jl:38
+        f[L["Phi"]] = dPhi
+        # photons                                                           photons.jl:30-33,56-59
+        Lg = M.lmax_g
+        thg = 3 * k * F[1] / 4
+        Pig = F[2] + G[0] + G[2]
+        iF, iG = L["F"], L["G"]
+        f[iF + 0] = -k * F[1] + 4 * dPhi
+        f[iF + 1] = k / 3 * (F[0] - 2 * F[2] + 4 * Psi) - R(4, 3) * kd / k * (thb - thg)
+        for l in range(2, Lg):
+            f[iF + l] = k / (2 * l + 1) * (l * F[l - 1] - (l + 1) * F[l + 1]) + kd * (F[l] - (Pig / 10 if l == 2 else 0))
+        f[iF + Lg] = k * F[Lg - 1] - (Lg + 1) / tau * F[Lg] + kd * F[Lg]
+        f[iG + 0] = k * (-G[1]) + kd * (G[0] - Pig / 2)
+        f[iG + 1] = k / 3 * (G[0] - 2 * G[2]) + kd * G[1]
+        for l in range(2, Lg):
+            f[iG + l] = k / (2 * l + 1) * (l * G[l - 1] - (l + 1) * G[l + 1]) + kd * (G[l] - (Pig / 10 if l == 2 else 0))
+        f[iG + Lg] = k * G[Lg - 1] - (Lg + 1) / tau * G[Lg] + kd * G[Lg]
+        # massless neutrinos                                                neutrinos.jl:23-26
+        if M.has_nu:
+            Ln, iN = M.lmax_nu, L["N"]
+            f[iN + 0] = -k * N[1] + 4 * dPhi
+            f[iN + 1] = k / 3 * (N[0] - 2 * N[2] + 4 * Psi)
+            for l in range(2, Ln):
+                f[iN + l] = k / (2 * l + 1) * (l * N[l - 1] - (l + 1) * N[l + 1])
+            f[iN + Ln] = k * N[Ln - 1] - (Ln + 1) / tau * N[Ln]
+        # massive neutrinos                                                 neutrinos.jl:128-132
+        if M.has_h:
+            for i in range(M.nx):
+                base = L["H"] + i * (Lh + 1)
+                q = xs[i] / E[i]
+                ps = psi[i]
+                f[base + 0] = -k * q * ps[1] - dPhi * dls[i]
+                f[base + 1] = k / 3 * q * (ps[0] - 2 * ps[2]) - k / 3 * E[i] / xs[i] * Psi * dls[i]
+                for l in range(2, Lh):
+                    f[base + l] = k / (2 * l + 1) * q * (l * ps[l - 1] - (l + 1) * ps[l + 1])
+                f[base + Lh] = k / (2 * Lh + 1) * q * (Lh * ps[Lh - 1] - (Lh + 1) * (
+                    (2 * Lh + 1) * E[i] / xs[i] * ps[Lh] / (k * tau) - ps[Lh - 1]))
+        # CDM (w = c_s² = 0) and baryons                                    generic_species.jl:46-47
+        f[L["C"]] = -(thc - 3 * dPhi)
+        f[L["C"] + 1] = -H * thc + k**2 * Psi
+        f[L["B"]] = -(thb - 3 * dPhi) - 3 * H * B["cs2"] * db
+        thint = -kd * 4 * B["rg"] / (3 * B["rb"]) * (thg - thb)              # cosmologies.jl:95
+        f[L["B"] + 1] = -H * thb + B["cs2"] * k**2 * db + k**2 * Psi + thint
+        # w0wa fluid                                                        dark_energy.jl:50-58
+        if M.de == "w0wa":
+            w, cs2 = B["wx"], C["cs2X"]
+            ca2 = w - B["wdx"] / (3 * H * (1 + w))
+            f[L["X"]] = -(1 + w) * (thX - 3 * dPhi) - 3 * H * (cs2 - w) * dX - 9 * (H / k) ** 2 * (1 + w) * (cs2 - ca2) * thX
+            f[L["X"] + 1] = -H * (1 - 3 * cs2) * thX + cs2 / (1 + w) * k**2 * dX + k**2 * Psi
+        self.Psi, self.dPhi = Psi, dPhi
+        return f
+
+    # -- derived structure ------------------------------------------------------------------
+    def jacobian_entries(self):
+        """Nonzero entries {(i, j): expr} of J = ∂f/∂u."""
+        J = {}
+        for i, fi in enumerate(self.f):
+            fi = sp.expand(fi) if False else fi
+            for j in range(self.n):
+                if fi.has(self.u[j]):
+                    d = sp.diff(fi, self.u[j])
+                    if d != 0:
+                        J[(i, j)] = d
+        return J
+
+    def ic(self):
+        """Adiabatic initial conditions as {index: expr} (zero elsewhere), Φ from the constraint."""
+        M, L, B, C, k = self.M, self.L, self.B, self.C, self.k
+        R = sp.Rational
+        tau, a, kd = B["tau"], B["a"], B["kd"]
+        fnu = (B["rn"] + B["rh"]) / (B["rg"] + B["rn"] + B["rh"])           # cosmologies.jl:89
+        Psi = 20 * R(1, 2) / (15 + 4 * fnu)                                  # cosmologies.jl:75
+        v = {}
+        iF, iG = L["F"], L["G"]
+        F1 = R(2, 3) * k * tau * Psi
+        F2 = -R(8, 15) * k / kd * F1
+        v[iF + 0] = -2 * Psi
+        v[iF + 1] = F1
+        v[iF + 2] = F2
+        v[iF + 3] = -R(3, 7) * k / kd * F2
+        G2 = R(1, 16) * F2
+        v[iG + 0] = R(5, 16) * F2
+        v[iG + 1] = -R(1, 16) * k / kd * F2
+        v[iG + 2] = G2
+        v[iG + 3] = -R(3, 7) * k / kd * G2
+        Pi = R(4, 3) * B["rg"] * F2 / 2
+        if M.has_nu:
+            iN = L["N"]
+            N2 = 2 * (R(1, 15) * (k * tau) ** 2 * Psi)
+            v[iN + 0] = -2 * Psi
+            v[iN + 1] = R(4, 3) / k * (R(1, 2) * k**2 * tau * Psi)
+            v[iN + 2] = N2
+            v[iN + 3] = R(3, 7) * k * tau * N2
+            Pi += R(4, 3) * B["rn"] * N2 / 2
+        if M.has_h:
+            num = 0
+            for i in range(M.nx):
+                base = L["H"] + i * (M.lmax_h + 1)
+                x, dl, E = C[f"x{i}"], C[f"dl{i}"], B[f"E{i}"]
+                v[base + 0] = -R(1, 4) * (-2 * Psi) * dl
+                v[base + 1] = -R(1, 3) * E / x * (R(1, 2) * k * tau * Psi) * dl
+                p2 = -R(1, 2) * (R(1, 15) * (k * tau) ** 2 * Psi) * dl
+                v[base + 2] = p2
+                num += C[f"w{i}"] * x**2 / E * p2
+            Pi += (B["rh"] + B["ph"]) * R(2, 3) * num / (B["Irho"] + B["IP"] / 3)
+        v[L["C"]] = -R(3, 2) * Psi
+        v[L["C"] + 1] = R(1, 2) * k**2 * tau * Psi
+        v[L["B"]] = -R(3, 2) * Psi
+        v[L["B"] + 1] = R(1, 2) * k**2 * tau * Psi
+        if M.de == "w0wa":
+            v[L["X"]] = -R(3, 2) * (1 + B["wx"]) * Psi
+            v[L["X"] + 1] = R(1, 2) * k**2 * tau * Psi
+        v[L["Phi"]] = Psi + 12 * sp.pi * a**2 * Pi / k**2                   # gravity.jl:39
+        return v
+
+    def delta_m(self):
+        """Δ_m = Σ ρ_s Δ_s / Σ ρ_s over c, b, h; Δ_s = δ_s + 3ℋ(1+w_s)θ_s/k² (fourier.jl:39-52)."""
+        M, L, u, B, C, k = self.M, self.L, self.u, self.B, self.C, self.k
+        H = B["Hc"]
+        Dc = u[L["C"]] + 3 * H * u[L["C"] + 1] / k**2
+        Db = u[L["B"]] + 3 * H * u[L["B"] + 1] / k**2
+        num = B["rc"] * Dc + B["rb"] * Db
+        den = B["rc"] + B["rb"]
+        if M.has_h:
+            Lh = M.lmax_h
+            Idr = sum(C[f"w{i}"] * B[f"E{i}"] * u[L["H"] + i * (Lh + 1)] for i in range(M.nx))
+            uh = sum(C[f"w{i}"] * C[f"x{i}"] * u[L["H"] + i * (Lh + 1) + 1] for i in range(M.nx)) / \
+                (B["Irho"] + B["IP"] / 3)
+            dh = Idr / B["Irho"]
+            wh = B["ph"] / B["rh"]
+            Dh = dh + 3 * H * (1 + wh) * (uh * k) / k**2
+            num += B["rh"] * Dh
+            den += B["rh"]
+        return num / den
+
+
+# ------------------------------------------------------------------------------- LU structure
+def min_degree_order(n, pattern):
+    """Greedy minimum-degree elimination order on the symmetrised pattern (fill-reducing)."""
+    adj = [set() for _ in range(n)]
+    for (i, j) in pattern:
+        if i != j:
+            adj[i].add(j)
+            adj[j].add(i)
+    alive = set(range(n))
+    order = []
+    while alive:
+        p = min(alive, key=lambda v: (len(adj[v]), v))
+        order.append(p)
+        nb = adj[p]
+        for a in nb:
+            adj[a].discard(p)
+            adj[a] |= (nb - {a})
+        alive.discard(p)
+        adj[p] = set()
+    return order
+
+
+def symbolic_lu(n, pattern, order):
+    """Fill pattern of the LU of a matrix with `pattern`, eliminated in `order` without pivoting.
+    Returns (filled set of (i, j), lower dict p -> [rows], upper dict p -> [cols])."""
+    pos = {v: r for r, v in enumerate(order)}
+    rows = {i: set() for i in range(n)}
+    for (i, j) in pattern:
+        rows[i].add(j)
+    for i in range(n):
+        rows[i].add(i)
+    filled = set()
+    lower, upper = {}, {}
+    cols = {j: set() for j in range(n)}
+    for i in range(n):
+        for j in rows[i]:
+            cols[j].add(i)
+    for p in order:
+        lo = sorted([i for i in cols[p] if pos[i] > pos[p]], key=lambda i: pos[i])
+        up = sorted([j for j in rows[p] if pos[j] > pos[p]], key=lambda j: pos[j])
+        lower[p], upper[p] = lo, up
+        for i in lo:
+            for j in up:
+                if j not in rows[i]:
+                    rows[i].add(j)
+                    cols[j].add(i)
+    for i in range(n):
+        for j in rows[i]:
+            filled.add((i, j))
+    return filled, lower, upper
+
+
+# ------------------------------------------------------------------------------- printing
+class _Printer(C99CodePrinter):
+    def _print_Pow(self, expr):
+        b, e = expr.as_base_exp()
+        if e.is_Integer:
+            ei = int(e)
+            bs = self.parenthesize(b, 100)
+            if ei == 1:
+                return bs
+            if ei == 2:
+                return f"({bs}*{bs})"
+            if ei == 3:
+                return f"({bs}*{bs}*{bs})"
+            if ei == -1:
+                return f"(1.0/{bs})"
+            if ei == -2:
+                return f"(1.0/({bs}*{bs}))"
+            if ei == -3:
+                return f"(1.0/({bs}*{bs}*{bs}))"
+        if e == sp.Rational(1, 2):
+            return f"sqrt({self._print(b)})"
+        return f"pow({self._print(b)}, {self._print(e)})"
+
+    def _print_Rational(self, expr):
+        return repr(float(expr.p) / float(expr.q))
+
+    def _print_Integer(self, expr):
+        return repr(float(int(expr)))
+
+    def _print_Float(self, expr):
+        return repr(float(expr))
+
+    def _print_Pi(self, expr):
+        return "3.141592653589793"
+
+    def _print_Symbol(self, expr):
+        nm = expr.name
+        if nm.startswith("B_"):
+            return f"B[{BIDX[nm[2:]]}]"
+        if nm.startswith("dB_"):
+            return f"dB[{BIDX[nm[3:]]}]"
+        if nm.startswith("C_"):
+            return f"Cs[{CIDX[nm[2:]]}]"
+        if re.fullmatch(r"u\d+", nm):
+            return f"u[{nm[1:]}]"
+        return nm
+
+
+_PR = _Printer()
+
+
+def cprint(e):
+    return _PR.doprint(e)
+
+
+def count_ops(exprs):
+    """Flop count of expressions: add/sub/mul = 1, division = 1 (reported also separately)."""
+    flops = 0
+    divs = 0
+    special = 0
+    for e in exprs:
+        for node in sp.preorder_traversal(e):
+            if isinstance(node, sp.Add):
+                flops += len(node.args) - 1
+            elif isinstance(node, sp.Mul):
+                nneg = sum(1 for a in node.args if a == -1)
+                flops += len(node.args) - 1 - nneg
+            elif isinstance(node, sp.Pow):
+                b, ex = node.as_base_exp()
+                if ex.is_Integer:
+                    ei = abs(int(ex))
+                    flops += max(ei - 1, 0)
+                    if int(ex) < 0:
+                        divs += 1
+                        flops += 1
+                else:
+                    special += 1
+            elif isinstance(node, sp.Function):
+                special += 1
+    return flops, divs, special
+
+
+def _emit_block(assign, tmp_prefix, opt=True):
+    """CSE a list of (lhs_code, expr) and return C lines plus the op count."""
+    exprs = [e for _, e in assign]
+    if opt:
+        repl, red = sp.cse(exprs, symbols=sp.numbered_symbols(tmp_prefix), optimizations="basic")
+    else:
+        repl, red = [], exprs
+    lines = []
+    for s, e in repl:
+        lines.append(f"    const double {s} = {cprint(e)};")
+    for (lhs, _), e in zip(assign, red):
+        lines.append(f"    {lhs} = {cprint(e)};")
+    ops = count_ops([e for _, e in repl] + list(red))
+    return lines, ops
+
+
+class Emitted:
+    """Emitted device source of one model, with its structure metadata."""
+
+    def __init__(self, M):
+        self.M = M
+        sysm = System(M)
+        self.sys = sysm
+        n = sysm.n
+        self.n = n
+        J = sysm.jacobian_entries()
+        self.J = J
+        pattern = set(J.keys())
+        order = min_degree_order(n, pattern | {(j, i) for (i, j) in pattern})
+        filled, lower, upper = symbolic_lu(n, pattern, order)
+        self.order, self.lower, self.upper = order, lower, upper
+        slots = sorted(filled, key=lambda ij: (ij[0], ij[1]))
+        self.slot = {ij: s for s, ij in enumerate(slots)}
+        self.nnz_J = len(J)
+        self.nnz_LU = len(slots)
+        self.ops = {}
+        self.src = self._emit()
+        self.hash = hashlib.sha1(self.src.encode()).hexdigest()[:16]
+
+    def _emit(self):
+        S, sysm, n = self.slot, self.sys, self.n
+        out = []
+        out.append(f"// Emitted by symboltz.emitter for model {self.M}")
+        out.append(f"#define SBZ_N {n}")
+        out.append(f"#define SBZ_NNZ_LU {self.nnz_LU}")
+        out.append(f"#define SBZ_NX {self.M.nx if self.M.has_h else 0}")
+        out.append(f"#define SBZ_HAS_W0WA {1 if self.M.de == 'w0wa' else 0}")
+        # f
+        lines, ops = _emit_block([(f"du[{i}]", e) for i, e in enumerate(sysm.f)], "tf")
+        self.ops["f"] = ops
+        out.append("template <class UI, class UO>")
+        out.append("SBZ_DEV void sbz_f(const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u, UO du) {")
+        out += lines
+        out.append("}")
+        # W = I/(γh) - J on the filled pattern
+        assign = []
+        for (i, j), s in sorted(S.items(), key=lambda kv: kv[1]):
+            e = -self.J.get((i, j), sp.Integer(0))
+            if i == j:
+                e = sp.Symbol("ig") + e
+            assign.append((f"W[{s}]", e))
+        lines, ops = _emit_block(assign, "tw")
+        self.ops["w"] = ops
+        out.append("template <class WT>")
+        out.append("SBZ_DEV void sbz_w(const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, WT W) {")
+        out += lines
+        out.append("}")
+        # LU (no pivoting): diagonal slots keep the reciprocal of the pivot
+        lu = ["template <class WT>", "SBZ_DEV bool sbz_lu(WT W) {", "    bool ok = true;"]
+        flops_lu = 0
+        for p in self.order:
+            dp = S[(p, p)]
+            lu.append(f"    {{ const double piv = W[{dp}]; ok = ok && (piv != 0.0) && (piv == piv) && (fabs(piv) < 1e300);")
+            lu.append(f"      const double rp = 1.0 / piv; W[{dp}] = rp;")
+            flops_lu += 1
+            for i in self.lower[p]:
+                li = S[(i, p)]
+                lu.append(f"      {{ const double l = W[{li}] * rp; W[{li}] = l;")
+                flops_lu += 1
+                for j in self.upper[p]:
+                    lu.append(f"        W[{S[(i, j)]}] -= l * W[{S[(p, j)]}];")
+                    flops_lu += 2
+                lu.append("      }")
+            lu.append("    }")
+        lu.append("    return ok;")
+        lu.append("}")
+        self.ops["lu"] = (flops_lu, n, 0)
+        out += lu
+        # solve
+        so = ["template <class WT, class BT>", "SBZ_DEV void sbz_solve(WT W, BT b) {"]
+        flops_s = 0
+        for p in self.order:
+            for i in self.lower[p]:
+                so.append(f"    b[{i}] -= W[{S[(i, p)]}] * b[{p}];")
+                flops_s += 2
+        for p in reversed(self.order):
+            terms = " ".join(f"- W[{S[(p, j)]}] * b[{j}]" for j in self.upper[p])
+            flops_s += 2 * len(self.upper[p]) + 1
+            so.append(f"    b[{p}] = (b[{p}] {terms}) * W[{S[(p, p)]}];")
+        so.append("}")
+        self.ops["solve"] = (flops_s, 0, 0)
+        out += so
+        # dfdt = Σ_b ∂f/∂B_b dB_b (B_tau has dB = 1 set by the caller)
+        dassign = []
+        for i, fi in enumerate(sysm.f):
+            e = sp.Integer(0)
+            for nm in B_NAMES:
+                bs = sysm.B[nm]
+                if fi.has(bs):
+                    e += sp.diff(fi, bs) * sysm.dB[nm]
+            dassign.append((f"out[{i}]", e))
+        lines, ops = _emit_block(dassign, "td")
+        self.ops["dfdt"] = ops
+        out.append("template <class UI, class UO>")
+        out.append("SBZ_DEV void sbz_dfdt(const double* __restrict__ B, const double* __restrict__ dB, const double* __restrict__ Cs, double k, UI u, UO out) {")
+        out += lines
+        out.append("}")
+        # initial conditions
+        ic = sysm.ic()
+        lines, _ = _emit_block([(f"u[{i}]", ic.get(i, sp.Integer(0))) for i in range(n)], "ti")
+        out.append("template <class UO>")
+        out.append("SBZ_DEV void sbz_ic(const double* __restrict__ B, const double* __restrict__ Cs, double k, UO u) {")
+        out += lines
+        out.append("}")
+        # Δ_m
+        lines, _ = _emit_block([("double r", sysm.delta_m())], "tm")
+        out.append("template <class UI>")
+        out.append("SBZ_DEV double sbz_dm(const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u) {")
+        out += [ln.replace("    double r =", "    const double r =") for ln in lines]
+        out.append("    return r;")
+        out.append("}")
+        f_f, f_w, f_lu, f_s, f_d = (self.ops["f"][0] + self.ops["f"][1], self.ops["w"][0] + self.ops["w"][1],
+                                    self.ops["lu"][0], self.ops["solve"][0],
+                                    self.ops["dfdt"][0] + self.ops["dfdt"][1])
+        # Rodas5P step: 8 f + W + LU + 8 solves + dfdt + stage combinations + error norm
+        lin = n * (2 * 28 + 2 * 28 + 2 * 8 + 4) + n * 6
+        self.flops_step = 8 * f_f + f_w + f_lu + 8 * f_s + f_d + lin
+        out.append(f"#define SBZ_FLOPS_F {f_f}")
+        out.append(f"#define SBZ_FLOPS_W {f_w}")
+        out.append(f"#define SBZ_FLOPS_LU {f_lu}")
+        out.append(f"#define SBZ_FLOPS_SOLVE {f_s}")
+        out.append(f"#define SBZ_FLOPS_DFDT {f_d}")
+        out.append(f"#define SBZ_FLOPS_STEP {self.flops_step}")
+        return "\n".join(out) + "\n"
+
+
+def expand_includes(path, seen=None):
+    """Inline local #include "..." directives (hiprtc gets one self-contained source)."""
+    path = Path(path)
+    seen = set() if seen is None else seen
+    out = []
+    for line in path.read_text().splitlines():
+        m = re.match(r'\s*#include\s+"([^"]+)"', line)
+        if m:
+            inc = (path.parent / m.group(1)).resolve()
+            if inc not in seen:
+                seen.add(inc)
+                out.append(expand_includes(inc, seen))
+            continue
+        if line.strip() == "#pragma once":
+            continue
+        out.append(line)
+    return "\n".join(out)
+
+
+def kernel_source(M):
+    """Full self-contained HIP source (emitted model + generic kernel) and its metadata."""
+    em = Emitted(M)
+    kern = expand_includes(CSRC / "sbz_kernel.hip")
+    src = kern.replace("// @@SBZ_MODEL@@", em.src)
+    em.full_src = src
+    em.full_hash = hashlib.sha1(src.encode()).hexdigest()[:16]
+    return em

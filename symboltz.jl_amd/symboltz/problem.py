@@ -1,0 +1,213 @@
+"""Host-side mirror of SymBoltz's problem/solve API for the perturbation stage.
+
+Names, argument meaning and error behaviour follow the reference:
+  CosmologyProblem(M, pars)            src/solve.jl:129-236 (compile step: here the emitted model)
+  remake / parameter_updater            src/solve.jl:248-307
+  solvebg(prob)                         src/solve.jl:427-435 (host C++ background solver)
+  solve(prob, ks; ptopts, ptivini)      src/solve.jl:380-402 → solvept :543-569 on the GPU
+  spectrum_matter(prob, ks)             src/observables/fourier.jl:79-101
+  spectrum_primordial(k, prob)          src/observables/fourier.jl:14-37
+  issuccess(sol)                        src/solve.jl:604-606
+Per-mode failures do not raise: they warn "Perturbation (mode k = …) solution failed with return
+code …" (solve.jl:557-564) and are visible in the retcodes.  Argument errors raise.
+
+Wavenumbers are dimensionless (units of H0/c) unless `k_unit="1/Mpc"` is given, as the
+reference converts Unitful wavenumbers with k_dimensionless (src/constants.jl:47-50).
+"""
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib
+from . import constants as C
+from .models import (P_AS, P_KAPPA0, P_KPIVOT, P_NS, P_TAU0, P_TAUREC, P_H, Model, make_params)
+
+RETCODES = _lib.RETCODE_NAMES
+
+
+class BackgroundSolution:
+    """Background solve: knots t, unknowns u (a, X_H+, X_He+, ΔT, _κ), derivatives du, parameters P."""
+
+    def __init__(self, P, t, u, du, n_rejected):
+        self.P, self.t, self.u, self.du, self.n_rejected = P, t, u, du, n_rejected
+        self.retcode = "Success"
+
+    @property
+    def tau0(self):
+        return float(self.P[P_TAU0])
+
+    @property
+    def kappa0(self):
+        return float(self.P[P_KAPPA0])
+
+    @property
+    def taurec(self):
+        return float(self.P[P_TAUREC])
+
+
+class CosmologyProblem:
+    """A model plus parameter values.  The perturbation system is emitted and compiled for the GPU
+    on first use (the reference compiles it in the constructor, src/solve.jl:215-227)."""
+
+    def __init__(self, M, pars, ivspan=(1e-6, 100.0)):
+        if not isinstance(M, Model):
+            raise TypeError("M must be a symboltz Model")
+        self.M = M
+        self.pars = dict(pars)
+        self.ivspan = ivspan
+        self.P = make_params(M, self.pars, tau_bg0=ivspan[0])
+        self._gpu = None
+
+    @property
+    def gpu(self):
+        if self._gpu is None:
+            self._gpu = _gpu_model(self.M)
+        return self._gpu
+
+    def __repr__(self):
+        return f"Cosmology problem for model {self.M.name}: perturbations {self.M.n_unknowns} unknowns, 5 splines"
+
+
+_gpu_models = {}
+
+
+def _gpu_model(M):
+    from .solver import GpuModel
+    key = M.key()
+    if key not in _gpu_models:
+        _gpu_models[key] = GpuModel(M)
+    return _gpu_models[key]
+
+
+def remake(prob, pars):
+    """New problem with some parameters replaced (src/solve.jl:248-264)."""
+    p = dict(prob.pars)
+    p.update(pars)
+    q = CosmologyProblem(prob.M, p, prob.ivspan)
+    q._gpu = prob._gpu
+    return q
+
+
+def parameter_updater(prob, names):
+    """Return θ ↦ remake(prob, dict(zip(names, θ))) (src/solve.jl:272-307)."""
+    names = list(names)
+
+    def updater(values):
+        if isinstance(values, dict):
+            values = [values[n] for n in names]
+        return remake(prob, dict(zip(names, values)))
+    return updater
+
+
+def solvebg(prob, reltol=1e-7, abstol=1e-7, max_knots=20000):
+    P, t, u, du, nrej = _lib.background_solve(prob.P, reltol, abstol, max_knots)
+    return BackgroundSolution(P, t, u, du, nrej)
+
+
+class CosmologySolution:
+    def __init__(self, prob, bg, ks, pts):
+        self.prob, self.bg, self.ks, self.pts = prob, bg, ks, pts
+
+    def __repr__(self):
+        s = f"Cosmology solution for model {self.prob.M.name}\n  Background: return code Success; {len(self.bg.t)} points"
+        if self.pts is not None:
+            codes = sorted({RETCODES[int(r)] for r in self.pts["retcode"]})
+            n = self.pts["stats"][:, 0]
+            s += (f"\n  Perturbations: return codes {', '.join(codes)}; solved with Rodas5P (MI355X); "
+                  f"{n.min()}-{n.max()} steps; x{len(self.ks)} k ∈ [{self.ks.min()}, {self.ks.max()}] H₀/c")
+        return s
+
+
+def _check_ks(ks):
+    ks = np.asarray(ks)
+    if ks.dtype.kind not in "fiu":
+        raise TypeError(f"wavenumbers must be real Float64 numbers, got dtype {ks.dtype} "
+                        "(dual numbers and other element types are not supported by the GPU solver)")
+    return np.ascontiguousarray(ks, dtype=np.float64).ravel()
+
+
+def _ivini(ptivini, ks, t0, t1):
+    if callable(ptivini):
+        ti = np.array([ptivini(k) for k in ks], dtype=np.float64)
+    else:
+        ti = np.full(len(ks), float(ptivini))
+    return np.clip(ti, t0, t1)  # clamp to the background timespan (solve.jl:527)
+
+
+def solve(prob, ks=None, bgopts=None, ptopts=None, ptivini=-math.inf, saveat=None, k_unit=None, lanes=0,
+          verbose=False, timing=None):
+    """Background on the host, perturbation modes on the GPU (one persistent launch)."""
+    bgopts = dict(reltol=1e-7, abstol=1e-7) | (bgopts or {})
+    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
+    bg = solvebg(prob, bgopts["reltol"], bgopts["abstol"])
+    if ks is None or len(np.atleast_1d(ks)) == 0:
+        return CosmologySolution(prob, bg, None, None)
+    ks = _check_ks(ks)
+    if k_unit == "1/Mpc":
+        ks = C.k_dimensionless(ks, prob.P[P_H])
+    elif k_unit is not None:
+        raise ValueError("k_unit must be None (H0/c) or '1/Mpc'")
+    gm = prob.gpu
+    gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
+    t0 = _ivini(ptivini, ks, bg.t[0], bg.t[-1])
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, rtol=ptopts["reltol"], atol=ptopts["abstol"],
+                   maxiters=ptopts["maxiters"], saveat=saveat, lanes=lanes)
+    ms = plan.run()
+    if timing is not None:
+        timing["kernel_ms"] = ms
+    res = plan.fetch(save=saveat is not None)
+    res["pk"] = plan.pk()
+    plan.close()
+    for i, r in enumerate(res["retcode"]):
+        if r != 0:
+            warnings.warn(f"Perturbation (mode k = {ks[i]}) solution failed with return code {RETCODES[int(r)]}.")
+        elif verbose:
+            print(f"\rSolved perturbations for wavenumber k = {ks[i]}", end="")
+    return CosmologySolution(prob, bg, ks, res)
+
+
+def issuccess(sol):
+    return sol.bg.retcode == "Success" and (sol.pts is None or bool(np.all(sol.pts["retcode"] == 0)))
+
+
+def spectrum_primordial(k, prob):
+    """P0(k) = 2π² As (k/k_pivot)^(ns-1) / k³ for dimensionless k (fourier.jl:14-23)."""
+    k = np.asarray(k, dtype=np.float64)
+    P = prob.P
+    return 2 * math.pi**2 * P[P_AS] / k**3 * (k / P[P_KPIVOT]) ** (P[P_NS] - 1)
+
+
+def spectrum_matter(prob, ks, ktini=1e-2, tauinimax=1e-4, **kw):
+    """P(k, τ0) of the total matter overdensity (fourier.jl:79-83 with kτini, τinimax)."""
+    sol = solve(prob, ks, ptivini=lambda k: min(ktini / k, tauinimax), **kw)
+    return sol.pts["pk"], sol
+
+
+def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=1e-4, lanes=0, timing=None):
+    """Many parameter sets × the same ks in ONE persistent launch (the batched form of
+    parameter_updater + solve, src/solve.jl:272-307 driven in a loop at test/runtests.jl:508-532).
+    Returns (P[n_sets][n_k], results, backgrounds)."""
+    ks = _check_ks(ks)
+    bgopts = dict(reltol=1e-7, abstol=1e-7) | (bgopts or {})
+    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
+    gm = _gpu_model(M)
+    bgs = []
+    inst_set, inst_k, inst_t0 = [], [], []
+    for s, pars in enumerate(par_sets):
+        prob = CosmologyProblem(M, pars)
+        bg = solvebg(prob, bgopts["reltol"], bgopts["abstol"])
+        gm.upload_background(s, bg.t, bg.u, bg.du, bg.P)
+        bgs.append(bg)
+        inst_set.append(np.full(len(ks), s, dtype=np.int32))
+        inst_k.append(ks)
+        inst_t0.append(np.clip(np.minimum(ktini / ks, tauinimax), bg.t[0], bg.t[-1]))
+    plan = gm.plan(np.concatenate(inst_set), np.concatenate(inst_k), np.concatenate(inst_t0),
+                   rtol=ptopts["reltol"], atol=ptopts["abstol"], maxiters=ptopts["maxiters"], lanes=lanes)
+    ms = plan.run()
+    if timing is not None:
+        timing["kernel_ms"] = ms
+    res = plan.fetch()
+    pk = plan.pk().reshape(len(par_sets), len(ks))
+    plan.close()
+    return pk, res, bgs

@@ -1,0 +1,169 @@
+"""The GPU perturbation solver behind the C ABI (include/sbz.h).
+
+`GpuModel(M)` owns a model handle: the emitted HIP source of the model's perturbation system,
+compiled (or loaded from the code-object cache) and bound to a HIP stream.  Background tables are
+uploaded per parameter set; `Plan` holds a batch of (set, k, τ_start) instances resident on the
+device, runs the persistent Rodas5P kernel and fetches final states, saved states, return codes,
+statistics and P(k).
+"""
+import ctypes
+import hashlib
+import json
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from .emitter import kernel_source
+from .build import KCACHE
+
+_src_cache = {}
+
+
+def model_source(M):
+    """Emitted source of a model, memoised in-process and on disk (emission costs seconds)."""
+    key = M.key()
+    if key in _src_cache:
+        return _src_cache[key]
+    kern = (Path(__file__).resolve().parent.parent / "csrc" / "sbz_kernel.hip").read_text()
+    emit = (Path(__file__).resolve().parent / "emitter.py").read_text()
+    tag = hashlib.sha1((repr(key) + kern + emit + _deps_text()).encode()).hexdigest()[:16]
+    meta = KCACHE / f"src_{tag}.json"
+    if meta.exists():
+        d = json.loads(meta.read_text())
+        em = _Stored(d)
+    else:
+        em = kernel_source(M)
+        KCACHE.mkdir(exist_ok=True)
+        meta.write_text(json.dumps(dict(full_src=em.full_src, full_hash=em.full_hash, n=em.n, nnz_LU=em.nnz_LU,
+                                        nnz_J=em.nnz_J, flops_step=em.flops_step, ops={k: list(v) for k, v in em.ops.items()})))
+    _src_cache[key] = em
+    return em
+
+
+def _deps_text():
+    base = Path(__file__).resolve().parent.parent / "csrc"
+    return "".join((base / f).read_text() for f in ("sbz_thermo.h", "sbz_dual.h", "sbz_rodas5p.h"))
+
+
+class _Stored:
+    def __init__(self, d):
+        self.__dict__.update(d)
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nnz_lu", ctypes.c_int), ("nw", ctypes.c_int), ("cache_path", ctypes.c_char_p)]
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiters", ctypes.c_longlong),
+                ("lanes", ctypes.c_int)]
+
+
+def _declare(L):
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    L.sbz_model_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(_Desc), ctypes.POINTER(vp)]
+    L.sbz_model_destroy.argtypes = [vp]
+    L.sbz_bg_upload.argtypes = [vp, ctypes.c_int, dp, dp, dp, ctypes.c_int, dp]
+    L.sbz_plan_create.argtypes = [vp, ip, dp, dp, ctypes.c_int, ctypes.POINTER(_Opts), dp, ctypes.c_int,
+                                  ctypes.POINTER(vp)]
+    L.sbz_plan_run.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.sbz_plan_fetch.argtypes = [vp, dp, dp, ip, llp]
+    L.sbz_plan_pk.argtypes = [vp, dp]
+    L.sbz_plan_destroy.argtypes = [vp]
+    L.sbz_compile.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+
+
+class GpuModel:
+    def __init__(self, M):
+        self.M = M
+        self.L = _lib.lib()
+        _declare(self.L)
+        em = model_source(M)
+        self.em = em
+        self.n = em.n
+        self.nw = 14 * em.n + em.nnz_LU
+        KCACHE.mkdir(exist_ok=True)
+        self.cache_path = KCACHE / f"{em.full_hash}.hsaco"
+        desc = _Desc(em.n, em.nnz_LU, self.nw, str(self.cache_path).encode())
+        h = ctypes.c_void_p()
+        _lib.check(self.L.sbz_model_create(em.full_src.encode(), ctypes.byref(desc), ctypes.byref(h)),
+                   "sbz_model_create")
+        self.h = h
+        self.n_sets = 0
+
+    def upload_background(self, set_index, t, u, du, P):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        du = np.ascontiguousarray(du, dtype=np.float64)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        _lib.check(self.L.sbz_bg_upload(self.h, set_index, _lib.ptr(t), _lib.ptr(u), _lib.ptr(du), len(t),
+                                        _lib.ptr(P)), "sbz_bg_upload")
+        self.n_sets = max(self.n_sets, set_index + 1)
+
+    def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0):
+        return Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbz_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    def __init__(self, gm, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes):
+        self.gm = gm
+        L = gm.L
+        self.set = np.ascontiguousarray(inst_set, dtype=np.int32)
+        self.k = np.ascontiguousarray(ks, dtype=np.float64)
+        self.t0 = np.ascontiguousarray(t0, dtype=np.float64)
+        self.n_inst = len(self.k)
+        self.saveat = np.ascontiguousarray(saveat if saveat is not None else np.zeros(0), dtype=np.float64)
+        self.n_save = len(self.saveat)
+        opts = _Opts(rtol, atol, int(maxiters), int(lanes))
+        h = ctypes.c_void_p()
+        _lib.check(L.sbz_plan_create(gm.h, _lib.ptr(self.set, ctypes.c_int), _lib.ptr(self.k), _lib.ptr(self.t0),
+                                     self.n_inst, ctypes.byref(opts), _lib.ptr(self.saveat), self.n_save,
+                                     ctypes.byref(h)), "sbz_plan_create")
+        self.h = h
+
+    def run(self):
+        ms = ctypes.c_float(0.0)
+        _lib.check(self.gm.L.sbz_plan_run(self.h, ctypes.byref(ms)), "sbz_plan_run")
+        return ms.value
+
+    def fetch(self, save=False):
+        n, m = self.n_inst, self.gm.n
+        u = np.zeros((n, m))
+        us = np.zeros((n, self.n_save, m)) if (save and self.n_save) else None
+        ret = np.zeros(n, dtype=np.int32)
+        st = np.zeros((n, 6), dtype=np.int64)
+        _lib.check(self.gm.L.sbz_plan_fetch(self.h, _lib.ptr(u), _lib.ptr(us) if us is not None else None,
+                                            _lib.ptr(ret, ctypes.c_int), _lib.ptr(st, ctypes.c_longlong)),
+                   "sbz_plan_fetch")
+        return dict(u=u, usave=us, retcode=ret, stats=st)
+
+    def pk(self):
+        out = np.zeros(self.n_inst)
+        _lib.check(self.gm.L.sbz_plan_pk(self.h, _lib.ptr(out)), "sbz_plan_pk")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.gm.L.sbz_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

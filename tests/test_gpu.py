@@ -1,0 +1,136 @@
+"""GPU parity: the HIP path through the C ABI against the oracle on the same inputs.
+
+The plain GPU path must match the CPU restatement of the same algorithm to ≤ 1e-4 relative in
+P(k) and in the perturbation variables (final states, compared relative to each mode's largest
+component); it is also held to the reference's CLASS fixture (norm-wise 1e-3,
+test/runtests.jl:872-876) and to its failure semantics (k = 0, k = NaN fail without raising,
+test/runtests.jl:31,360)."""
+import sys
+import warnings
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, Setup
+from symboltz import models
+from symboltz.problem import solve, solve_sets, spectrum_matter, CosmologyProblem, issuccess
+from symboltz.solver import GpuModel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_solve(S, ks, t0, **kw):
+    gm = GpuModel(S.M)
+    gm.upload_background(0, S.t, S.u, S.du, S.P)
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, **kw)
+    ms = plan.run()
+    r = plan.fetch()
+    r["pk"] = plan.pk()
+    r["ms"] = ms
+    plan.close()
+    return r
+
+
+def rel_state_err(a, b):
+    scale = np.maximum(np.abs(b).max(axis=1, keepdims=True), 1e-300)
+    return np.max(np.abs(a - b) / scale)
+
+
+def test_fixture_model_matches_oracle(fixture5):
+    S = fixture5
+    say(f"case: ΛCDM lmax=5, 24 modes, n={S.O.n}")
+    ks = 10 ** np.linspace(np.log10(0.05), np.log10(5000.0), 24)
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g = gpu_solve(S, ks, t0)
+    o = S.O.solve(ks, t0)
+    assert np.all(g["retcode"] == 0)
+    assert np.all(o["retcode"] == 0)
+    Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
+    assert np.max(np.abs(g["pk"] / Po - 1)) < 1e-4
+    assert rel_state_err(g["u"], o["u"]) < 1e-4
+    # identical algorithm: step counts agree closely
+    assert np.max(np.abs(g["stats"][:, 0] - o["stats"][:, 0]) / o["stats"][:, 0]) < 0.05
+
+
+def test_class_pk_fixture_on_gpu(fixture5):
+    S = fixture5
+    say("case: CLASS P(k) fixture, 516 modes")
+    d = np.loadtxt(GOLDEN / "class_Pk.dat")
+    ks, Pc = d[:, 0], d[:, 1]
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g = gpu_solve(S, ks, t0)
+    assert np.all(g["retcode"] == 0)
+    P = g["pk"]
+    assert np.linalg.norm(P - Pc) / max(np.linalg.norm(P), np.linalg.norm(Pc)) < 1e-3
+
+
+def test_massless_lmax10_matches_oracle(massless10):
+    S = massless10
+    say("case: ΛCDM massless ν lmax=10, 40 modes in [1e-4, 1]/Mpc")
+    h = S.pars["h"]
+    from symboltz.constants import k_dimensionless
+    ks = k_dimensionless(10 ** np.linspace(-4, 0, 40), h)
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g = gpu_solve(S, ks, t0)
+    o = S.O.solve(ks, t0)
+    Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
+    assert np.all(g["retcode"] == 0)
+    assert np.max(np.abs(g["pk"] / Po - 1)) < 1e-4
+    assert rel_state_err(g["u"], o["u"]) < 1e-4
+
+
+def test_failures_are_reported_not_raised(fixture5):
+    say("case: failure semantics k=0, k=NaN")
+    M = models.LCDM(lmax=5)
+    prob = CosmologyProblem(M, models.parameters_Planck18(M))
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        sol = solve(prob, np.array([1.0, 0.0, np.nan]))
+    assert sol.pts["retcode"][0] == 0
+    assert sol.pts["retcode"][1] != 0 and sol.pts["retcode"][2] != 0
+    assert not issuccess(sol)
+    assert any("Perturbation (mode k = nan) solution failed" in str(x.message) for x in w)
+    with pytest.raises(TypeError):
+        solve(prob, np.array([1.0 + 1.0j]))
+
+
+def test_saveat_dense_output_matches_oracle(fixture5):
+    S = fixture5
+    say("case: saveat 50 times, 6 modes")
+    ks = np.array([0.3, 3.0, 30.0, 300.0, 1000.0, 3000.0])
+    t0 = np.full(len(ks), S.t[0])
+    saveat = np.linspace(0.01, S.t[-1], 50)
+    gm = GpuModel(S.M)
+    gm.upload_background(0, S.t, S.u, S.du, S.P)
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, saveat=saveat)
+    plan.run()
+    g = plan.fetch(save=True)
+    plan.close()
+    o = S.O.solve(ks, t0, saveat=saveat)
+    for i in range(len(ks)):
+        assert rel_state_err(g["usave"][i], o["usave"][i]) < 1e-4
+
+
+def test_batched_parameter_sets_match_oracle():
+    say("case: 4 parameter sets x 12 modes in one launch")
+    M = models.LCDM(lmax=5)
+    base = models.parameters_Planck18(M)
+    sets = []
+    for i, (dh, dc) in enumerate([(0.0, 0.0), (0.03, 0.01), (-0.04, -0.015), (0.05, 0.02)]):
+        p = dict(base)
+        p["h"] = base["h"] + dh
+        p["Omega_c0"] = (0.12 + dc) / p["h"] ** 2
+        p["Omega_b0"] = 0.0224 / p["h"] ** 2
+        sets.append(p)
+    ks = 10 ** np.linspace(-1, 3, 12)
+    pk, res, bgs = solve_sets(M, sets, ks)
+    assert np.all(res["retcode"] == 0)
+    for s, pars in enumerate(sets):
+        S = Setup(M, pars)
+        P, r = S.O.spectrum_matter(ks)
+        assert np.max(np.abs(pk[s] / P - 1)) < 1e-4
