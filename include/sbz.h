@@ -69,13 +69,19 @@ typedef struct {
     const char* cache_path;
 } sbz_model_desc;
 
+/* What a plan stores at its saveat times. */
+#define SBZ_SAVE_STATES 1   /* full perturbation states [n_inst][n_save][n] */
+#define SBZ_SAVE_SOURCES 2  /* CMB sources S_T, S_E, S_ψ [n_inst][n_save][3] (cosmologies.jl:99-105) */
+
 /* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number
- * of concurrent modes in flight on the device (0: one per instance). */
+ * of concurrent modes in flight on the device (0: one per instance).  save: SBZ_SAVE_* flags
+ * (0 with saveat given: states). */
 typedef struct {
     double rtol;
     double atol;
     long long maxiters;
     int lanes;
+    int save;
 } sbz_opts;
 
 /* Compile emitted HIP source with hiprtc for gfx950 into a code-object file. */
@@ -105,6 +111,9 @@ int sbz_plan_run(sbz_plan* p, float* kernel_ms);
  * null), per-mode return codes, and stats [n_inst][6] = accepted steps, rejected steps, f evals,
  * J evals, LU factorizations, linear solves (DEStats, src/solve.jl:885-891). */
 int sbz_plan_fetch(sbz_plan* p, double* out_u, double* out_save, int* retcode, long long* stats);
+/* CMB source functions at the saveat times, [n_inst][n_save][3] = (S_T, S_E, S_ψ): the
+ * source_grid of src/observables/fourier.jl:267-281 with Ss = [ST, SE, Sψ]. */
+int sbz_plan_fetch_sources(sbz_plan* p, double* out_src);
 /* Matter power spectrum P(k, τ0) = P0(k) Δ_m² per instance from the final states
  * (spectrum_matter, src/observables/fourier.jl:79-101). */
 int sbz_plan_pk(sbz_plan* p, double* pk);

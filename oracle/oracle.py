@@ -46,6 +46,7 @@ def lib():
         L.ora_background.argtypes = [_dp] + sp + [ctypes.c_double, _dp]
         L.ora_delta_m.argtypes = [_ip, _dp] + sp + [ctypes.c_double, ctypes.c_double, _dp]
         L.ora_delta_m.restype = ctypes.c_double
+        L.ora_sources.argtypes = [_ip, _dp] + sp + [ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.ora_p0.argtypes = [_dp, ctypes.c_double]
         L.ora_p0.restype = ctypes.c_double
         L.ora_solve.argtypes = [_ip, _dp] + sp + [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -118,6 +119,13 @@ class Oracle:
     def delta_m(self, k, tau, y):
         y = np.ascontiguousarray(y, dtype=np.float64)
         return self.L.ora_delta_m(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), k, tau, _p(y))
+
+    def sources(self, k, tau, y):
+        """(S_T, S_E, S_ψ) at (τ, y)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(3)
+        self.L.ora_sources(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), k, tau, _p(y), _p(out))
+        return out
 
     def p0(self, k):
         return self.L.ora_p0(_p(self.P), k)

@@ -68,6 +68,38 @@ using std::tanh;
 using std::pow;
 using std::fabs;
 
+// second-order Taylor numbers (value, first and second derivative), for v'' in the CMB sources
+struct T2 {
+    double v, d, dd;
+    T2(double x = 0.0, double dx = 0.0, double ddx = 0.0) : v(x), d(dx), dd(ddx) {}
+};
+inline T2 operator+(T2 a, T2 b) { return {a.v + b.v, a.d + b.d, a.dd + b.dd}; }
+inline T2 operator-(T2 a, T2 b) { return {a.v - b.v, a.d - b.d, a.dd - b.dd}; }
+inline T2 operator-(T2 a) { return {-a.v, -a.d, -a.dd}; }
+inline T2 operator*(T2 a, T2 b) { return {a.v * b.v, a.v * b.d + a.d * b.v, a.v * b.dd + 2 * a.d * b.d + a.dd * b.v}; }
+inline T2 inv(T2 b) {  // 1/b: (1/b)' = -b'/b², (1/b)'' = 2b'²/b³ - b''/b²
+    double r = 1.0 / b.v;
+    return {r, -b.d * r * r, 2 * b.d * b.d * r * r * r - b.dd * r * r};
+}
+inline T2 operator/(T2 a, T2 b) { return a * inv(b); }
+inline T2 operator+(T2 a, double b) { return {a.v + b, a.d, a.dd}; }
+inline T2 operator+(double a, T2 b) { return {a + b.v, b.d, b.dd}; }
+inline T2 operator-(T2 a, double b) { return {a.v - b, a.d, a.dd}; }
+inline T2 operator-(double a, T2 b) { return {a - b.v, -b.d, -b.dd}; }
+inline T2 operator*(T2 a, double b) { return {a.v * b, a.d * b, a.dd * b}; }
+inline T2 operator*(double a, T2 b) { return {a * b.v, a * b.d, a * b.dd}; }
+inline T2 operator/(T2 a, double b) { return {a.v / b, a.d / b, a.dd / b}; }
+inline T2 operator/(double a, T2 b) { return a * inv(b); }
+// chain rule for y = φ(x): y' = φ' x', y'' = φ'' x'² + φ' x''
+inline T2 chain(T2 x, double f0, double f1, double f2) { return {f0, f1 * x.d, f2 * x.d * x.d + f1 * x.dd}; }
+inline T2 exp(T2 x) { double e = std::exp(x.v); return chain(x, e, e, e); }
+inline T2 sqrt(T2 x) { double r = std::sqrt(x.v); return chain(x, r, 0.5 / r, -0.25 / (r * x.v)); }
+inline T2 tanh(T2 x) { double t = std::tanh(x.v); return chain(x, t, 1 - t * t, -2 * t * (1 - t * t)); }
+inline T2 pow(T2 x, double p) {
+    return chain(x, std::pow(x.v, p), p * std::pow(x.v, p - 1), p * (p - 1) * std::pow(x.v, p - 2));
+}
+inline T2 fabs(T2 x) { return x.v < 0 ? -x : x; }
+
 // ---------------------------------------------------------------- constants (constants.jl:3-24)
 const double C_LIGHT = 299792458.0;
 const double H_PLANCK = 6.62607015e-34;
@@ -443,6 +475,62 @@ double p0(const double* P, double k) {
     return 2.0 * PI * PI / (k * k * k) * P[P_AS] * std::pow(k / P[P_KPIVOT], P[P_NS] - 1.0);
 }
 
+// Ψ = Φ - 12π a² Π / k² for a state (linear in u), Π = Σ (1+w) ρ σ (gravity.jl:39, cosmologies.jl:94).
+template <class T, class U>
+U psi_of(const Model& m, const double* P, const Bg<T>& b, double k, const U* u) {
+    const U* F = u + m.iF;
+    U Pi = (4.0 / 3.0) * b.rg * (F[2] / 2.0);
+    if (m.lnu >= 0) Pi = Pi + (4.0 / 3.0) * b.rn * (u[m.iN + 2] / 2.0);
+    if (m.nx > 0) {
+        U num = 0.0;
+        for (int i = 0; i < m.nx; ++i) {
+            double x = P[P_X + i];
+            num = num + P[P_W + i] * x * x / b.E[i] * u[m.iH + i * (m.lh + 1) + 2];
+        }
+        Pi = Pi + (b.rh + b.ph) * ((2.0 / 3.0) * num / (b.Irho + b.IP / 3.0));
+    }
+    return u[m.iPhi] - 12.0 * PI * b.a * b.a * Pi / (k * k);
+}
+
+// CMB source functions at (τ, u) (cosmologies.jl:99-105):
+//   S_T = v(δγ/4 + Ψ + Πγ/16) + e^{-κ}(Ψ' + Φ') + (v θ_b)'/k² + 3/(16k²)(v Πγ)'',
+//   S_E = 3 v Πγ / (16 (kχ)²),   S_ψ = -(Ψ+Φ)(τ-τrec)/(τ0-τrec)/(τ0-τ) for τ ≥ τrec.
+// u' = f(τ,u), u'' = ∂f/∂τ + J u'; v'' through second-order Taylor numbers of the background.
+void sources(const Model& m, const double* P, const Spline& s, double k, double tau, const double* u, double* S) {
+    int n = m.n;
+    std::vector<double> f(n), g(n), tmp(n);
+    f_eval(m, P, s, k, tau, u, f.data());
+    f_eval(m, P, s, k, tau, f.data(), g.data());
+    dfdt(m, P, s, k, tau, u, tmp.data());
+    for (int i = 0; i < n; ++i) g[i] += tmp[i];
+    Bg<T2> bt;
+    background_at<T2>(P, s, T2(tau, 1.0, 0.0), bt);
+    double kd = bt.kd.v, kd1 = bt.kd.d, kd2 = bt.kd.dd, eK = std::exp(-bt.kappa.v);
+    double v = -kd * eK, vd = (-kd1 + kd * kd) * eK, vdd = (-kd2 + 3 * kd * kd1 - kd * kd * kd) * eK;
+    Bg<double> b;
+    background_at<double>(P, s, tau, b);
+    Bg<D> bd;
+    background_at<D>(P, s, D(tau, 1.0), bd);
+    std::vector<D> ud(u, u + n);
+    double Psi = psi_of<double, double>(m, P, b, k, u);
+    double Psid = psi_of<D, D>(m, P, bd, k, ud.data()).d + psi_of<double, double>(m, P, b, k, f.data());
+    const double* F = u + m.iF;
+    const double* G = u + m.iG;
+    double Pi = F[2] + G[0] + G[2];
+    double Pid = f[m.iF + 2] + f[m.iG] + f[m.iG + 2];
+    double Pidd = g[m.iF + 2] + g[m.iG] + g[m.iG + 2];
+    double thb = u[m.iB + 1], thbd = f[m.iB + 1];
+    double tau0 = P[P_TAU0], taurec = P[25];
+    double ST = v * (F[0] / 4.0 + Psi + Pi / 16.0) + eK * (Psid + f[m.iPhi]) + (vd * thb + v * thbd) / (k * k) +
+                3.0 / (16.0 * k * k) * (vdd * Pi + 2.0 * vd * Pid + v * Pidd);
+    double chi = tau0 - tau;
+    double SE = chi > 0 ? 3.0 / 16.0 * v * Pi / ((k * chi) * (k * chi)) : 0.0;
+    double Spsi = (tau >= taurec && chi > 0) ? -(Psi + u[m.iPhi]) * (tau - taurec) / (tau0 - taurec) / chi : 0.0;
+    S[0] = ST;
+    S[1] = SE;
+    S[2] = Spsi;
+}
+
 // ---------------------------------------------------------------- dense LU (partial pivoting)
 bool lu_factor(int n, double* A, int* piv) {
     for (int c = 0; c < n; ++c) {
@@ -715,6 +803,13 @@ double ora_delta_m(const int* md, const double* P, const double* kt, const doubl
 }
 
 double ora_p0(const double* P, double k) { return p0(P, k); }
+
+void ora_sources(const int* md, const double* P, const double* kt, const double* ku, const double* kdu, int nk,
+                 double k, double tau, const double* u, double* S) {
+    Model m = make_model(md);
+    Spline s{kt, ku, kdu, nk};
+    sources(m, P, s, k, tau, u, S);
+}
 
 // Solve n_modes k-modes with `nthreads` worker threads (dynamic scheduling, one task per mode as
 // the reference's @spawn per k, src/solve.jl:566).  stats: 6 int64 per mode.

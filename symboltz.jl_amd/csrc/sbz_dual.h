@@ -56,3 +56,49 @@ SBZ_HD Dual sbz_sqrt(Dual x) { double s = sqrt(x.v); return Dual(s, 0.5 * x.d / 
 SBZ_HD Dual sbz_tanh(Dual x) { double t = tanh(x.v); return Dual(t, (1.0 - t * t) * x.d); }
 SBZ_HD Dual sbz_pow(Dual x, double p) { double y = pow(x.v, p); return Dual(y, p * y / x.v * x.d); }
 SBZ_HD Dual sbz_abs(Dual x) { return x.v < 0 ? -x : x; }
+
+// Second-order Taylor numbers (value, d/dτ, d²/dτ²): the CMB source functions need the second
+// time derivative of the visibility function v = -κ̇ e^{-κ} (cosmologies.jl:99-105), which the
+// reference obtains by expanding D(D(...)) symbolically.
+struct Tay2 {
+    double v, d, dd;
+    SBZ_HD Tay2() : v(0.0), d(0.0), dd(0.0) {}
+    SBZ_HD Tay2(double x) : v(x), d(0.0), dd(0.0) {}
+    SBZ_HD Tay2(double x, double dx, double ddx) : v(x), d(dx), dd(ddx) {}
+};
+SBZ_HD Tay2 operator+(Tay2 a, Tay2 b) { return Tay2(a.v + b.v, a.d + b.d, a.dd + b.dd); }
+SBZ_HD Tay2 operator-(Tay2 a, Tay2 b) { return Tay2(a.v - b.v, a.d - b.d, a.dd - b.dd); }
+SBZ_HD Tay2 operator-(Tay2 a) { return Tay2(-a.v, -a.d, -a.dd); }
+SBZ_HD Tay2 operator*(Tay2 a, Tay2 b) {
+    return Tay2(a.v * b.v, a.d * b.v + a.v * b.d, a.dd * b.v + 2.0 * a.d * b.d + a.v * b.dd);
+}
+SBZ_HD Tay2 operator/(Tay2 a, Tay2 b) {
+    double q = a.v / b.v;
+    double qd = (a.d - q * b.d) / b.v;
+    return Tay2(q, qd, (a.dd - 2.0 * qd * b.d - q * b.dd) / b.v);
+}
+SBZ_HD Tay2 operator+(Tay2 a, double b) { return Tay2(a.v + b, a.d, a.dd); }
+SBZ_HD Tay2 operator+(double a, Tay2 b) { return Tay2(a + b.v, b.d, b.dd); }
+SBZ_HD Tay2 operator-(Tay2 a, double b) { return Tay2(a.v - b, a.d, a.dd); }
+SBZ_HD Tay2 operator-(double a, Tay2 b) { return Tay2(a - b.v, -b.d, -b.dd); }
+SBZ_HD Tay2 operator*(Tay2 a, double b) { return Tay2(a.v * b, a.d * b, a.dd * b); }
+SBZ_HD Tay2 operator*(double a, Tay2 b) { return Tay2(a * b.v, a * b.d, a * b.dd); }
+SBZ_HD Tay2 operator/(Tay2 a, double b) { return Tay2(a.v / b, a.d / b, a.dd / b); }
+SBZ_HD Tay2 operator/(double a, Tay2 b) { return Tay2(a) / b; }
+SBZ_HD double dval(Tay2 x) { return x.v; }
+SBZ_HD Tay2 sbz_exp(Tay2 x) { double e = exp(x.v); return Tay2(e, e * x.d, e * (x.dd + x.d * x.d)); }
+SBZ_HD Tay2 sbz_log(Tay2 x) { return Tay2(log(x.v), x.d / x.v, x.dd / x.v - (x.d * x.d) / (x.v * x.v)); }
+SBZ_HD Tay2 sbz_sqrt(Tay2 x) {
+    double s = sqrt(x.v);
+    double sd = 0.5 * x.d / s;
+    return Tay2(s, sd, (0.5 * x.dd - sd * sd) / s);
+}
+SBZ_HD Tay2 sbz_tanh(Tay2 x) {
+    double t = tanh(x.v), s = 1.0 - t * t;
+    return Tay2(t, s * x.d, s * x.dd - 2.0 * t * s * x.d * x.d);
+}
+SBZ_HD Tay2 sbz_pow(Tay2 x, double p) {
+    double y1 = pow(x.v, p - 1.0), y = y1 * x.v;
+    return Tay2(y, p * y1 * x.d, p * y1 * x.dd + p * (p - 1.0) * (y1 / x.v) * x.d * x.d);
+}
+SBZ_HD Tay2 sbz_abs(Tay2 x) { return x.v < 0 ? -x : x; }

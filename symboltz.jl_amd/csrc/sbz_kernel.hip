@@ -28,11 +28,15 @@
 #error "model code missing"
 #endif
 
-// Strided view into the [slot][lane] workspace.
+// Views into the workspace.  Each wavefront owns a contiguous block laid out [slot][64 lanes]:
+// the lane stride is the compile-time constant 64, so every slot is a distinct constant offset the
+// compiler can disambiguate (no false store→load aliasing), and a wavefront's access to one slot
+// is 64 consecutive doubles (512 B, fully coalesced).
+#define SBZ_WAVE 64
+#define SBZ_NB 25
 struct SV {
     double* p;
-    int s;
-    SBZ_DEV double& operator[](int i) const { return p[(size_t)i * s]; }
+    SBZ_DEV double& operator[](int i) const { return p[i * SBZ_WAVE]; }
 };
 struct CSV {
     const double* p;
@@ -56,11 +60,12 @@ struct SbzSolveArgs {
     int n_save;
     double* out_u;             // [n_inst][SBZ_N] final state
     double* out_save;          // [n_inst][n_save][SBZ_N] or null
+    double* out_src;           // [n_inst][n_save][3] CMB sources (S_T, S_E, S_ψ) or null
     int* retcode;              // [n_inst]
     long long* stats;          // [n_inst][6]: accepted, rejected, f evals, J evals, LU, solves
     unsigned int* queue;       // work counter (zeroed by the host before launch)
-    double* ws;                // workspace [SBZ_NW][S]
-    int S;                     // lane stride of the workspace
+    double* ws;                // workspace: per wavefront [SBZ_NW][64]
+    int S;                     // number of lanes (multiple of 64)
 };
 
 // workspace slots (in units of SBZ_N doubles, then the LU)
@@ -116,6 +121,7 @@ SBZ_DEV void sbz_pt_background(const double* __restrict__ P, const double* __res
     B[15] = (P[SBZ_P_DE] == 1.0) ? -P[SBZ_P_WA] * s[0] * d.Hc : T(0.0);  // D(w) = -wa a ℋ
 #pragma unroll
     for (int i = 0; i < 8; ++i) B[16 + i] = (i < SBZ_NX) ? d.E[i] : T(1.0);
+    B[24] = s[4] - P[SBZ_P_KAPPA0];  // κ = _κ - κ0                              baryons.jl:177
 }
 
 SBZ_DEV void sbz_bg_plain(const double* P, const double* kt, int nk, int& idx, double tau, double* B) {
@@ -123,13 +129,38 @@ SBZ_DEV void sbz_bg_plain(const double* P, const double* kt, int nk, int& idx, d
 }
 
 SBZ_DEV void sbz_bg_dual(const double* P, const double* kt, int nk, int& idx, double tau, double* B, double* dB) {
-    Dual Bd[24];
+    Dual Bd[SBZ_NB];
     sbz_pt_background<Dual>(P, kt, nk, idx, Dual(tau, 1.0), Bd);
 #pragma unroll
-    for (int i = 0; i < 24; ++i) {
+    for (int i = 0; i < SBZ_NB; ++i) {
         B[i] = Bd[i].v;
         dB[i] = Bd[i].d;
     }
+}
+
+// Background with first and second τ-derivatives; returns d²κ̇/dτ² (for v̈ in the sources).
+SBZ_DEV double sbz_bg_tay2(const double* P, const double* kt, int nk, int& idx, double tau, double* B, double* dB) {
+    Tay2 Bt[SBZ_NB];
+    sbz_pt_background<Tay2>(P, kt, nk, idx, Tay2(tau, 1.0, 0.0), Bt);
+#pragma unroll
+    for (int i = 0; i < SBZ_NB; ++i) {
+        B[i] = Bt[i].v;
+        dB[i] = Bt[i].d;
+    }
+    return Bt[3].dd;
+}
+
+// CMB sources (S_T, S_E, S_ψ) at time ts for the state in slot y; f and g slots are scratch.
+template <class SVT>
+SBZ_DEV void sbz_sources_at(const double* P, const double* kt, int nk, int& idx, const double* Cs, double k,
+                            double ts, SVT y, SVT fy, SVT gy, SVT tmp, double* S) {
+    double B[SBZ_NB], dB[SBZ_NB];
+    const double ddkd = sbz_bg_tay2(P, kt, nk, idx, ts, B, dB);
+    sbz_f(B, Cs, k, y, fy);        // u'
+    sbz_f(B, Cs, k, fy, gy);       // J u' (the system is linear)
+    sbz_dfdt(B, dB, Cs, k, y, tmp);
+    for (int i = 0; i < SBZ_N; ++i) gy[i] = gy[i] + tmp[i];  // u'' = ∂f/∂τ + J u'
+    sbz_src(B, dB, ddkd, Cs, k, P[SBZ_P_TAU0], P[SBZ_P_TAUREC], y, fy, gy, S);
 }
 
 SBZ_DEV int sbz_find_knot(const double* kt, int nk, double t) {
@@ -174,10 +205,10 @@ __constant__ double R5d[9] = {0, SBZ_R5_D1, SBZ_R5_D2, SBZ_R5_D3, SBZ_R5_D4, SBZ
 extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.S) return;
-    const int S = a.S;
-    double* wsl = a.ws + lane;
-    auto vec = [&](int slot) { return SV{wsl + (size_t)slot * SBZ_N * S, S}; };
-    const SV W{wsl + (size_t)WS_W * S, S};
+    // this wavefront's workspace block
+    double* wsl = a.ws + (size_t)(lane / SBZ_WAVE) * SBZ_NW * SBZ_WAVE + (lane % SBZ_WAVE);
+    auto vec = [&](int slot) { return SV{wsl + (size_t)slot * SBZ_N * SBZ_WAVE}; };
+    const SV W{wsl + (size_t)WS_W * SBZ_WAVE};
 
 
     for (;;) {
@@ -193,7 +224,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
         double t = a.inst_t0[inst];
         double Cs[25];
         sbz_consts(P, Cs);
-        double B[24], dB[24];
+        double B[SBZ_NB], dB[SBZ_NB];
         int kidx = sbz_find_knot(kt, nk, t);
         long long st_acc = 0, st_rej = 0, st_f = 0, st_j = 0;
         int ret = SBZ_RET_SUCCESS;
@@ -213,8 +244,16 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
             if (!(v == v) || fabs(v) > 1e300) finite0 = false;
         }
         if (!finite0) ret = SBZ_RET_INITIALFAILURE;
-        while (osave && isave < a.n_save && a.saveat[isave] <= t) {
-            for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = vec(su)[i];
+        double* osrc = a.out_src ? a.out_src + (size_t)inst * a.n_save * 3 : nullptr;
+        while ((osave || osrc) && isave < a.n_save && a.saveat[isave] <= t) {
+            if (osave)
+                for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = vec(su)[i];
+            if (osrc && ret == SBZ_RET_SUCCESS) {
+                double S3[3];
+                sbz_sources_at(P, kt, nk, kidx, Cs, k, a.saveat[isave], vec(su), vec(WS_K1), vec(WS_K1 + 1),
+                               vec(WS_K1 + 2), S3);
+                for (int c = 0; c < 3; ++c) osrc[(size_t)isave * 3 + c] = S3[c];
+            }
             ++isave;
         }
         double dt = 0.0, qold = SBZ_CTL_QOLDINIT;
@@ -319,12 +358,21 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
             sbz_bg_plain(P, kt, nk, kidx, tn, B);
             sbz_f(B, Cs, k, u1, f1);
             st_f++;
-            while (osave && isave < a.n_save && a.saveat[isave] <= tn) {
+            while ((osave || osrc) && isave < a.n_save && a.saveat[isave] <= tn) {
+                // cubic Hermite dense output over the accepted step
                 double th = (a.saveat[isave] - t) / dt;
                 double h00 = (1 + 2 * th) * (1 - th) * (1 - th), h10 = th * (1 - th) * (1 - th);
                 double h01 = th * th * (3 - 2 * th), h11 = th * th * (th - 1);
-                for (int i = 0; i < SBZ_N; ++i)
-                    osave[(size_t)isave * SBZ_N + i] = h00 * u[i] + h10 * dt * f0[i] + h01 * u1[i] + h11 * dt * f1[i];
+                SV ys = vec(WS_K1);
+                for (int i = 0; i < SBZ_N; ++i) ys[i] = h00 * u[i] + h10 * dt * f0[i] + h01 * u1[i] + h11 * dt * f1[i];
+                if (osave)
+                    for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = ys[i];
+                if (osrc) {
+                    double S3[3];
+                    sbz_sources_at(P, kt, nk, kidx, Cs, k, a.saveat[isave], ys, vec(WS_K1 + 1), vec(WS_K1 + 2),
+                                   vec(WS_K1 + 3), S3);
+                    for (int c = 0; c < 3; ++c) osrc[(size_t)isave * 3 + c] = S3[c];
+                }
                 ++isave;
             }
             t = tn;
@@ -361,7 +409,7 @@ extern "C" __global__ void sbz_pk_kernel(const double* __restrict__ params, cons
     const double* kt = knots + knot_off[set];
     int nk = knot_n[set];
     double k = inst_k[i];
-    double Cs[25], B[24];
+    double Cs[25], B[SBZ_NB];
     sbz_consts(P, Cs);
     int idx = nk - 2;
     sbz_bg_plain(P, kt, nk, idx, kt[nk - 1], B);
@@ -383,9 +431,17 @@ extern "C" int sbz_probe_n(void) { return SBZ_N; }
 extern "C" int sbz_probe_nnz(void) { return SBZ_NNZ_LU; }
 extern "C" void sbz_probe_bg(const double* P, const double* kt, int nk, double tau, double* B, double* dB) {
     int idx = sbz_find_knot(kt, nk, tau);
-    Dual Bd[24];
+    Dual Bd[SBZ_NB];
     sbz_pt_background<Dual>(P, kt, nk, idx, Dual(tau, 1.0), Bd);
-    for (int i = 0; i < 24; ++i) { B[i] = Bd[i].v; dB[i] = Bd[i].d; }
+    for (int i = 0; i < SBZ_NB; ++i) { B[i] = Bd[i].v; dB[i] = Bd[i].d; }
+}
+extern "C" void sbz_probe_src(const double* P, const double* kt, int nk, double k, double tau, const double* u,
+                              double* S, double* work) {
+    double Cs[25]; sbz_consts(P, Cs);
+    int idx = sbz_find_knot(kt, nk, tau);
+    for (int i = 0; i < SBZ_N; ++i) work[i] = u[i];
+    sbz_sources_at(P, kt, nk, idx, Cs, k, tau, HV{work}, HV{work + SBZ_N}, HV{work + 2 * SBZ_N},
+                   HV{work + 3 * SBZ_N}, S);
 }
 extern "C" void sbz_probe_f(const double* B, const double* P, double k, const double* u, double* du) {
     double Cs[25]; sbz_consts(P, Cs);

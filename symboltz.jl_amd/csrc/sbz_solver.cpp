@@ -57,6 +57,7 @@ struct SbzSolveArgsHost {
     int n_save;
     double* out_u;
     double* out_save;
+    double* out_src;
     int* retcode;
     long long* stats;
     unsigned int* queue;
@@ -71,7 +72,8 @@ struct sbz_plan {
     long long maxiters = 100000;
     // device buffers
     double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
-    double *d_out = nullptr, *d_save = nullptr, *d_ws = nullptr, *d_pk = nullptr;
+    double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_ws = nullptr, *d_pk = nullptr;
+    int save_flags = 0;
     long long *d_koff = nullptr, *d_stats = nullptr;
     int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
     unsigned int* d_queue = nullptr;
@@ -214,7 +216,7 @@ extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const doubl
 }
 
 static void plan_free(sbz_plan* p) {
-    void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_ws,
+    void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src, p->d_ws,
                     p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -263,6 +265,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->rtol = opts->rtol;
     p->atol = opts->atol;
     p->maxiters = opts->maxiters > 0 ? opts->maxiters : 100000;
+    p->save_flags = n_save > 0 ? (opts->save ? opts->save : SBZ_SAVE_STATES) : 0;
     p->block = 64;
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
@@ -296,8 +299,10 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     UP(&p->d_saveat, saveat, (size_t)n_save);
 #undef UP
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_out, sizeof(double) * std::max<size_t>((size_t)n_inst * m->n, 1));
-    if (e == hipSuccess && n_save > 0)
+    if (e == hipSuccess && (p->save_flags & SBZ_SAVE_STATES))
         e = hipMalloc((void**)&p->d_save, sizeof(double) * (size_t)n_inst * n_save * m->n);
+    if (e == hipSuccess && (p->save_flags & SBZ_SAVE_SOURCES))
+        e = hipMalloc((void**)&p->d_src, sizeof(double) * (size_t)n_inst * n_save * 3);
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_ret, sizeof(int) * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, sizeof(long long) * 6 * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_pk, sizeof(double) * std::max(n_inst, 1));
@@ -341,6 +346,7 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     a.n_save = p->n_save;
     a.out_u = p->d_out;
     a.out_save = p->d_save;
+    a.out_src = p->d_src;
     a.retcode = p->d_ret;
     a.stats = p->d_stats;
     a.queue = p->d_queue;
@@ -386,12 +392,25 @@ extern "C" int sbz_plan_fetch(sbz_plan* p, double* out_u, double* out_save, int*
     size_t n = (size_t)p->n_inst;
     if (n == 0) return SBZ_OK;
     if (out_u) HIPCHK(hipMemcpyAsync(out_u, p->d_out, sizeof(double) * n * m->n, hipMemcpyDeviceToHost, m->stream));
-    if (out_save && p->n_save)
+    if (out_save && p->d_save)
         HIPCHK(hipMemcpyAsync(out_save, p->d_save, sizeof(double) * n * p->n_save * m->n, hipMemcpyDeviceToHost,
                               m->stream));
     if (retcode) HIPCHK(hipMemcpyAsync(retcode, p->d_ret, sizeof(int) * n, hipMemcpyDeviceToHost, m->stream));
     if (stats) HIPCHK(hipMemcpyAsync(stats, p->d_stats, sizeof(long long) * 6 * n, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_fetch_sources(sbz_plan* p, double* out_src) {
+    sbz_clear_error();
+    if (!p || !out_src) return SBZ_ERR_ARG;
+    if (!p->d_src) {
+        sbz_set_error("sbz_plan_fetch_sources: plan was created without SBZ_SAVE_SOURCES");
+        return SBZ_ERR_ARG;
+    }
+    HIPCHK(hipMemcpyAsync(out_src, p->d_src, sizeof(double) * (size_t)p->n_inst * p->n_save * 3,
+                          hipMemcpyDeviceToHost, p->m->stream));
+    HIPCHK(hipStreamSynchronize(p->m->stream));
     return SBZ_OK;
 }
 

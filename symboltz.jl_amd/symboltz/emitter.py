@@ -34,7 +34,7 @@ CSRC = Path(__file__).resolve().parent.parent / "csrc"
 
 # Background vector layout B[] (device: sbz_pt_background in csrc/sbz_kernel.h)
 B_NAMES = ["tau", "a", "Hc", "kd", "cs2", "rg", "rn", "rc", "rb", "rh", "ph", "Irho", "IP", "rx", "wx", "wdx"] + \
-          [f"E{i}" for i in range(8)]
+          [f"E{i}" for i in range(8)] + ["kappa"]
 NB = len(B_NAMES)
 BIDX = {n: i for i, n in enumerate(B_NAMES)}
 
@@ -230,6 +230,43 @@ class System:
         v[L["Phi"]] = Psi + 12 * sp.pi * a**2 * Pi / k**2                   # gravity.jl:39
         return v
 
+    def sources(self):
+        """CMB source functions (S_T, S_E, S_ψ) of cosmologies.jl:99-105 in terms of u, f = u',
+        g = u'', the background B, its derivative dB and the second derivative of κ̇."""
+        L, u, B, dB, k = self.L, self.u, self.B, self.dB, self.k
+        n = self.n
+        fs = sp.symbols(f"f0:{n}", real=True)
+        gs = sp.symbols(f"g0:{n}", real=True)
+        ddkd = sp.Symbol("ddkd", real=True)
+        tau0, taurec = sp.Symbol("tau0", real=True), sp.Symbol("taurec", real=True)
+
+        def D(expr):
+            out = sum(sp.diff(expr, u[j]) * fs[j] for j in range(n) if expr.has(u[j]))
+            out += sum(sp.diff(expr, B[nm]) * dB[nm] for nm in B_NAMES if expr.has(B[nm]))
+            return out
+
+        kd, kdd = B["kd"], dB["kd"]
+        eK = sp.exp(-B["kappa"])
+        v = -kd * eK                                       # v = D(e^{-κ}), baryons.jl:179
+        vd = (-kdd + kd**2) * eK
+        vdd = (-ddkd + 3 * kd * kdd - kd**3) * eK
+        iF, iG = L["F"], L["G"]
+        Pig = u[iF + 2] + u[iG] + u[iG + 2]                 # Π_γ, photons.jl:39
+        Pigd = fs[iF + 2] + fs[iG] + fs[iG + 2]
+        Pigdd = gs[iF + 2] + gs[iG] + gs[iG + 2]
+        Psi = self.Psi
+        Phi = u[L["Phi"]]
+        thb, thbd = u[L["B"] + 1], fs[L["B"] + 1]
+        ST_SW = v * (u[iF] / 4 + Psi + Pig / 16)
+        ST_ISW = eK * (D(Psi) + fs[L["Phi"]])
+        ST_Dop = (vd * thb + v * thbd) / k**2
+        ST_pol = sp.Rational(3, 16) / k**2 * (vdd * Pig + 2 * vd * Pigd + v * Pigdd)
+        chi = tau0 - B["tau"]
+        ST = ST_SW + ST_ISW + ST_Dop + ST_pol
+        SE = sp.Rational(3, 16) * v * Pig / (k * chi) ** 2
+        Spsi = -(Psi + Phi) * (B["tau"] - taurec) / (tau0 - taurec) / (tau0 - B["tau"])
+        return ST, SE, Spsi
+
     def delta_m(self):
         """Δ_m = Σ ρ_s Δ_s / Σ ρ_s over c, b, h; Δ_s = δ_s + 3ℋ(1+w_s)θ_s/k² (fourier.jl:39-52)."""
         M, L, u, B, C, k = self.M, self.L, self.u, self.B, self.C, self.k
@@ -348,6 +385,10 @@ class _Printer(C99CodePrinter):
             return f"Cs[{CIDX[nm[2:]]}]"
         if re.fullmatch(r"u\d+", nm):
             return f"u[{nm[1:]}]"
+        if re.fullmatch(r"f\d+", nm):
+            return f"fv[{nm[1:]}]"
+        if re.fullmatch(r"g\d+", nm):
+            return f"gv[{nm[1:]}]"
         return nm
 
 
@@ -404,8 +445,9 @@ def _emit_block(assign, tmp_prefix, opt=True):
 class Emitted:
     """Emitted device source of one model, with its structure metadata."""
 
-    def __init__(self, M):
+    def __init__(self, M, fence=0):
         self.M = M
+        self.fence = fence  # emit a compiler memory fence every `fence` LU pivots / solve rows (0: none)
         sysm = System(M)
         self.sys = sysm
         n = sysm.n
@@ -469,6 +511,8 @@ class Emitted:
                     flops_lu += 2
                 lu.append("      }")
             lu.append("    }")
+            if self.fence and (self.order.index(p) + 1) % self.fence == 0:
+                lu.append('    asm volatile("" ::: "memory");')
         lu.append("    return ok;")
         lu.append("}")
         self.ops["lu"] = (flops_lu, n, 0)
@@ -476,14 +520,18 @@ class Emitted:
         # solve
         so = ["template <class WT, class BT>", "SBZ_DEV void sbz_solve(WT W, BT b) {"]
         flops_s = 0
-        for p in self.order:
+        for r, p in enumerate(self.order):
             for i in self.lower[p]:
                 so.append(f"    b[{i}] -= W[{S[(i, p)]}] * b[{p}];")
                 flops_s += 2
-        for p in reversed(self.order):
+            if self.fence and (r + 1) % self.fence == 0:
+                so.append('    asm volatile("" ::: "memory");')
+        for r, p in enumerate(reversed(self.order)):
             terms = " ".join(f"- W[{S[(p, j)]}] * b[{j}]" for j in self.upper[p])
             flops_s += 2 * len(self.upper[p]) + 1
             so.append(f"    b[{p}] = (b[{p}] {terms}) * W[{S[(p, p)]}];")
+            if self.fence and (r + 1) % self.fence == 0:
+                so.append('    asm volatile("" ::: "memory");')
         so.append("}")
         self.ops["solve"] = (flops_s, 0, 0)
         out += so
@@ -508,6 +556,18 @@ class Emitted:
         out.append("template <class UO>")
         out.append("SBZ_DEV void sbz_ic(const double* __restrict__ B, const double* __restrict__ Cs, double k, UO u) {")
         out += lines
+        out.append("}")
+        # CMB sources at one time (S_ψ is zero before τrec; S_E and S_ψ are set to zero at χ = 0)
+        ST, SE, Spsi = sysm.sources()
+        lines, ops = _emit_block([("S[0]", ST), ("S[1]", SE), ("const double spsi", Spsi)], "ts")
+        self.ops["src"] = ops
+        out.append("template <class UI, class FI, class GI>")
+        out.append("SBZ_DEV void sbz_src(const double* __restrict__ B, const double* __restrict__ dB, double ddkd, "
+                   "const double* __restrict__ Cs, double k, double tau0, double taurec, UI u, FI fv, GI gv, double* S) {")
+        out += [ln.replace("    const double spsi =", "    const double spsi =") for ln in lines]
+        out.append("    const double tau = B[0];")
+        out.append("    S[2] = (tau >= taurec && tau < tau0) ? spsi : 0.0;")
+        out.append("    if (!(tau < tau0)) S[1] = 0.0;")
         out.append("}")
         # Δ_m
         lines, _ = _emit_block([("double r", sysm.delta_m())], "tm")
@@ -550,9 +610,9 @@ def expand_includes(path, seen=None):
     return "\n".join(out)
 
 
-def kernel_source(M):
+def kernel_source(M, fence=0):
     """Full self-contained HIP source (emitted model + generic kernel) and its metadata."""
-    em = Emitted(M)
+    em = Emitted(M, fence)
     kern = expand_includes(CSRC / "sbz_kernel.hip")
     src = kern.replace("// @@SBZ_MODEL@@", em.src)
     em.full_src = src

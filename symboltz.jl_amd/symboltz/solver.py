@@ -57,7 +57,11 @@ class _Desc(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiters", ctypes.c_longlong),
-                ("lanes", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("save", ctypes.c_int)]
+
+
+SAVE_STATES = 1
+SAVE_SOURCES = 2
 
 
 def _declare(L):
@@ -73,6 +77,7 @@ def _declare(L):
     L.sbz_plan_run.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.sbz_plan_fetch.argtypes = [vp, dp, dp, ip, llp]
     L.sbz_plan_pk.argtypes = [vp, dp]
+    L.sbz_plan_fetch_sources.argtypes = [vp, dp]
     L.sbz_plan_destroy.argtypes = [vp]
     L.sbz_compile.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
 
@@ -104,8 +109,8 @@ class GpuModel:
                                         _lib.ptr(P)), "sbz_bg_upload")
         self.n_sets = max(self.n_sets, set_index + 1)
 
-    def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0):
-        return Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes)
+    def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0, save=0):
+        return Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save)
 
     def close(self):
         if getattr(self, "h", None):
@@ -120,7 +125,7 @@ class GpuModel:
 
 
 class Plan:
-    def __init__(self, gm, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes):
+    def __init__(self, gm, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save=0):
         self.gm = gm
         L = gm.L
         self.set = np.ascontiguousarray(inst_set, dtype=np.int32)
@@ -129,7 +134,8 @@ class Plan:
         self.n_inst = len(self.k)
         self.saveat = np.ascontiguousarray(saveat if saveat is not None else np.zeros(0), dtype=np.float64)
         self.n_save = len(self.saveat)
-        opts = _Opts(rtol, atol, int(maxiters), int(lanes))
+        opts = _Opts(rtol, atol, int(maxiters), int(lanes), int(save))
+        self.save = save if (save or self.n_save == 0) else SAVE_STATES
         h = ctypes.c_void_p()
         _lib.check(L.sbz_plan_create(gm.h, _lib.ptr(self.set, ctypes.c_int), _lib.ptr(self.k), _lib.ptr(self.t0),
                                      self.n_inst, ctypes.byref(opts), _lib.ptr(self.saveat), self.n_save,
@@ -144,13 +150,18 @@ class Plan:
     def fetch(self, save=False):
         n, m = self.n_inst, self.gm.n
         u = np.zeros((n, m))
-        us = np.zeros((n, self.n_save, m)) if (save and self.n_save) else None
+        us = np.zeros((n, self.n_save, m)) if (save and self.n_save and (self.save & SAVE_STATES)) else None
         ret = np.zeros(n, dtype=np.int32)
         st = np.zeros((n, 6), dtype=np.int64)
         _lib.check(self.gm.L.sbz_plan_fetch(self.h, _lib.ptr(u), _lib.ptr(us) if us is not None else None,
                                             _lib.ptr(ret, ctypes.c_int), _lib.ptr(st, ctypes.c_longlong)),
                    "sbz_plan_fetch")
         return dict(u=u, usave=us, retcode=ret, stats=st)
+
+    def sources(self):
+        out = np.zeros((self.n_inst, self.n_save, 3))
+        _lib.check(self.gm.L.sbz_plan_fetch_sources(self.h, _lib.ptr(out)), "sbz_plan_fetch_sources")
+        return out
 
     def pk(self):
         out = np.zeros(self.n_inst)

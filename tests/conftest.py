@@ -59,6 +59,7 @@ class Probe:
         L.sbz_probe_ic.argtypes = [dp, dp, ctypes.c_double, dp]
         L.sbz_probe_dm.argtypes = [dp, dp, ctypes.c_double, dp]
         L.sbz_probe_dm.restype = ctypes.c_double
+        L.sbz_probe_src.argtypes = [dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp, dp]
         self.L = L
         self.n = L.sbz_probe_n()
         self.nnz = L.sbz_probe_nnz()
@@ -68,7 +69,7 @@ class Probe:
         return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
     def bg(self, S, tau):
-        B, dB = np.zeros(24), np.zeros(24)
+        B, dB = np.zeros(25), np.zeros(25)
         self.L.sbz_probe_bg(self.p(S.P), self.p(S.knots), len(S.t), tau, self.p(B), self.p(dB))
         return B, dB
 
@@ -104,6 +105,13 @@ class Probe:
         u = np.zeros(self.n)
         self.L.sbz_probe_ic(self.p(B), self.p(S.P), k, self.p(u))
         return u
+
+    def src(self, S, k, tau, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(3)
+        work = np.zeros(4 * self.n)
+        self.L.sbz_probe_src(self.p(S.P), self.p(S.knots), len(S.t), k, tau, self.p(u), self.p(out), self.p(work))
+        return out
 
     def dm(self, S, k, tau, u):
         B, _ = self.bg(S, tau)

@@ -63,3 +63,15 @@ def test_initial_conditions_and_overdensity(pair):
             a, b = pr.ic(S, k, tau), S.O.initial_state(k, tau)
             assert np.allclose(a, b, rtol=1e-12, atol=1e-300)
             assert abs(pr.dm(S, k, 3.0, b) / S.O.delta_m(k, 3.0, b) - 1) < 1e-12
+
+
+def test_cmb_sources(pair):
+    S, pr = pair
+    rng = np.random.default_rng(4)
+    taurec = S.P[25]
+    for tau in [0.01, 0.05, taurec, 0.2, 1.0, 3.0]:
+        for k in KS:
+            u = rng.standard_normal(pr.n)
+            a, b = pr.src(S, k, tau, u), S.O.sources(k, tau, u)
+            scale = np.abs(b).max() + 1e-300
+            assert np.max(np.abs(a - b)) <= 1e-9 * scale, (tau, k, a, b)

@@ -134,3 +134,25 @@ def test_batched_parameter_sets_match_oracle():
         S = Setup(M, pars)
         P, r = S.O.spectrum_matter(ks)
         assert np.max(np.abs(pk[s] / P - 1)) < 1e-4
+
+
+def test_cmb_sources_match_oracle(fixture5):
+    S = fixture5
+    say("case: CMB sources on 60 times, 8 modes")
+    from symboltz.solver import SAVE_SOURCES, SAVE_STATES
+    ks = np.array([0.5, 2.0, 8.0, 30.0, 100.0, 300.0, 900.0, 2500.0])
+    t0 = np.full(len(ks), S.t[0])
+    saveat = np.linspace(0.02, S.t[-1], 60)
+    gm = GpuModel(S.M)
+    gm.upload_background(0, S.t, S.u, S.du, S.P)
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, saveat=saveat, save=SAVE_SOURCES | SAVE_STATES)
+    plan.run()
+    g = plan.fetch(save=True)
+    src = plan.sources()
+    plan.close()
+    o = S.O.solve(ks, t0, saveat=saveat)
+    for i, k in enumerate(ks):
+        ref = np.array([S.O.sources(k, tau, o["usave"][i, j]) for j, tau in enumerate(saveat)])
+        for c in range(3):
+            scale = np.abs(ref[:, c]).max() + 1e-300
+            assert np.max(np.abs(src[i, :, c] - ref[:, c])) <= 1e-4 * scale, (k, c)

