@@ -124,6 +124,17 @@ int sbz_solve_batch(sbz_model* m, const int* inst_set, const double* inst_k, con
                     const sbz_opts* opts, const double* saveat, int n_save, double* out_u, double* out_save,
                     int* retcode, long long* stats);
 
+/* ---- line-of-sight integration ----------------------------------------------------------- */
+
+/* I_l(k) = Σ_τ w_τ S(τ, k) j_l(k(τ0 - τ)) for every k and tabulated l (los_integrate,
+ * src/observables/angular.jl:109-185, non-Limber branch; trapezoid weights over τs, τ0 = τs[n_tau-1]).
+ * S(iτ, ik) = S[ik*s_k + iτ*s_tau]; y/dy: j_l and j_l' on the uniform grid x_i = i dx, stored
+ * [n_x][n_l] (SphericalBesselCache, angular.jl:9-48); out: [n_k][n_l].  Host buffers; the kernel
+ * time is returned in *kernel_ms. */
+int sbz_los_integrate(const double* S, long long s_k, long long s_tau, const double* taus, int n_tau,
+                      const double* ks, int n_k, const double* y, const double* dy, int n_x, double dx, int n_l,
+                      double* out, float* kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-f
 def build_library(verbose=False):
     LIBDIR.mkdir(exist_ok=True)
     out = LIBDIR / "libsbz_hip.so"
-    srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp")]
+    srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_los.hip")]
     cmd = ["hipcc", "-fPIC", "-shared", "-I", str(INCLUDE)] + HIP_FLAGS + srcs + ["-lhiprtc", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -90,4 +90,9 @@ def build_models(models=None, jobs=4, verbose=False):
         for p in ex.map(lambda w: compile_code_object(*w), work):
             if verbose:
                 print("built", p)
+    keep = {f"{model_source(M).full_hash}.hsaco" for M in models}
+    keep |= {f"src_{getattr(model_source(M), 'tag', '')}.json" for M in models}
+    for f in KCACHE.iterdir():   # drop code objects and emitted sources of older kernel versions
+        if f.name not in keep and (f.suffix == ".hsaco" or f.name.startswith("src_")):
+            f.unlink()
     return [KCACHE / f"{model_source(M).full_hash}.hsaco" for M in models]

@@ -32,11 +32,13 @@ def model_source(M):
     if meta.exists():
         d = json.loads(meta.read_text())
         em = _Stored(d)
+        em.tag = tag
     else:
         em = kernel_source(M)
         KCACHE.mkdir(exist_ok=True)
         meta.write_text(json.dumps(dict(full_src=em.full_src, full_hash=em.full_hash, n=em.n, nnz_LU=em.nnz_LU,
                                         nnz_J=em.nnz_J, flops_step=em.flops_step, ops={k: list(v) for k, v in em.ops.items()})))
+    em.tag = tag
     _src_cache[key] = em
     return em
 
