@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg1")
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--algo", choices=["wave", "lane"], default="wave",
+                    help="wave: one wavefront per k-mode (default); lane: one lane per k-mode")
     ap.add_argument("--cpu-sample", type=int, default=32, help="oracle baseline: every N-th mode")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -71,7 +73,7 @@ def main():
 
     from symboltz import _lib
     from symboltz.problem import CosmologyProblem, solvebg
-    from symboltz.solver import GpuModel
+    from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_LANE
 
     if _lib.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
@@ -83,7 +85,9 @@ def main():
     gm = GpuModel(M)
     gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
     t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), bg.t[0], bg.t[-1])
-    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes)
+    algo = ALGO_WAVE if args.algo == "wave" else ALGO_LANE
+    kname = "sbz_wave_kernel" if args.algo == "wave" else "sbz_rodas5p_kernel"
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes, algo=algo)
     log(f"workload {args.workload}: n={gm.n}, {len(ks)} modes, background {t_bg*1e3:.1f} ms ({len(bg.t)} knots)")
     for w in range(args.warmup):
         ms = plan.run()
@@ -140,11 +144,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "description": desc, "n_modes": n_modes, "n_unknowns": gm.n,
+        "config": {"workload": args.workload, "description": desc, "algo": args.algo, "n_modes": n_modes, "n_unknowns": gm.n,
                    "background_ms": t_bg * 1e3, "all_success": ok},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "sbz_rodas5p_kernel", "kernel_ms": float(np.mean(kernel_ms)),
+                     "kernel": kname, "kernel_ms": float(np.mean(kernel_ms)),
                      "flops_per_launch": flops, "flops_per_step": gm.em.flops_step, "steps_per_launch": steps_total},
         "cpu_baseline": cpu,
     }

@@ -73,15 +73,20 @@ typedef struct {
 #define SBZ_SAVE_STATES 1   /* full perturbation states [n_inst][n_save][n] */
 #define SBZ_SAVE_SOURCES 2  /* CMB sources S_T, S_E, S_ψ [n_inst][n_save][3] (cosmologies.jl:99-105) */
 
+/* Kernel variants. */
+#define SBZ_ALGO_WAVE 1  /* one 64-lane wavefront per mode: chains in lanes, row-parallel border LU (default) */
+#define SBZ_ALGO_LANE 2  /* one lane per mode, state in a strided global workspace */
+
 /* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number
- * of concurrent modes in flight on the device (0: one per instance).  save: SBZ_SAVE_* flags
- * (0 with saveat given: states). */
+ * of concurrently resident modes (0: one per instance).  save: SBZ_SAVE_* flags (0 with saveat
+ * given: states).  algo: SBZ_ALGO_* (0: default). */
 typedef struct {
     double rtol;
     double atol;
     long long maxiters;
     int lanes;
     int save;
+    int algo;
 } sbz_opts;
 
 /* Compile emitted HIP source with hiprtc for gfx950 into a code-object file. */

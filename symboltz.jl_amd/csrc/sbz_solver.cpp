@@ -34,7 +34,7 @@ struct SbzSetTable {
 struct sbz_model {
     int n = 0, nnz_lu = 0, nw = 0;
     hipModule_t module = nullptr;
-    hipFunction_t k_solve = nullptr, k_pk = nullptr;
+    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -74,6 +74,7 @@ struct sbz_plan {
     double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
     double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_ws = nullptr, *d_pk = nullptr;
     int save_flags = 0;
+    int algo = SBZ_ALGO_WAVE;
     long long *d_koff = nullptr, *d_stats = nullptr;
     int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
     unsigned int* d_queue = nullptr;
@@ -167,6 +168,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         return SBZ_ERR_COMPILE;
     }
     if (hipModuleGetFunction(&m->k_solve, m->module, "sbz_rodas5p_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_wave, m->module, "sbz_wave_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk, m->module, "sbz_pk_kernel") != hipSuccess) {
         sbz_set_error("code object lacks the solver kernels");
         (void)hipModuleUnload(m->module);
@@ -266,10 +268,17 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->atol = opts->atol;
     p->maxiters = opts->maxiters > 0 ? opts->maxiters : 100000;
     p->save_flags = n_save > 0 ? (opts->save ? opts->save : SBZ_SAVE_STATES) : 0;
+    if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE) {
+        sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
+        delete p;
+        return SBZ_ERR_ARG;
+    }
+    p->algo = opts->algo ? opts->algo : SBZ_ALGO_WAVE;
     p->block = 64;
+    // lane-per-mode: `lanes` concurrent modes; wave-per-mode: `lanes` concurrent wavefronts
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
-    lanes = (lanes + p->block - 1) / p->block * p->block;
+    if (p->algo == SBZ_ALGO_LANE) lanes = (lanes + p->block - 1) / p->block * p->block;
     p->lanes = lanes;
     // concatenated background tables
     std::vector<double> params((size_t)n_sets * SBZ_NPAR), knots;
@@ -307,7 +316,8 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, sizeof(long long) * 6 * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_pk, sizeof(double) * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_queue, sizeof(unsigned int));
-    if (e == hipSuccess) e = hipMalloc((void**)&p->d_ws, sizeof(double) * (size_t)m->nw * lanes);
+    if (e == hipSuccess && p->algo == SBZ_ALGO_LANE)
+        e = hipMalloc((void**)&p->d_ws, sizeof(double) * (size_t)m->nw * lanes);
     if (e == hipSuccess) e = hipEventCreate(&p->ev0);
     if (e == hipSuccess) e = hipEventCreate(&p->ev1);
     if (e != hipSuccess) {
@@ -353,9 +363,10 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     a.ws = p->d_ws;
     a.S = p->lanes;
     void* args[] = {&a};
-    unsigned grid = (unsigned)(p->lanes / p->block);
+    hipFunction_t kf = (p->algo == SBZ_ALGO_LANE) ? m->k_solve : m->k_wave;
+    unsigned grid = (p->algo == SBZ_ALGO_LANE) ? (unsigned)(p->lanes / p->block) : (unsigned)p->lanes;
     HIPCHK(hipEventRecord(p->ev0, m->stream));
-    HIPCHK(hipModuleLaunchKernel(m->k_solve, grid, 1, 1, (unsigned)p->block, 1, 1, 0, m->stream, args, nullptr));
+    HIPCHK(hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, args, nullptr));
     HIPCHK(hipEventRecord(p->ev1, m->stream));
     HIPCHK(hipEventSynchronize(p->ev1));
     HIPCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));

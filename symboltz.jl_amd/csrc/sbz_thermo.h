@@ -64,12 +64,18 @@ SBZ_HD void sbz_nu_integrals(const double* P, T a, T* E, T& Irho, T& IP) {
     T y = P[SBZ_P_Y0H] * a;
     Irho = T(0.0);
     IP = T(0.0);
-    for (int i = 0; i < nx; ++i) {
-        double x = P[SBZ_P_X + i];
-        T Ei = sbz_sqrt(x * x + y * y);
-        E[i] = Ei;
-        Irho = Irho + P[SBZ_P_W + i] * Ei;
-        IP = IP + P[SBZ_P_W + i] * (x * x / Ei);
+    // fixed trip count with a predicate: E[] stays in registers on the device
+#pragma unroll
+    for (int i = 0; i < SBZ_MAX_NX; ++i) {
+        if (i < nx) {
+            double x = P[SBZ_P_X + i];
+            T Ei = sbz_sqrt(x * x + y * y);
+            E[i] = Ei;
+            Irho = Irho + P[SBZ_P_W + i] * Ei;
+            IP = IP + P[SBZ_P_W + i] * (x * x / Ei);
+        } else {
+            E[i] = T(1.0);
+        }
     }
 }
 

@@ -576,6 +576,7 @@ class Emitted:
         out += [ln.replace("    double r =", "    const double r =") for ln in lines]
         out.append("    return r;")
         out.append("}")
+        out += self._emit_wave()
         f_f, f_w, f_lu, f_s, f_d = (self.ops["f"][0] + self.ops["f"][1], self.ops["w"][0] + self.ops["w"][1],
                                     self.ops["lu"][0], self.ops["solve"][0],
                                     self.ops["dfdt"][0] + self.ops["dfdt"][1])
@@ -589,6 +590,152 @@ class Emitted:
         out.append(f"#define SBZ_FLOPS_DFDT {f_d}")
         out.append(f"#define SBZ_FLOPS_STEP {self.flops_step}")
         return "\n".join(out) + "\n"
+
+    def _emit_wave(self):
+        """Structure and straight-line code of the wavefront-per-mode kernel (see sbz_kernel.hip)."""
+        M, sysm, n = self.M, self.sys, self.n
+        chains, border = wave_structure(M)
+        bset = set(border)
+        # border elimination order: minimum degree on the border block of W
+        bpat = {(i, j) for (i, j) in self.J if i in bset and j in bset}
+        loc = {c: r for r, c in enumerate(border)}
+        lpat = {(loc[i], loc[j]) for (i, j) in bpat}
+        nb = len(border)
+        if nb > 64 or len(chains) > 64:
+            raise ValueError("wave kernel supports at most 64 border unknowns and 64 chains")
+        order = min_degree_order(nb, lpat | {(j, i) for (i, j) in lpat})
+        lane_of = {border[r]: ln for ln, r in enumerate(order)}     # comp -> lane
+        bcomp = [border[r] for r in order]                         # lane -> comp
+        # pattern in lane space; eliminate lanes in order 0..nb-1
+        Lpat = {(lane_of[i], lane_of[j]) for (i, j) in bpat}
+        filled, lower, upper = symbolic_lu(nb, Lpat, list(range(nb)))
+        self.wave = dict(nb=nb, nc=len(chains), bcomp=bcomp, chains=chains)
+        mch = max(len(ch[3]) for ch in chains)
+        out = [f"#define SBZ_NBORD {nb}", f"#define SBZ_NCHAIN {len(chains)}", f"#define SBZ_MCHAIN {mch}"]
+        out.append("__constant__ int sbz_bord_comp[SBZ_NBORD] = {" + ", ".join(map(str, bcomp)) + "};")
+        comps = []
+        for ch in chains:
+            comps += ch[3] + [-1] * (mch - len(ch[3]))
+        out.append("__constant__ int sbz_chain_comp[SBZ_NCHAIN * SBZ_MCHAIN] = {" + ", ".join(map(str, comps)) + "};")
+        out.append("__constant__ int sbz_chain_len[SBZ_NCHAIN] = {" + ", ".join(str(len(ch[3])) for ch in chains) + "};")
+        out.append("__constant__ int sbz_chain_att[SBZ_NCHAIN] = {" + ", ".join(str(lane_of[ch[4]]) for ch in chains) + "};")
+        out.append("__constant__ int sbz_chain_attc[SBZ_NCHAIN] = {" + ", ".join(str(ch[4]) for ch in chains) + "};")
+        out.append("__constant__ int sbz_chain_kind[SBZ_NCHAIN] = {" + ", ".join(str(ch[0]) for ch in chains) + "};")
+        out.append("__constant__ int sbz_chain_L[SBZ_NCHAIN] = {" + ", ".join(str(ch[1]) for ch in chains) + "};")
+        out.append("__constant__ int sbz_chain_bin[SBZ_NCHAIN] = {" + ", ".join(str(ch[2]) for ch in chains) + "};")
+        bch = [-1] * nb
+        for ci, ch in enumerate(chains):
+            bch[lane_of[ch[4]]] = ci
+        out.append("__constant__ int sbz_bord_chain[SBZ_NBORD] = {" + ", ".join(map(str, bch)) + "};")
+        # border RHS and ∂f/∂τ: every lane evaluates all border rows, keeps its own
+        assign = [(f"const double wf{j}", sysm.f[c]) for j, c in enumerate(bcomp)]
+        lines, _ = _emit_block(assign, "tbf")
+        out.append("template <class UI>")
+        out.append("SBZ_DEV double sbz_wb_f(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u) {")
+        out += lines
+        out.append("    double r = 0.0;")
+        out += [f"    r = (lane == {j}) ? wf{j} : r;" for j in range(nb)]
+        out.append("    return r;")
+        out.append("}")
+        dassign = []
+        for j, c in enumerate(bcomp):
+            fi = sysm.f[c]
+            e = sp.Integer(0)
+            for nm in B_NAMES:
+                if fi.has(sysm.B[nm]):
+                    e += sp.diff(fi, sysm.B[nm]) * sysm.dB[nm]
+            dassign.append((f"const double wd{j}", e))
+        lines, _ = _emit_block(dassign, "tbd")
+        out.append("template <class UI>")
+        out.append("SBZ_DEV double sbz_wb_dfdt(int lane, const double* __restrict__ B, const double* __restrict__ dB, const double* __restrict__ Cs, double k, UI u) {")
+        out += lines
+        out.append("    double r = 0.0;")
+        out += [f"    r = (lane == {j}) ? wd{j} : r;" for j in range(nb)]
+        out.append("    return r;")
+        out.append("}")
+        # W row of each border lane (dense over border lanes) and its coupling to the attached
+        # chain bottom; the chain's Schur complement xc = W[c0][p]/d'_0 is folded into the diagonal.
+        att_bottom = {lane_of[ch[4]]: ch[3][0] for ch in chains}
+        wassign = []
+        for (i, j) in sorted(filled):
+            ci, cj = bcomp[i], bcomp[j]
+            e = -self.J.get((ci, cj), sp.Integer(0))
+            if ci == cj:
+                e = sp.Symbol("ig") + e
+            wassign.append((i, j, e))
+        out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* row, double& cpl) {")
+        out.append("    cpl = 0.0;")
+        out.append("    switch (lane) {")
+        for i in range(nb):
+            items = [(f"row[{j}]", e) for (ii, j, e) in wassign if ii == i]
+            if i in att_bottom:
+                items.append(("cpl", -self.J.get((bcomp[i], att_bottom[i]), sp.Integer(0))))
+            lines, _ = _emit_block(items, f"tw{i}_")
+            out.append(f"    case {i}: {{")
+            out += ["    " + ln for ln in lines]
+            if i in att_bottom:
+                out.append(f"        row[{i}] -= cpl * xc;")
+            out.append("        break; }")
+        out.append("    default: break;")
+        out.append("    }")
+        out.append("}")
+        # row-parallel LU of the border block: lane r owns row r; pivots in lane order
+        lu = ["SBZ_DEV bool sbz_wb_lu(int lane, double* row, double* sP) {", "    bool ok = true;"]
+        for j in range(nb):
+            lo_mask = sum(1 << r for r in lower[j])
+            lu.append(f"    if (lane == {j}) {{ sP[{j}] = row[{j}];" + "".join(f" sP[{c}] = row[{c}];" for c in upper[j]) + " }")
+            lu.append("    __syncthreads();")
+            lu.append(f"    {{ const double piv = sP[{j}]; ok = ok && (piv != 0.0) && (piv == piv) && (fabs(piv) < 1e300);")
+            lu.append(f"      const double rp = 1.0 / piv;")
+            lu.append(f"      if (lane == {j}) row[{j}] = rp;")
+            if lower[j]:
+                lu.append(f"      if ((0x{lo_mask:x}ull >> lane) & 1ull) {{ const double l = row[{j}] * rp; row[{j}] = l;" +
+                          "".join(f" row[{c}] -= l * sP[{c}];" for c in upper[j]) + " }")
+            lu.append("    }")
+            lu.append("    __syncthreads();")
+        lu.append("    return ok;")
+        lu.append("}")
+        out += lu
+        # row-parallel triangular solves: lane r holds b_r; x_j broadcast with readlane
+        so = ["SBZ_DEV double sbz_wb_solve(int lane, const double* row, double b) {"]
+        for j in range(nb):
+            if lower[j]:
+                lo_mask = sum(1 << r for r in lower[j])
+                so.append(f"    {{ const double xj = sbz_readlane(b, {j}); if ((0x{lo_mask:x}ull >> lane) & 1ull) b -= row[{j}] * xj; }}")
+        ucol = {j: [r for r in range(nb) if j in upper[r]] for j in range(nb)}
+        for j in reversed(range(nb)):
+            so.append(f"    if (lane == {j}) b *= row[{j}];")
+            if ucol[j]:
+                up_mask = sum(1 << r for r in ucol[j])
+                so.append(f"    {{ const double xj = sbz_readlane(b, {j}); if ((0x{up_mask:x}ull >> lane) & 1ull) b -= row[{j}] * xj; }}")
+        so.append("    return b;")
+        so.append("}")
+        out += so
+        return out
+
+
+def wave_structure(M):
+    """Partition of the unknowns for the wavefront-per-mode kernel.
+
+    Chains: the interiors l = 3..L of every hierarchy (photon F and G, massless ν, each massive-ν
+    momentum bin).  Each is tridiagonal and couples to the rest only through its l = 2 member (its
+    attachment).  Border: every other unknown (Φ, l ≤ 2 multipoles, fluids).
+    Returns (chains, border) with chains = [(kind, L, bin, [comp for l = 3..L], attach_comp)];
+    kind 0: photon (damped by κ̇), 1: massless ν, 2: massive ν."""
+    Lt = layout(M)
+    chains = []
+    def add(kind, base, L, b=0):
+        chains.append((kind, L, b, [base + l for l in range(3, L + 1)], base + 2))
+    add(0, Lt["F"], M.lmax_g)
+    add(0, Lt["G"], M.lmax_g)
+    if M.has_nu:
+        add(1, Lt["N"], M.lmax_nu)
+    if M.has_h:
+        for i in range(M.nx):
+            add(2, Lt["H"] + i * (M.lmax_h + 1), M.lmax_h, i)
+    in_chain = {c for ch in chains for c in ch[3]}
+    border = [i for i in range(Lt["n"]) if i not in in_chain]
+    return chains, border
 
 
 def expand_includes(path, seen=None):

@@ -59,7 +59,11 @@ class _Desc(ctypes.Structure):
 
 class _Opts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiters", ctypes.c_longlong),
-                ("lanes", ctypes.c_int), ("save", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("save", ctypes.c_int), ("algo", ctypes.c_int)]
+
+
+ALGO_WAVE = 1
+ALGO_LANE = 2
 
 
 SAVE_STATES = 1
@@ -111,8 +115,8 @@ class GpuModel:
                                         _lib.ptr(P)), "sbz_bg_upload")
         self.n_sets = max(self.n_sets, set_index + 1)
 
-    def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0, save=0):
-        return Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save)
+    def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0, save=0, algo=0):
+        return Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save, algo)
 
     def close(self):
         if getattr(self, "h", None):
@@ -127,7 +131,7 @@ class GpuModel:
 
 
 class Plan:
-    def __init__(self, gm, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save=0):
+    def __init__(self, gm, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save=0, algo=0):
         self.gm = gm
         L = gm.L
         self.set = np.ascontiguousarray(inst_set, dtype=np.int32)
@@ -136,7 +140,7 @@ class Plan:
         self.n_inst = len(self.k)
         self.saveat = np.ascontiguousarray(saveat if saveat is not None else np.zeros(0), dtype=np.float64)
         self.n_save = len(self.saveat)
-        opts = _Opts(rtol, atol, int(maxiters), int(lanes), int(save))
+        opts = _Opts(rtol, atol, int(maxiters), int(lanes), int(save), int(algo))
         self.save = save if (save or self.n_save == 0) else SAVE_STATES
         h = ctypes.c_void_p()
         _lib.check(L.sbz_plan_create(gm.h, _lib.ptr(self.set, ctypes.c_int), _lib.ptr(self.k), _lib.ptr(self.t0),
