@@ -27,7 +27,7 @@ SBZ_DEV double sbz_readlane(double v, int l) {
     const long long x = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), l);
     const int hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
 }
 
 // @@SBZ_MODEL@@
@@ -41,6 +41,7 @@ SBZ_DEV double sbz_readlane(double v, int l) {
 // compiler can disambiguate (no false store→load aliasing), and a wavefront's access to one slot
 // is 64 consecutive doubles (512 B, fully coalesced).
 #define SBZ_WAVE 64
+#define SBZ_OWN ((SBZ_N + 63) / 64)   // state components per lane in the wave kernel
 #define SBZ_NB 25
 struct SV {
     double* p;
@@ -440,42 +441,39 @@ SBZ_DEV double sbz_pick8(const double* v, int i) {
     return r;
 }
 
-// chain coefficients of row l at background B: du_l = a u_{l-1} + d u_l + b u_{l+1}
-SBZ_DEV void sbz_chain_coef(int kind, int L, int bin, const double* B, const double* Cs, double k, int l,
-                            double& ca, double& cb, double& cd) {
-    const double s = (kind == 2) ? sbz_pick8(Cs, bin) / sbz_pick8(B + 16, bin) : 1.0;  // q = x/E for massive ν
-    const double dd = (kind == 0) ? B[3] : 0.0;                                           // κ̇ for photons
-    if (l < L) {
-        ca = s * k * l / (2.0 * l + 1.0);
-        cb = -s * k * (l + 1.0) / (2.0 * l + 1.0);
-        cd = dd;
-    } else {
-        ca = s * k;
-        cb = 0.0;
-        cd = dd - (L + 1.0) / B[0];
-    }
-}
-// and their τ-derivatives
-SBZ_DEV void sbz_chain_dcoef(int kind, int L, int bin, const double* B, const double* dB, const double* Cs, double k,
-                             int l, double& da, double& db, double& ddd) {
-    const double Eb = sbz_pick8(B + 16, bin);
-    const double sd = (kind == 2) ? -sbz_pick8(Cs, bin) * sbz_pick8(dB + 16, bin) / (Eb * Eb) : 0.0;
-    const double dd = (kind == 0) ? dB[3] : 0.0;
-    if (l < L) {
-        da = sd * k * l / (2.0 * l + 1.0);
-        db = -sd * k * (l + 1.0) / (2.0 * l + 1.0);
-        ddd = dd;
-    } else {
-        da = sd * k;
-        db = 0.0;
-        ddd = dd + (L + 1.0) / (B[0] * B[0]);
-    }
-}
-
+// Chain rows.  Row l of a chain (3 <= l <= L) reads  du_l = ca u_{l-1} + cd u_l + cb u_{l+1}  with
+//   l < L:  ca =  s k l/(2l+1),  cb = -s k (l+1)/(2l+1),  cd = dd
+//   l = L:  ca =  s k,           cb = 0,                  cd = dd - (L+1)/τ
+// (photons.jl:32-33,58-59, neutrinos.jl:25-26,131-132), where dd = κ̇ (B[3], negative in the
+// reference's convention) for photons and 0 otherwise, and s = x_i/E_i for massive-ν bin i
+// (1 for massless species).  The loops over l are unrolled, so l/(2l+1) and (l+1)/(2l+1) are
+// compile-time constants.  SbzChainRow holds the l-independent per-lane part at one time.
 struct WaveCtx {
-    int lane, isB, bcomp, isC, ckind, cL, cbin, clen, catt, cattc, bchain;
-    int ccomp[SBZ_MCHAIN];
+    int lane, isB, bcomp, isC, ckind, cL, cbin, clen, catt, cattc, bchain, cbase;
 };
+struct SbzChainRow {
+    double sk;   // s k
+    double dd;   // diagonal of the rows l < L
+    double top;  // diagonal of the row l = L
+};
+SBZ_DEV SbzChainRow sbz_chain_row(const WaveCtx& c, const double* B, const double* Cs, double k) {
+    SbzChainRow r;
+    r.sk = (c.ckind == 2) ? k * sbz_pick8(Cs, c.cbin) / sbz_pick8(B + 16, c.cbin) : k;
+    r.dd = (c.ckind == 0) ? B[3] : 0.0;
+    r.top = r.dd - (c.cL + 1.0) / B[0];
+    return r;
+}
+// the same coefficients differentiated in τ at fixed u
+SBZ_DEV SbzChainRow sbz_chain_drow(const WaveCtx& c, const double* B, const double* dB, const double* Cs, double k) {
+    SbzChainRow r;
+    const double E = sbz_pick8(B + 16, c.cbin);
+    r.sk = (c.ckind == 2) ? -k * sbz_pick8(Cs, c.cbin) * sbz_pick8(dB + 16, c.cbin) / (E * E) : 0.0;
+    r.dd = (c.ckind == 0) ? dB[3] : 0.0;
+    r.top = r.dd + (c.cL + 1.0) / (B[0] * B[0]);
+    return r;
+}
+#define SBZ_RL(e) ((3.0 + (e)) / (2.0 * (3.0 + (e)) + 1.0))    // l/(2l+1) at l = 3 + e
+#define SBZ_RL1(e) ((4.0 + (e)) / (2.0 * (3.0 + (e)) + 1.0))   // (l+1)/(2l+1) at l = 3 + e
 
 // out = J(τ) in, i.e. f(τ, in) (f is linear in u); with dB given, out = ∂f/∂τ at fixed `in`.
 // B, dB, Cs, in, out are LDS arrays; each lane writes only its own components of `out`.
@@ -484,16 +482,16 @@ SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB,
     const double fb = dB ? sbz_wb_dfdt(c.lane, B, dB, Cs, k, in) : sbz_wb_f(c.lane, B, Cs, k, in);
     if (c.isB) out[c.bcomp] = fb;
     if (c.isC) {
+        const SbzChainRow r = dB ? sbz_chain_drow(c, B, dB, Cs, k) : sbz_chain_row(c, B, Cs, k);
+        const double* ci = in + c.cbase;
+        double* co = out + c.cbase;
         double prev = in[c.cattc];
 #pragma unroll
         for (int e = 0; e < SBZ_MCHAIN; ++e) {
             if (e < c.clen) {
-                double ca, cb, cd;
-                if (dB) sbz_chain_dcoef(c.ckind, c.cL, c.cbin, B, dB, Cs, k, 3 + e, ca, cb, cd);
-                else sbz_chain_coef(c.ckind, c.cL, c.cbin, B, Cs, k, 3 + e, ca, cb, cd);
-                const double cur = in[c.ccomp[e]];
-                const double nxt = (e + 1 < c.clen) ? in[c.ccomp[e + 1]] : 0.0;
-                out[c.ccomp[e]] = ca * prev + cd * cur + cb * nxt;
+                const double cur = ci[e];
+                if (e + 1 < c.clen) co[e] = r.sk * SBZ_RL(e) * prev + r.dd * cur - r.sk * SBZ_RL1(e) * ci[e + 1];
+                else co[e] = r.sk * prev + r.top * cur;
                 prev = cur;
             }
         }
@@ -523,11 +521,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
     c.clen = sbz_chain_len[ci];
     c.catt = sbz_chain_att[ci];
     c.cattc = sbz_chain_attc[ci];
-#pragma unroll
-    for (int e = 0; e < SBZ_MCHAIN; ++e) {
-        const int cc = sbz_chain_comp[ci * SBZ_MCHAIN + e];
-        c.ccomp[e] = cc < 0 ? 0 : cc;
-    }
+    c.cbase = sbz_chain_base[ci];
     double* const DT = L.v[4];
     double* const US = L.v[5];
     double* const TMP = L.v[14];
@@ -615,7 +609,8 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
         }
         double row[SBZ_NBORD];
         double cpl = 0.0;
-        double cinv[SBZ_MCHAIN], cg[SBZ_MCHAIN], caw[SBZ_MCHAIN];
+        double cinv[SBZ_MCHAIN];   // reciprocal modified pivots of this lane's chain
+        double csk = 0.0;          // s k of this lane's chain at t_n
         int bidx = kidx;   // knot cursor of this lane's stage-time background
         long long iters = 0;
         double tprev = t, hprev = 0.0;  // the last accepted step, for dense output
@@ -673,6 +668,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             if (t + dt >= t1 - 1e-12 * fabs(t1)) dt = t1 - t;
             else if (dt < 1e-14 * fabs(t)) { ret = SBZ_RET_DTLESSTHANMIN; break; }
             const double ig = 1.0 / (SBZ_R5_GAMMA * dt);
+            const double idt = 1.0 / dt;
             // background at the six distinct stage times, one lane each (dual: d/dτ from lane 0)
             if (c.lane < 6) {
                 const double cl = (c.lane == 0) ? 0.0 : (c.lane == 1) ? SBZ_R5_C2 : (c.lane == 2) ? SBZ_R5_C3
@@ -688,25 +684,23 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             }
             __syncthreads();
             const double* B0 = L.B[0];
-            // chain factorization (top-down elimination) and its Schur term for the attachment
+            // chain factorization (top-down elimination from l = L) and its Schur term for the
+            // attachment: d'_e = W[e][e] - W[e][e+1] W[e+1][e] / d'_{e+1}, with W = I/(γh) - J
             if (c.isC) {
-                double dnext = 0.0, wup = 0.0;
+                const SbzChainRow r = sbz_chain_row(c, B0, L.Cs, k);
+                csk = r.sk;
+                double ca_next = 0.0;  // -W[e+1][e]
 #pragma unroll
                 for (int e = SBZ_MCHAIN - 1; e >= 0; --e) {
                     if (e < c.clen) {
-                        double ca, cb, cd;
-                        sbz_chain_coef(c.ckind, c.cL, c.cbin, B0, L.Cs, k, 3 + e, ca, cb, cd);
-                        const double wupe = -cb;  // W[e][e+1]
-                        double dp = ig - cd;
-                        if (e + 1 < c.clen) dp -= wupe * wup / dnext;  // wup: W[e+1][e]
+                        const bool last = (e + 1 == c.clen);
+                        double dp = ig - (last ? r.top : r.dd);
+                        if (!last) dp += r.sk * SBZ_RL1(e) * ca_next * cinv[e + 1];
                         cinv[e] = 1.0 / dp;
-                        cg[e] = (e + 1 < c.clen) ? wupe / dnext : 0.0;
-                        caw[e] = -ca;             // W[e][e-1]
-                        dnext = dp;
-                        wup = -ca;
+                        ca_next = last ? r.sk : r.sk * SBZ_RL(e);
                     }
                 }
-                L.C[c.lane] = caw[0] * cinv[0];
+                L.C[c.lane] = -ca_next * cinv[0];  // W[0][att] / d'_0
             }
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
@@ -728,10 +722,16 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                     } else {
                         double* in = (s <= 8) ? US : U1;
                         if (s <= 8) {
-                            for (int i = c.lane; i < SBZ_N; i += 64) {
-                                double acc = Un[i];
-                                for (int j = 1; j < s; ++j) acc += R5A[s][j - 1] * L.v[5 + j][i];
-                                US[i] = acc;
+#pragma unroll
+                            for (int r = 0; r < SBZ_OWN; ++r) {
+                                const int i = c.lane + 64 * r;
+                                if (i < SBZ_N) {
+                                    double acc = Un[i];
+#pragma unroll
+                                    for (int j = 1; j < 8; ++j)
+                                        if (j < s) acc += R5A[s][j - 1] * L.v[5 + j][i];
+                                    US[i] = acc;
+                                }
                             }
                         }
                         __syncthreads();
@@ -740,26 +740,34 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                         __syncthreads();
                         if (s == 9) break;
                         const double dd = dt * R5d[s];
-                        for (int i = c.lane; i < SBZ_N; i += 64) {
-                            double acc = Ks[i] + dd * DT[i];
-                            for (int j = 1; j < s; ++j) acc += (R5C[s][j - 1] / dt) * L.v[5 + j][i];
-                            Ks[i] = acc;
+#pragma unroll
+                        for (int r = 0; r < SBZ_OWN; ++r) {
+                            const int i = c.lane + 64 * r;
+                            if (i < SBZ_N) {
+                                double acc = Ks[i] + dd * DT[i];
+#pragma unroll
+                                for (int j = 1; j < 8; ++j)
+                                    if (j < s) acc += (R5C[s][j - 1] * idt) * L.v[5 + j][i];
+                                Ks[i] = acc;
+                            }
                         }
                     }
                     __syncthreads();
                     // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution
-                    double bp[SBZ_MCHAIN];
+                    // forward sweep b'_e = b_e - W[e][e+1] b'_{e+1} / d'_{e+1}, stored in place
                     if (c.isC) {
+                        double* kc = Ks + c.cbase;
                         double nxt = 0.0;
 #pragma unroll
                         for (int e = SBZ_MCHAIN - 1; e >= 0; --e) {
                             if (e < c.clen) {
-                                const double be = Ks[c.ccomp[e]] - cg[e] * nxt;
-                                bp[e] = be;
+                                double be = kc[e];
+                                if (e + 1 < c.clen) be -= csk * SBZ_RL1(e) * cinv[e + 1] * nxt;
+                                kc[e] = be;
                                 nxt = be;
                             }
                         }
-                        L.C[c.lane] = bp[0] * cinv[0];
+                        L.C[c.lane] = nxt * cinv[0];
                     }
                     __syncthreads();
                     {
@@ -772,13 +780,16 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                         }
                     }
                     __syncthreads();
+                    // back substitution x_e = (b'_e - W[e][e-1] x_{e-1}) / d'_e from the attachment
                     if (c.isC) {
+                        double* kc = Ks + c.cbase;
                         double prev = L.X[c.catt];
 #pragma unroll
                         for (int e = 0; e < SBZ_MCHAIN; ++e) {
                             if (e < c.clen) {
-                                const double x = (bp[e] - caw[e] * prev) * cinv[e];
-                                Ks[c.ccomp[e]] = x;
+                                const double ca = (e + 1 < c.clen) ? csk * SBZ_RL(e) : csk;
+                                const double x = (kc[e] + ca * prev) * cinv[e];
+                                kc[e] = x;
                                 prev = x;
                             }
                         }

@@ -613,10 +613,10 @@ class Emitted:
         mch = max(len(ch[3]) for ch in chains)
         out = [f"#define SBZ_NBORD {nb}", f"#define SBZ_NCHAIN {len(chains)}", f"#define SBZ_MCHAIN {mch}"]
         out.append("__constant__ int sbz_bord_comp[SBZ_NBORD] = {" + ", ".join(map(str, bcomp)) + "};")
-        comps = []
-        for ch in chains:
-            comps += ch[3] + [-1] * (mch - len(ch[3]))
-        out.append("__constant__ int sbz_chain_comp[SBZ_NCHAIN * SBZ_MCHAIN] = {" + ", ".join(map(str, comps)) + "};")
+        for ch in chains:   # the kernel addresses chain members as base + (l - 3)
+            if ch[3] != list(range(ch[3][0], ch[3][0] + len(ch[3]))):
+                raise ValueError("wave kernel needs every chain stored contiguously")
+        out.append("__constant__ int sbz_chain_base[SBZ_NCHAIN] = {" + ", ".join(str(ch[3][0]) for ch in chains) + "};")
         out.append("__constant__ int sbz_chain_len[SBZ_NCHAIN] = {" + ", ".join(str(len(ch[3])) for ch in chains) + "};")
         out.append("__constant__ int sbz_chain_att[SBZ_NCHAIN] = {" + ", ".join(str(lane_of[ch[4]]) for ch in chains) + "};")
         out.append("__constant__ int sbz_chain_attc[SBZ_NCHAIN] = {" + ", ".join(str(ch[4]) for ch in chains) + "};")
@@ -725,6 +725,8 @@ def wave_structure(M):
     Lt = layout(M)
     chains = []
     def add(kind, base, L, b=0):
+        if L < 3:      # no interior: the whole hierarchy stays in the border
+            return
         chains.append((kind, L, b, [base + l for l in range(3, L + 1)], base + 2))
     add(0, Lt["F"], M.lmax_g)
     add(0, Lt["G"], M.lmax_g)

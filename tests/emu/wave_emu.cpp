@@ -1,0 +1,177 @@
+// Host emulation of the wavefront-per-mode kernel (sbz_wave_kernel) — TEST INFRASTRUCTURE ONLY.
+//
+// The emitted kernel source is compiled unchanged for the CPU.  One 64-lane workgroup is
+// emulated with 64 cooperative contexts (ucontext) on one OS thread: every __syncthreads, lane
+// exchange (readlane, shfl_xor) and vote (__all) is a real barrier at which all 64 lanes must
+// meet, and the emulator aborts if lanes arrive at different barrier sites or some lanes exit
+// while others wait (non-uniform control flow around a barrier).  This checks the kernel's
+// arithmetic, its cross-lane communication and, built with AddressSanitizer, its indexing of the
+// global buffers without a GPU.
+#include <ucontext.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#define __HIPCC_RTC__ 1  // the kernel then skips <hip/hip_runtime.h>
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __restrict__ __restrict
+#define __launch_bounds__(x)
+#define __constant__ static
+#define __shared__ static
+
+struct EmuDim3 {
+    unsigned x = 0, y = 0, z = 0;
+};
+static EmuDim3 threadIdx, blockIdx, blockDim{64, 1, 1}, gridDim{1, 1, 1};
+
+namespace emu {
+constexpr int W = 64;
+constexpr size_t STACK = 1 << 20;
+static ucontext_t sched_ctx, lane_ctx[W];
+static std::vector<char> stacks;
+static int cur = 0;
+static bool done[W];
+static int site[W];   // source line of the barrier each lane waits at
+static double xd[2][W];   // exchange buffers, alternated per exchange (one barrier suffices)
+static long long xi[2][W];
+static int gen[W];
+static void (*body)(void*) = nullptr;
+static void* body_arg = nullptr;
+static char error[256];
+
+static void barrier_at(int line) {
+    site[cur] = line;
+    swapcontext(&lane_ctx[cur], &sched_ctx);
+}
+static void syncthreads(int line) { barrier_at(line); }
+static int readlane_i(int v, int l, int s) {
+    const int g = gen[cur] ^= 1;
+    xi[g][cur] = v;
+    barrier_at(s);
+    return (int)xi[g][l];
+}
+static double shfl_xor_d(double v, int off, int s) {
+    const int g = gen[cur] ^= 1;
+    xd[g][cur] = v;
+    barrier_at(s);
+    return xd[g][cur ^ off];
+}
+static int all(int p, int s) {
+    const int g = gen[cur] ^= 1;
+    xi[g][cur] = p ? 1 : 0;
+    barrier_at(s);
+    int r = 1;
+    for (int l = 0; l < W; ++l) r &= (int)xi[g][l];
+    return r;
+}
+static void lane_main() {
+    body(body_arg);
+    done[cur] = true;
+    swapcontext(&lane_ctx[cur], &sched_ctx);
+}
+// run body(arg) on 64 lanes; returns 0 or -1 (message in `error`)
+static int run(void (*fn)(void*), void* arg) {
+    body = fn;
+    body_arg = arg;
+    stacks.assign(STACK * W, 0);
+    for (int l = 0; l < W; ++l) {
+        done[l] = false;
+        gen[l] = 0;
+        getcontext(&lane_ctx[l]);
+        lane_ctx[l].uc_stack.ss_sp = stacks.data() + STACK * l;
+        lane_ctx[l].uc_stack.ss_size = STACK;
+        lane_ctx[l].uc_link = nullptr;
+        makecontext(&lane_ctx[l], lane_main, 0);
+    }
+    for (;;) {
+        for (int l = 0; l < W; ++l) {
+            cur = l;
+            threadIdx.x = l;
+            site[l] = -1;
+            swapcontext(&sched_ctx, &lane_ctx[l]);
+        }
+        int nd = 0;
+        for (int l = 0; l < W; ++l) nd += done[l];
+        if (nd == W) return 0;
+        if (nd != 0) {
+            snprintf(error, sizeof error, "%d lanes exited while %d wait at a barrier", nd, W - nd);
+            return -1;
+        }
+        for (int l = 1; l < W; ++l)
+            if (site[l] != site[0]) {
+                snprintf(error, sizeof error, "lanes 0 and %d wait at different barriers (source lines %d, %d)",
+                         l, site[0], site[l]);
+                return -1;
+            }
+    }
+}
+}  // namespace emu
+
+#define __syncthreads() emu::syncthreads(__LINE__)
+#define __builtin_amdgcn_readlane(v, l) emu::readlane_i((v), (l), __LINE__)
+#define __shfl_xor(v, off, w) emu::shfl_xor_d((v), (off), __LINE__)
+#define __all(p) emu::all((p), __LINE__)
+static inline unsigned atomicAdd(unsigned* p, unsigned v) {
+    const unsigned o = *p;
+    *p = o + v;
+    return o;
+}
+static inline long long __double_as_longlong(double d) {
+    long long r;
+    std::memcpy(&r, &d, 8);
+    return r;
+}
+static inline double __longlong_as_double(long long x) {
+    double r;
+    std::memcpy(&r, &x, 8);
+    return r;
+}
+
+#include "emu_kernel.inc"
+
+static void wave_body(void* a) { sbz_wave_kernel(*(SbzSolveArgs*)a); }
+
+extern "C" const char* emu_error(void) { return emu::error; }
+extern "C" int emu_n(void) { return SBZ_N; }
+
+// Solve n_inst instances with the wave kernel on one emulated workgroup.  The knot table of every
+// set is laid out as the runtime uploads it: t[nk], u[nk][5], du[nk][5] at knot_off[set].
+extern "C" int emu_wave_solve(const double* params, const double* knots, const long long* knot_off,
+                              const int* knot_n, const int* inst_set, const double* inst_k,
+                              const double* inst_t0, int n_inst, double rtol, double atol, long long maxiters,
+                              const double* saveat, int n_save, double* out_u, double* out_save, double* out_src,
+                              int* retcode, long long* stats) {
+    unsigned queue = 0;
+    SbzSolveArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.params = params;
+    a.knots = knots;
+    a.knot_off = knot_off;
+    a.knot_n = knot_n;
+    a.inst_set = inst_set;
+    a.inst_k = inst_k;
+    a.inst_t0 = inst_t0;
+    a.order = nullptr;
+    a.n_inst = n_inst;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.maxiters = maxiters;
+    a.saveat = n_save ? saveat : nullptr;
+    a.n_save = n_save;
+    a.out_u = out_u;
+    a.out_save = out_save;
+    a.out_src = out_src;
+    a.retcode = retcode;
+    a.stats = stats;
+    a.queue = &queue;
+    a.ws = nullptr;
+    a.S = 1;
+    return emu::run(wave_body, &a);
+}

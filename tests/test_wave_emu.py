@@ -1,0 +1,45 @@
+"""The wavefront-per-mode kernel (sbz_wave_kernel) run on the CPU through the lane emulator
+(tests/emu) against the oracle: final states, saved states and CMB sources ≤ 1e-4 relative, and
+the failure return codes (k = 0, k = NaN give InitialFailure, as the lane-per-mode kernel)."""
+import numpy as np
+import pytest
+
+from conftest import Setup
+from emu.emu import WaveEmu
+from symboltz import models
+from symboltz.solver import model_source
+
+
+def rel(a, b):
+    sc = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)
+    return np.max(np.abs(a - b) / sc)
+
+
+def test_wave_kernel_emulated_matches_oracle(fixture5):
+    S = fixture5
+    E = WaveEmu(model_source(S.M).full_src)
+    ks = np.array([0.3, 30.0, 0.0, np.nan])
+    t0 = np.clip(np.minimum(1e-2 / np.where(ks > 0, ks, 1.0), 1e-4), S.t[0], S.t[-1])
+    saveat = np.linspace(0.05, S.t[-1], 9)
+    g = E.solve(S.P, S.t, S.u, S.du, ks, t0, saveat=saveat, sources=True)
+    o = S.O.solve(ks[:2], t0[:2], saveat=saveat)
+    assert list(g["retcode"]) == [0, 0, 4, 4]
+    assert rel(g["u"][:2], o["u"]) < 1e-4
+    for i in range(2):
+        assert rel(g["usave"][i], o["usave"][i]) < 1e-4
+        ref = np.array([S.O.sources(ks[i], tau, o["usave"][i, j]) for j, tau in enumerate(saveat)])
+        for c in range(3):
+            assert np.max(np.abs(g["src"][i, :, c] - ref[:, c])) <= 1e-4 * (np.abs(ref[:, c]).max() + 1e-300)
+    assert np.all(np.abs(g["stats"][:2, 0] - o["stats"][:, 0]) <= 0.05 * o["stats"][:, 0])
+
+
+def test_wave_kernel_emulated_massive_nu_lmax16():
+    M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
+    S = Setup(M, models.parameters_Planck18(M))
+    E = WaveEmu(model_source(M).full_src)
+    ks = np.array([0.5])
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g = E.solve(S.P, S.t, S.u, S.du, ks, t0)
+    o = S.O.solve(ks, t0)
+    assert g["retcode"][0] == 0
+    assert rel(g["u"], o["u"]) < 1e-4
