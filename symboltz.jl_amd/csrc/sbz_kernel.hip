@@ -483,17 +483,21 @@ SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB,
     if (c.isB) out[c.bcomp] = fb;
     if (c.isC) {
         const SbzChainRow r = dB ? sbz_chain_drow(c, B, dB, Cs, k) : sbz_chain_row(c, B, Cs, k);
+        // branch-free over the longest chain: all loads first (clamped to the chain), then the
+        // rows, then stores predicated on e < clen
         const double* ci = in + c.cbase;
         double* co = out + c.cbase;
-        double prev = in[c.cattc];
+        double v[SBZ_MCHAIN + 1];
+        v[0] = in[c.cattc];
+#pragma unroll
+        for (int e = 0; e < SBZ_MCHAIN; ++e) v[e + 1] = ci[e < c.clen ? e : c.clen - 1];
 #pragma unroll
         for (int e = 0; e < SBZ_MCHAIN; ++e) {
-            if (e < c.clen) {
-                const double cur = ci[e];
-                if (e + 1 < c.clen) co[e] = r.sk * SBZ_RL(e) * prev + r.dd * cur - r.sk * SBZ_RL1(e) * ci[e + 1];
-                else co[e] = r.sk * prev + r.top * cur;
-                prev = cur;
-            }
+            const bool last = (e + 1 == c.clen);
+            const double nxt = (e + 1 < SBZ_MCHAIN) ? v[e + 2] : 0.0;
+            const double o = last ? r.sk * v[e] + r.top * v[e + 1]
+                                  : r.sk * SBZ_RL(e) * v[e] + r.dd * v[e + 1] - r.sk * SBZ_RL1(e) * nxt;
+            if (e < c.clen) co[e] = o;
         }
     }
 }
@@ -692,13 +696,11 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                 double ca_next = 0.0;  // -W[e+1][e]
 #pragma unroll
                 for (int e = SBZ_MCHAIN - 1; e >= 0; --e) {
-                    if (e < c.clen) {
-                        const bool last = (e + 1 == c.clen);
-                        double dp = ig - (last ? r.top : r.dd);
-                        if (!last) dp += r.sk * SBZ_RL1(e) * ca_next * cinv[e + 1];
-                        cinv[e] = 1.0 / dp;
-                        ca_next = last ? r.sk : r.sk * SBZ_RL(e);
-                    }
+                    const bool on = e < c.clen, last = (e + 1 == c.clen);
+                    double dp = ig - (last ? r.top : r.dd);
+                    if (e + 1 < SBZ_MCHAIN && !last) dp += r.sk * SBZ_RL1(e) * ca_next * cinv[e + 1];
+                    cinv[e] = on ? 1.0 / dp : 0.0;
+                    ca_next = on ? (last ? r.sk : r.sk * SBZ_RL(e)) : 0.0;
                 }
                 L.C[c.lane] = -ca_next * cinv[0];  // W[0][att] / d'_0
             }
@@ -757,16 +759,21 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                     // forward sweep b'_e = b_e - W[e][e+1] b'_{e+1} / d'_{e+1}, stored in place
                     if (c.isC) {
                         double* kc = Ks + c.cbase;
+                        double bv[SBZ_MCHAIN];
+#pragma unroll
+                        for (int e = 0; e < SBZ_MCHAIN; ++e) bv[e] = kc[e < c.clen ? e : c.clen - 1];
                         double nxt = 0.0;
 #pragma unroll
                         for (int e = SBZ_MCHAIN - 1; e >= 0; --e) {
-                            if (e < c.clen) {
-                                double be = kc[e];
-                                if (e + 1 < c.clen) be -= csk * SBZ_RL1(e) * cinv[e + 1] * nxt;
-                                kc[e] = be;
-                                nxt = be;
-                            }
+                            double be = bv[e];
+                            if (e + 1 < SBZ_MCHAIN) be -= csk * SBZ_RL1(e) * cinv[e + 1] * nxt;  // cinv = 0 past the chain
+                            be = (e < c.clen) ? be : 0.0;
+                            bv[e] = be;
+                            nxt = be;
                         }
+#pragma unroll
+                        for (int e = 0; e < SBZ_MCHAIN; ++e)
+                            if (e < c.clen) kc[e] = bv[e];
                         L.C[c.lane] = nxt * cinv[0];
                     }
                     __syncthreads();
@@ -783,16 +790,19 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                     // back substitution x_e = (b'_e - W[e][e-1] x_{e-1}) / d'_e from the attachment
                     if (c.isC) {
                         double* kc = Ks + c.cbase;
+                        double bv[SBZ_MCHAIN];
+#pragma unroll
+                        for (int e = 0; e < SBZ_MCHAIN; ++e) bv[e] = kc[e < c.clen ? e : c.clen - 1];
                         double prev = L.X[c.catt];
 #pragma unroll
                         for (int e = 0; e < SBZ_MCHAIN; ++e) {
-                            if (e < c.clen) {
-                                const double ca = (e + 1 < c.clen) ? csk * SBZ_RL(e) : csk;
-                                const double x = (kc[e] + ca * prev) * cinv[e];
-                                kc[e] = x;
-                                prev = x;
-                            }
+                            const double ca = (e + 1 == c.clen) ? csk : csk * SBZ_RL(e);
+                            prev = (bv[e] + ca * prev) * cinv[e];
+                            bv[e] = prev;
                         }
+#pragma unroll
+                        for (int e = 0; e < SBZ_MCHAIN; ++e)
+                            if (e < c.clen) kc[e] = bv[e];
                     }
                     __syncthreads();
                     if (s == 8) {
