@@ -426,7 +426,7 @@ struct WaveLDS {
     double B[7][SBZ_NB];           // background at the distinct stage times (0..5) and a spare slot (6)
     double dB[2][SBZ_NB];          // d/dτ of the background at t_n (0) and at the spare time (1)
     double Cs[25];                 // per-set constants (momentum nodes, weights, d ln f0/d ln x, c_s²)
-    double P[SBZ_NBORD];           // pivot-row broadcast
+    double W[SBZ_NBORD * SBZ_WSTRIDE];  // border block of W, staged for the row loads
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
     double ddkd;
@@ -706,10 +706,8 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             }
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
-#pragma unroll
-            for (int q = 0; q < SBZ_NBORD; ++q) row[q] = 0.0;
-            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, row, cpl);
-            bool ok = sbz_wb_lu(c.lane, row, L.P);
+            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.W, row, cpl);
+            bool ok = sbz_wb_lu(c.lane, row);
             ok = __all(ok || !c.isB);
             sbz_wave_apply(c, B0, L.dB[0], L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
             st_j++;

@@ -653,9 +653,13 @@ class Emitted:
         out += [f"    r = (lane == {j}) ? wd{j} : r;" for j in range(nb)]
         out.append("    return r;")
         out.append("}")
-        # W row of each border lane (dense over border lanes) and its coupling to the attached
-        # chain bottom; the chain's Schur complement xc = W[c0][p]/d'_0 is folded into the diagonal.
+        # W of the border block, computed once by the whole wavefront (one CSE over all entries,
+        # so subexpressions shared between rows are evaluated once) and staged through LDS, from
+        # where every lane loads its own row.  Column nb of a row holds its coupling W[p][c0] to
+        # the bottom of an attached chain; the chain's Schur complement xc = W[c0][p]/d'_0 is then
+        # folded into the diagonal.
         att_bottom = {lane_of[ch[4]]: ch[3][0] for ch in chains}
+        ws = nb + 1 + (nb + 1) % 2          # odd row stride in doubles: conflict-free row loads
         wassign = []
         for (i, j) in sorted(filled):
             ci, cj = bcomp[i], bcomp[j]
@@ -663,51 +667,95 @@ class Emitted:
             if ci == cj:
                 e = sp.Symbol("ig") + e
             wassign.append((i, j, e))
-        out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* row, double& cpl) {")
-        out.append("    cpl = 0.0;")
-        out.append("    switch (lane) {")
-        for i in range(nb):
-            items = [(f"row[{j}]", e) for (ii, j, e) in wassign if ii == i]
-            if i in att_bottom:
-                items.append(("cpl", -self.J.get((bcomp[i], att_bottom[i]), sp.Integer(0))))
-            lines, _ = _emit_block(items, f"tw{i}_")
-            out.append(f"    case {i}: {{")
-            out += ["    " + ln for ln in lines]
-            if i in att_bottom:
-                out.append(f"        row[{i}] -= cpl * xc;")
-            out.append("        break; }")
-        out.append("    default: break;")
+        items = [(f"sW[{i * ws + j}]", e) for (i, j, e) in wassign if e != 0]
+        for i in sorted(att_bottom):
+            items.append((f"sW[{i * ws + nb}]", -self.J.get((bcomp[i], att_bottom[i]), sp.Integer(0))))
+        lines, _ = _emit_block(items, "tw")
+        out.append(f"#define SBZ_WSTRIDE {ws}")
+        out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* sW, double* row, double& cpl) {")
+        out.append("    if (lane == 0) {")
+        out += ["    " + ln for ln in lines]
         out.append("    }")
-        out.append("}")
-        # row-parallel LU of the border block: lane r owns row r; pivots in lane order
-        lu = ["SBZ_DEV bool sbz_wb_lu(int lane, double* row, double* sP) {", "    bool ok = true;"]
+        out.append("    __syncthreads();")
+        zero = {(i, j) for (i, j, e) in wassign if e == 0}
+        out.append("    const int r = lane < SBZ_NBORD ? lane : 0;")
+        out.append("    const double* w = sW + r * SBZ_WSTRIDE;")
         for j in range(nb):
-            lo_mask = sum(1 << r for r in lower[j])
-            lu.append(f"    if (lane == {j}) {{ sP[{j}] = row[{j}];" + "".join(f" sP[{c}] = row[{c}];" for c in upper[j]) + " }")
-            lu.append("    __syncthreads();")
-            lu.append(f"    {{ const double piv = sP[{j}]; ok = ok && (piv != 0.0) && (piv == piv) && (fabs(piv) < 1e300);")
-            lu.append(f"      const double rp = 1.0 / piv;")
-            lu.append(f"      if (lane == {j}) row[{j}] = rp;")
-            if lower[j]:
-                lu.append(f"      if ((0x{lo_mask:x}ull >> lane) & 1ull) {{ const double l = row[{j}] * rp; row[{j}] = l;" +
-                          "".join(f" row[{c}] -= l * sP[{c}];" for c in upper[j]) + " }")
+            col = [i for (i, jj) in filled if jj == j and (i, j) not in zero]
+            if len(col) == 0:
+                out.append(f"    row[{j}] = 0.0;")
+            elif len(col) == nb:
+                out.append(f"    row[{j}] = w[{j}];")
+            else:
+                m = sum(1 << i for i in col)
+                out.append(f"    row[{j}] = ((0x{m:x}ull >> r) & 1ull) ? w[{j}] : 0.0;")
+        am = sum(1 << i for i in att_bottom)
+        out.append(f"    cpl = ((0x{am:x}ull >> r) & 1ull) ? w[SBZ_NBORD] : 0.0;")
+        for i in sorted(att_bottom):
+            out.append(f"    if (lane == {i}) row[{i}] -= cpl * xc;")
+        out.append("    __syncthreads();   // sW is reused by the next step")
+        out.append("}")
+        # row-parallel LU of the border block (lane r owns row r, pivots in lane order), level
+        # scheduled: pivots whose rows and columns do not depend on each other are eliminated in
+        # the same round; pivot rows are broadcast with readlane (no LDS round trip)
+        lev = [0] * nb
+        for j in range(nb):
+            deps = [q for q in range(j) if (j, q) in filled or (q, j) in filled]
+            lev[j] = 1 + max((lev[q] for q in deps), default=-1)
+        nlev = max(lev) + 1
+        ucol = {j: [r for r in range(nb) if j in upper[r]] for j in range(nb)}
+        levb = [0] * nb
+        for j in reversed(range(nb)):
+            levb[j] = 1 + max((levb[c] for c in upper[j]), default=-1)
+        nlevb = max(levb) + 1
+        self.wave.update(lu_rounds=nlev, bwd_rounds=nlevb)
+        lu = ["SBZ_DEV bool sbz_wb_lu(int lane, double* row) {", "    bool ok = true;"]
+        for L_ in range(nlev):
+            piv = [j for j in range(nb) if lev[j] == L_]
+            lu.append(f"    {{  // round {L_}: pivots {piv}")
+            for j in piv:
+                lu.append(f"        const double p{j} = sbz_readlane(row[{j}], {j});")
+                for c in upper[j]:
+                    lu.append(f"        const double u{j}_{c} = sbz_readlane(row[{c}], {j});")
+            for j in piv:
+                lu.append(f"        const double rp{j} = 1.0 / p{j};")
+                lu.append(f"        ok = ok && (p{j} != 0.0) && (p{j} == p{j}) && (fabs(p{j}) < 1e300);")
+            for j in piv:
+                lu.append(f"        if (lane == {j}) row[{j}] = rp{j};")
+                if lower[j]:
+                    m = sum(1 << r for r in lower[j])
+                    lu.append(f"        if ((0x{m:x}ull >> lane) & 1ull) {{ const double l = row[{j}] * rp{j}; row[{j}] = l;" +
+                              "".join(f" row[{c}] -= l * u{j}_{c};" for c in upper[j]) + " }")
             lu.append("    }")
-            lu.append("    __syncthreads();")
         lu.append("    return ok;")
         lu.append("}")
         out += lu
-        # row-parallel triangular solves: lane r holds b_r; x_j broadcast with readlane
+        # row-parallel triangular solves, level scheduled: lane r holds b_r; x_j broadcast with readlane
         so = ["SBZ_DEV double sbz_wb_solve(int lane, const double* row, double b) {"]
-        for j in range(nb):
-            if lower[j]:
-                lo_mask = sum(1 << r for r in lower[j])
-                so.append(f"    {{ const double xj = sbz_readlane(b, {j}); if ((0x{lo_mask:x}ull >> lane) & 1ull) b -= row[{j}] * xj; }}")
-        ucol = {j: [r for r in range(nb) if j in upper[r]] for j in range(nb)}
-        for j in reversed(range(nb)):
-            so.append(f"    if (lane == {j}) b *= row[{j}];")
-            if ucol[j]:
-                up_mask = sum(1 << r for r in ucol[j])
-                so.append(f"    {{ const double xj = sbz_readlane(b, {j}); if ((0x{up_mask:x}ull >> lane) & 1ull) b -= row[{j}] * xj; }}")
+        for L_ in range(nlev):
+            piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
+            if not piv:
+                continue
+            so.append("    {")
+            for j in piv:
+                so.append(f"        const double x{j} = sbz_readlane(b, {j});")
+            for j in piv:
+                m = sum(1 << r for r in lower[j])
+                so.append(f"        if ((0x{m:x}ull >> lane) & 1ull) b -= row[{j}] * x{j};")
+            so.append("    }")
+        for L_ in range(nlevb):
+            piv = [j for j in range(nb) if levb[j] == L_]
+            so.append("    {")
+            for j in piv:
+                so.append(f"        if (lane == {j}) b *= row[{j}];")
+            for j in piv:
+                if ucol[j]:
+                    so.append(f"        const double x{j} = sbz_readlane(b, {j});")
+            for j in piv:
+                if ucol[j]:
+                    m = sum(1 << r for r in ucol[j])
+                    so.append(f"        if ((0x{m:x}ull >> lane) & 1ull) b -= row[{j}] * x{j};")
+            so.append("    }")
         so.append("    return b;")
         so.append("}")
         out += so
