@@ -426,7 +426,7 @@ struct WaveLDS {
     double B[7][SBZ_NB];           // background at the distinct stage times (0..5) and a spare slot (6)
     double dB[2][SBZ_NB];          // d/dτ of the background at t_n (0) and at the spare time (1)
     double Cs[25];                 // per-set constants (momentum nodes, weights, d ln f0/d ln x, c_s²)
-    double W[SBZ_NBORD * SBZ_WSTRIDE];  // border block of W, staged for the row loads
+    double M[2][SBZ_NBORD * SBZ_WSTRIDE];  // border block: W staging (0), then its L (0) and U (1) factors
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
     double ddkd;
@@ -571,6 +571,9 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             }
             if (!__all(fin)) ret = SBZ_RET_INITIALFAILURE;
         }
+        for (int i = c.lane; i < SBZ_N; i += 64)
+#pragma unroll
+            for (int j = 6; j <= 13; ++j) L.v[j][i] = 0.0;   // stage vectors start finite (LDS is not initialised)
         double dt = 0.0, qold = SBZ_CTL_QOLDINIT;
         if (ret == SBZ_RET_SUCCESS && t1 > t) {
             // Hairer–Wanner initial step: f(t, u0) and f(t + dt0, u0 + dt0 f0)
@@ -706,8 +709,10 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             }
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
-            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.W, row, cpl);
-            bool ok = sbz_wb_lu(c.lane, row);
+            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl);
+            double rpo;   // reciprocal pivot of this lane's row
+            bool ok = sbz_wb_lu(c.lane, row, rpo);
+            sbz_wb_store(c.lane, row, rpo, L.M[0], L.M[1]);
             ok = __all(ok || !c.isB);
             sbz_wave_apply(c, B0, L.dB[0], L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
             st_j++;
@@ -728,8 +733,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                                 if (i < SBZ_N) {
                                     double acc = Un[i];
 #pragma unroll
-                                    for (int j = 1; j < 8; ++j)
-                                        if (j < s) acc += R5A[s][j - 1] * L.v[5 + j][i];
+                                    for (int j = 1; j < 8; ++j) acc += R5A[s][j - 1] * L.v[5 + j][i];
                                     US[i] = acc;
                                 }
                             }
@@ -746,8 +750,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                             if (i < SBZ_N) {
                                 double acc = Ks[i] + dd * DT[i];
 #pragma unroll
-                                for (int j = 1; j < 8; ++j)
-                                    if (j < s) acc += (R5C[s][j - 1] * idt) * L.v[5 + j][i];
+                                for (int j = 1; j < 8; ++j) acc += (R5C[s][j - 1] * idt) * L.v[5 + j][i];
                                 Ks[i] = acc;
                             }
                         }
@@ -778,7 +781,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                     {
                         double b = c.isB ? Ks[c.bcomp] : 0.0;
                         if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                        const double x = sbz_wb_solve(c.lane, row, b);
+                        const double x = sbz_wb_solve(c.lane, L.M[0], L.M[1], b);
                         if (c.isB) {
                             Ks[c.bcomp] = x;
                             L.X[c.lane] = x;
@@ -826,6 +829,14 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                 const double q11 = (EEst < __builtin_inf()) ? pow(EEst, SBZ_CTL_BETA1) : 1.0 / SBZ_CTL_QMIN;
                 dt /= fmin(1.0 / SBZ_CTL_QMIN, q11 / SBZ_CTL_GAMMA);
                 st_rej++;
+                // the stage combinations read every stage vector with zero weights past the
+                // stage, so a non-finite attempt must not leave non-finite stages behind
+                if (!(EEst < __builtin_inf())) {
+                    for (int i = c.lane; i < SBZ_N; i += 64)
+#pragma unroll
+                        for (int j = 6; j <= 13; ++j) L.v[j][i] = 0.0;
+                    __syncthreads();
+                }
                 continue;
             }
             const double q11 = pow(EEst, SBZ_CTL_BETA1);

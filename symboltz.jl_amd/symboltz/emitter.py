@@ -426,18 +426,40 @@ def count_ops(exprs):
     return flops, divs, special
 
 
-def _emit_block(assign, tmp_prefix, opt=True):
-    """CSE a list of (lhs_code, expr) and return C lines plus the op count."""
+def _emit_block(assign, tmp_prefix, opt=True, interleave=False):
+    """CSE a list of (lhs_code, expr) and return C lines plus the op count.
+
+    With interleave, each output follows directly the temporaries it is the first to need, so
+    temporaries die early (lower register pressure for long blocks that store their outputs)."""
     exprs = [e for _, e in assign]
     if opt:
         repl, red = sp.cse(exprs, symbols=sp.numbered_symbols(tmp_prefix), optimizations="basic")
     else:
         repl, red = [], exprs
     lines = []
-    for s, e in repl:
-        lines.append(f"    const double {s} = {cprint(e)};")
-    for (lhs, _), e in zip(assign, red):
-        lines.append(f"    {lhs} = {cprint(e)};")
+    if not interleave:
+        for s, e in repl:
+            lines.append(f"    const double {s} = {cprint(e)};")
+        for (lhs, _), e in zip(assign, red):
+            lines.append(f"    {lhs} = {cprint(e)};")
+    else:
+        defs = {s: e for s, e in repl}
+        pos = {s: i for i, (s, _) in enumerate(repl)}
+        done = set()
+
+        def need(e, acc):
+            for sym in e.free_symbols:
+                if sym in defs and sym not in done and sym not in acc:
+                    acc.add(sym)
+                    need(defs[sym], acc)
+
+        for (lhs, _), e in zip(assign, red):
+            acc = set()
+            need(e, acc)
+            for sym in sorted(acc, key=pos.get):
+                lines.append(f"    const double {sym} = {cprint(defs[sym])};")
+                done.add(sym)
+            lines.append(f"    {lhs} = {cprint(e)};")
     ops = count_ops([e for _, e in repl] + list(red))
     return lines, ops
 
@@ -670,7 +692,7 @@ class Emitted:
         items = [(f"sW[{i * ws + j}]", e) for (i, j, e) in wassign if e != 0]
         for i in sorted(att_bottom):
             items.append((f"sW[{i * ws + nb}]", -self.J.get((bcomp[i], att_bottom[i]), sp.Integer(0))))
-        lines, _ = _emit_block(items, "tw")
+        lines, _ = _emit_block(items, "tw", interleave=True)
         out.append(f"#define SBZ_WSTRIDE {ws}")
         out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* sW, double* row, double& cpl) {")
         out.append("    if (lane == 0) {")
@@ -691,8 +713,8 @@ class Emitted:
                 out.append(f"    row[{j}] = ((0x{m:x}ull >> r) & 1ull) ? w[{j}] : 0.0;")
         am = sum(1 << i for i in att_bottom)
         out.append(f"    cpl = ((0x{am:x}ull >> r) & 1ull) ? w[SBZ_NBORD] : 0.0;")
-        for i in sorted(att_bottom):
-            out.append(f"    if (lane == {i}) row[{i}] -= cpl * xc;")
+        for i in sorted(att_bottom):   # selects on values, never on row positions: row stays in registers
+            out.append(f"    row[{i}] -= (lane == {i}) ? cpl * xc : 0.0;")
         out.append("    __syncthreads();   // sW is reused by the next step")
         out.append("}")
         # row-parallel LU of the border block (lane r owns row r, pivots in lane order), level
@@ -709,29 +731,46 @@ class Emitted:
             levb[j] = 1 + max((levb[c] for c in upper[j]), default=-1)
         nlevb = max(levb) + 1
         self.wave.update(lu_rounds=nlev, bwd_rounds=nlevb)
-        lu = ["SBZ_DEV bool sbz_wb_lu(int lane, double* row) {", "    bool ok = true;"]
+        lu = ["SBZ_DEV bool sbz_wb_lu(int lane, double* row, double& rpo) {", "    bool ok = true;", "    rpo = 0.0;"]
         for L_ in range(nlev):
             piv = [j for j in range(nb) if lev[j] == L_]
-            lu.append(f"    {{  // round {L_}: pivots {piv}")
-            for j in piv:
+            lu.append(f"    // round {L_}: pivots {piv} (independent of each other)")
+            for j in piv:   # one pivot at a time keeps its broadcast row short-lived
+                lu.append("    {")
                 lu.append(f"        const double p{j} = sbz_readlane(row[{j}], {j});")
-                for c in upper[j]:
-                    lu.append(f"        const double u{j}_{c} = sbz_readlane(row[{c}], {j});")
-            for j in piv:
                 lu.append(f"        const double rp{j} = 1.0 / p{j};")
                 lu.append(f"        ok = ok && (p{j} != 0.0) && (p{j} == p{j}) && (fabs(p{j}) < 1e300);")
-            for j in piv:
-                lu.append(f"        if (lane == {j}) row[{j}] = rp{j};")
+                lu.append(f"        rpo = (lane == {j}) ? rp{j} : rpo;")
                 if lower[j]:
                     m = sum(1 << r for r in lower[j])
-                    lu.append(f"        if ((0x{m:x}ull >> lane) & 1ull) {{ const double l = row[{j}] * rp{j}; row[{j}] = l;" +
-                              "".join(f" row[{c}] -= l * u{j}_{c};" for c in upper[j]) + " }")
-            lu.append("    }")
+                    lu.append(f"        const bool on = (0x{m:x}ull >> lane) & 1ull;")
+                    lu.append(f"        const double l = on ? row[{j}] * rp{j} : 0.0;")
+                    for c in upper[j]:
+                        lu.append(f"        row[{c}] -= l * sbz_readlane(row[{c}], {j});")
+                    lu.append(f"        row[{j}] = on ? l : row[{j}];")
+                lu.append("    }")
         lu.append("    return ok;")
         lu.append("}")
         out += lu
-        # row-parallel triangular solves, level scheduled: lane r holds b_r; x_j broadcast with readlane
-        so = ["SBZ_DEV double sbz_wb_solve(int lane, const double* row, double b) {"]
+        # the factors go to LDS as separate strictly-lower (sL) and strictly-upper (sU) rows, with
+        # the reciprocal pivot in column nb of sU, so the solves need no per-lane masks
+        st = ["SBZ_DEV void sbz_wb_store(int lane, const double* row, double rpo, double* sL, double* sU) {",
+              "    if (lane < SBZ_NBORD) {",
+              "        double* lr = sL + lane * SBZ_WSTRIDE;",
+              "        double* ur = sU + lane * SBZ_WSTRIDE;"]
+        for j in range(nb):
+            st.append(f"        lr[{j}] = ({j} < lane) ? row[{j}] : 0.0; ur[{j}] = ({j} > lane) ? row[{j}] : 0.0;")
+        st.append("        ur[SBZ_NBORD] = rpo;")
+        st.append("    }")
+        st.append("    __syncthreads();")
+        st.append("}")
+        out += st
+        # row-parallel triangular solves, level scheduled: lane r holds b_r, x_j is broadcast with
+        # readlane and every lane subtracts its (possibly zero) factor entry times x_j
+        so = ["SBZ_DEV double sbz_wb_solve(int lane, const double* __restrict__ sL, const double* __restrict__ sU, double b) {",
+              "    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;",
+              "    const double* lr = sL + r * SBZ_WSTRIDE;",
+              "    const double* ur = sU + r * SBZ_WSTRIDE;"]
         for L_ in range(nlev):
             piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
             if not piv:
@@ -739,24 +778,18 @@ class Emitted:
             so.append("    {")
             for j in piv:
                 so.append(f"        const double x{j} = sbz_readlane(b, {j});")
-            for j in piv:
-                m = sum(1 << r for r in lower[j])
-                so.append(f"        if ((0x{m:x}ull >> lane) & 1ull) b -= row[{j}] * x{j};")
+            so.append("        b -= " + " + ".join(f"lr[{j}] * x{j}" for j in piv) + ";")
             so.append("    }")
         for L_ in range(nlevb):
-            piv = [j for j in range(nb) if levb[j] == L_]
+            piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]
+            if not piv:
+                continue
             so.append("    {")
             for j in piv:
-                so.append(f"        if (lane == {j}) b *= row[{j}];")
-            for j in piv:
-                if ucol[j]:
-                    so.append(f"        const double x{j} = sbz_readlane(b, {j});")
-            for j in piv:
-                if ucol[j]:
-                    m = sum(1 << r for r in ucol[j])
-                    so.append(f"        if ((0x{m:x}ull >> lane) & 1ull) b -= row[{j}] * x{j};")
+                so.append(f"        const double x{j} = sbz_readlane(b, {j}) * sU[{j} * SBZ_WSTRIDE + SBZ_NBORD];")
+            so.append("        b -= " + " + ".join(f"ur[{j}] * x{j}" for j in piv) + ";")
             so.append("    }")
-        so.append("    return b;")
+        so.append("    return b * ur[SBZ_NBORD];")
         so.append("}")
         out += so
         return out
