@@ -449,7 +449,7 @@ SBZ_DEV double sbz_pick8(const double* v, int i) {
 // (1 for massless species).  The loops over l are unrolled, so l/(2l+1) and (l+1)/(2l+1) are
 // compile-time constants.  SbzChainRow holds the l-independent per-lane part at one time.
 struct WaveCtx {
-    int lane, isB, bcomp, isC, ckind, cL, cbin, clen, catt, cattc, bchain, cbase;
+    int lane, isB, bcomp, isC, ckind, cL, cbin, catt, cattc, bchain, cbase;
 };
 struct SbzChainRow {
     double sk;   // s k
@@ -475,30 +475,74 @@ SBZ_DEV SbzChainRow sbz_chain_drow(const WaveCtx& c, const double* B, const doub
 #define SBZ_RL(e) ((3.0 + (e)) / (2.0 * (3.0 + (e)) + 1.0))    // l/(2l+1) at l = 3 + e
 #define SBZ_RL1(e) ((4.0 + (e)) / (2.0 * (3.0 + (e)) + 1.0))   // (l+1)/(2l+1) at l = 3 + e
 
+// Chain kernels, compiled per chain length LEN (chains of one length share a lane group, so
+// their loops have compile-time bounds and no per-lane predicates).  Element e holds l = 3 + e.
+//   rows:   out_e = ca_e u_{e-1} + cd_e u_e + cb_e u_{e+1},  u_{-1} = attachment (l = 2)
+//   factor: d'_e = W[e][e] - W[e][e+1] W[e+1][e] / d'_e+1 from the top (l = L) down, W = I/(γh) - J
+//   fwd:    b'_e = b_e - W[e][e+1] b'_{e+1} / d'_{e+1}          (in place)
+//   bwd:    x_e  = (b'_e - W[e][e-1] x_{e-1}) / d'_e              (in place, x_{-1} = attachment)
+template <int LEN>
+SBZ_DEV void sbz_chain_rows(const SbzChainRow& r, double att, const double* ci, double* co) {
+    double v[LEN + 2];
+    v[0] = att;
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) v[e + 1] = ci[e];
+    v[LEN + 1] = 0.0;
+#pragma unroll
+    for (int e = 0; e < LEN - 1; ++e) co[e] = r.sk * SBZ_RL(e) * v[e] + r.dd * v[e + 1] - r.sk * SBZ_RL1(e) * v[e + 2];
+    co[LEN - 1] = r.sk * v[LEN - 1] + r.top * v[LEN];
+}
+template <int LEN>
+SBZ_DEV double sbz_chain_factor(const SbzChainRow& r, double ig, double* cinv) {
+    double ca_next = r.sk;   // -W[e+1][e]; the top row's is s k
+    cinv[LEN - 1] = 1.0 / (ig - r.top);
+#pragma unroll
+    for (int e = LEN - 2; e >= 0; --e) {
+        cinv[e] = 1.0 / (ig - r.dd + r.sk * SBZ_RL1(e) * ca_next * cinv[e + 1]);
+        ca_next = r.sk * SBZ_RL(e);
+    }
+    return -ca_next * cinv[0];   // W[0][att] / d'_0: the Schur term of the attachment row
+}
+template <int LEN>
+SBZ_DEV double sbz_chain_fwd(double sk, const double* cinv, double* kc) {
+    double b[LEN];
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) b[e] = kc[e];
+#pragma unroll
+    for (int e = LEN - 2; e >= 0; --e) b[e] -= sk * SBZ_RL1(e) * cinv[e + 1] * b[e + 1];
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) kc[e] = b[e];
+    return b[0] * cinv[0];
+}
+template <int LEN>
+SBZ_DEV void sbz_chain_bwd(double sk, const double* cinv, double xatt, double* kc) {
+    double b[LEN];
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) b[e] = kc[e];
+    double prev = xatt;
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) {
+        prev = (b[e] + ((e + 1 == LEN) ? sk : sk * SBZ_RL(e)) * prev) * cinv[e];
+        b[e] = prev;
+    }
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) kc[e] = b[e];
+}
+#define SBZ_IN_GROUP(lo, hi) (c.lane >= (lo) && c.lane < (hi))
+
 // out = J(τ) in, i.e. f(τ, in) (f is linear in u); with dB given, out = ∂f/∂τ at fixed `in`.
-// B, dB, Cs, in, out are LDS arrays; each lane writes only its own components of `out`.
+// B, dB, Cs, in, out are LDS arrays; lane 0 writes the border rows, chain lanes their chains.
 SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB, const double* Cs, double k,
                             const double* in, double* out) {
-    const double fb = dB ? sbz_wb_dfdt(c.lane, B, dB, Cs, k, in) : sbz_wb_f(c.lane, B, Cs, k, in);
-    if (c.isB) out[c.bcomp] = fb;
+    if (dB) sbz_wb_dfdt(c.lane, B, dB, Cs, k, in, out);
+    else sbz_wb_f(c.lane, B, Cs, k, in, out);
     if (c.isC) {
         const SbzChainRow r = dB ? sbz_chain_drow(c, B, dB, Cs, k) : sbz_chain_row(c, B, Cs, k);
-        // branch-free over the longest chain: all loads first (clamped to the chain), then the
-        // rows, then stores predicated on e < clen
-        const double* ci = in + c.cbase;
-        double* co = out + c.cbase;
-        double v[SBZ_MCHAIN + 1];
-        v[0] = in[c.cattc];
-#pragma unroll
-        for (int e = 0; e < SBZ_MCHAIN; ++e) v[e + 1] = ci[e < c.clen ? e : c.clen - 1];
-#pragma unroll
-        for (int e = 0; e < SBZ_MCHAIN; ++e) {
-            const bool last = (e + 1 == c.clen);
-            const double nxt = (e + 1 < SBZ_MCHAIN) ? v[e + 2] : 0.0;
-            const double o = last ? r.sk * v[e] + r.top * v[e + 1]
-                                  : r.sk * SBZ_RL(e) * v[e] + r.dd * v[e + 1] - r.sk * SBZ_RL1(e) * nxt;
-            if (e < c.clen) co[e] = o;
-        }
+        const double att = in[c.cattc];
+#define SBZ_CG_ROWS(lo, hi, len) \
+        if (SBZ_IN_GROUP(lo, hi)) sbz_chain_rows<len>(r, att, in + c.cbase, out + c.cbase);
+        SBZ_CHAIN_GROUPS(SBZ_CG_ROWS)
+#undef SBZ_CG_ROWS
     }
 }
 
@@ -522,7 +566,6 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
     c.ckind = sbz_chain_kind[ci];
     c.cL = sbz_chain_L[ci];
     c.cbin = sbz_chain_bin[ci];
-    c.clen = sbz_chain_len[ci];
     c.catt = sbz_chain_att[ci];
     c.cattc = sbz_chain_attc[ci];
     c.cbase = sbz_chain_base[ci];
@@ -691,21 +734,16 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             }
             __syncthreads();
             const double* B0 = L.B[0];
-            // chain factorization (top-down elimination from l = L) and its Schur term for the
-            // attachment: d'_e = W[e][e] - W[e][e+1] W[e+1][e] / d'_{e+1}, with W = I/(γh) - J
+            // chain factorization (top-down elimination from l = L) and its Schur term
             if (c.isC) {
                 const SbzChainRow r = sbz_chain_row(c, B0, L.Cs, k);
                 csk = r.sk;
-                double ca_next = 0.0;  // -W[e+1][e]
-#pragma unroll
-                for (int e = SBZ_MCHAIN - 1; e >= 0; --e) {
-                    const bool on = e < c.clen, last = (e + 1 == c.clen);
-                    double dp = ig - (last ? r.top : r.dd);
-                    if (e + 1 < SBZ_MCHAIN && !last) dp += r.sk * SBZ_RL1(e) * ca_next * cinv[e + 1];
-                    cinv[e] = on ? 1.0 / dp : 0.0;
-                    ca_next = on ? (last ? r.sk : r.sk * SBZ_RL(e)) : 0.0;
-                }
-                L.C[c.lane] = -ca_next * cinv[0];  // W[0][att] / d'_0
+                double xcs = 0.0;
+#define SBZ_CG_FACTOR(lo, hi, len) \
+                if (SBZ_IN_GROUP(lo, hi)) xcs = sbz_chain_factor<len>(r, ig, cinv);
+                SBZ_CHAIN_GROUPS(SBZ_CG_FACTOR)
+#undef SBZ_CG_FACTOR
+                L.C[c.lane] = xcs;
             }
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
@@ -757,25 +795,14 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                     }
                     __syncthreads();
                     // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution
-                    // forward sweep b'_e = b_e - W[e][e+1] b'_{e+1} / d'_{e+1}, stored in place
+                    // chain forward sweeps (in place); the attachment rows take their Schur terms
                     if (c.isC) {
-                        double* kc = Ks + c.cbase;
-                        double bv[SBZ_MCHAIN];
-#pragma unroll
-                        for (int e = 0; e < SBZ_MCHAIN; ++e) bv[e] = kc[e < c.clen ? e : c.clen - 1];
-                        double nxt = 0.0;
-#pragma unroll
-                        for (int e = SBZ_MCHAIN - 1; e >= 0; --e) {
-                            double be = bv[e];
-                            if (e + 1 < SBZ_MCHAIN) be -= csk * SBZ_RL1(e) * cinv[e + 1] * nxt;  // cinv = 0 past the chain
-                            be = (e < c.clen) ? be : 0.0;
-                            bv[e] = be;
-                            nxt = be;
-                        }
-#pragma unroll
-                        for (int e = 0; e < SBZ_MCHAIN; ++e)
-                            if (e < c.clen) kc[e] = bv[e];
-                        L.C[c.lane] = nxt * cinv[0];
+                        double y = 0.0;
+#define SBZ_CG_FWD(lo, hi, len) \
+                        if (SBZ_IN_GROUP(lo, hi)) y = sbz_chain_fwd<len>(csk, cinv, Ks + c.cbase);
+                        SBZ_CHAIN_GROUPS(SBZ_CG_FWD)
+#undef SBZ_CG_FWD
+                        L.C[c.lane] = y;
                     }
                     __syncthreads();
                     {
@@ -788,22 +815,13 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                         }
                     }
                     __syncthreads();
-                    // back substitution x_e = (b'_e - W[e][e-1] x_{e-1}) / d'_e from the attachment
+                    // chain back substitutions from the solved attachments
                     if (c.isC) {
-                        double* kc = Ks + c.cbase;
-                        double bv[SBZ_MCHAIN];
-#pragma unroll
-                        for (int e = 0; e < SBZ_MCHAIN; ++e) bv[e] = kc[e < c.clen ? e : c.clen - 1];
-                        double prev = L.X[c.catt];
-#pragma unroll
-                        for (int e = 0; e < SBZ_MCHAIN; ++e) {
-                            const double ca = (e + 1 == c.clen) ? csk : csk * SBZ_RL(e);
-                            prev = (bv[e] + ca * prev) * cinv[e];
-                            bv[e] = prev;
-                        }
-#pragma unroll
-                        for (int e = 0; e < SBZ_MCHAIN; ++e)
-                            if (e < c.clen) kc[e] = bv[e];
+                        const double xatt = L.X[c.catt];
+#define SBZ_CG_BWD(lo, hi, len) \
+                        if (SBZ_IN_GROUP(lo, hi)) sbz_chain_bwd<len>(csk, cinv, xatt, Ks + c.cbase);
+                        SBZ_CHAIN_GROUPS(SBZ_CG_BWD)
+#undef SBZ_CG_BWD
                     }
                     __syncthreads();
                     if (s == 8) {

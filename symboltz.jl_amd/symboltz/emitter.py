@@ -631,9 +631,17 @@ class Emitted:
         # pattern in lane space; eliminate lanes in order 0..nb-1
         Lpat = {(lane_of[i], lane_of[j]) for (i, j) in bpat}
         filled, lower, upper = symbolic_lu(nb, Lpat, list(range(nb)))
+        chains = sorted(chains, key=lambda ch: -len(ch[3]))   # equal lengths on adjacent lanes
         self.wave = dict(nb=nb, nc=len(chains), bcomp=bcomp, chains=chains)
         mch = max(len(ch[3]) for ch in chains)
+        groups = []   # (first lane, last lane + 1, length): chain code is compiled per group
+        for ci, ch in enumerate(chains):
+            if groups and groups[-1][2] == len(ch[3]):
+                groups[-1][1] = ci + 1
+            else:
+                groups.append([ci, ci + 1, len(ch[3])])
         out = [f"#define SBZ_NBORD {nb}", f"#define SBZ_NCHAIN {len(chains)}", f"#define SBZ_MCHAIN {mch}"]
+        out.append("#define SBZ_CHAIN_GROUPS(X) " + " ".join(f"X({a}, {b}, {n_})" for a, b, n_ in groups))
         out.append("__constant__ int sbz_bord_comp[SBZ_NBORD] = {" + ", ".join(map(str, bcomp)) + "};")
         for ch in chains:   # the kernel addresses chain members as base + (l - 3)
             if ch[3] != list(range(ch[3][0], ch[3][0] + len(ch[3]))):
@@ -653,11 +661,11 @@ class Emitted:
         assign = [(f"const double wf{j}", sysm.f[c]) for j, c in enumerate(bcomp)]
         lines, _ = _emit_block(assign, "tbf")
         out.append("template <class UI>")
-        out.append("SBZ_DEV double sbz_wb_f(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u) {")
+        out.append("SBZ_DEV void sbz_wb_f(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u, double* out) {")
         out += lines
-        out.append("    double r = 0.0;")
-        out += [f"    r = (lane == {j}) ? wf{j} : r;" for j in range(nb)]
-        out.append("    return r;")
+        out.append("    if (lane == 0) {")
+        out += [f"        out[{c}] = wf{j};" for j, c in enumerate(bcomp)]
+        out.append("    }")
         out.append("}")
         dassign = []
         for j, c in enumerate(bcomp):
@@ -669,11 +677,11 @@ class Emitted:
             dassign.append((f"const double wd{j}", e))
         lines, _ = _emit_block(dassign, "tbd")
         out.append("template <class UI>")
-        out.append("SBZ_DEV double sbz_wb_dfdt(int lane, const double* __restrict__ B, const double* __restrict__ dB, const double* __restrict__ Cs, double k, UI u) {")
+        out.append("SBZ_DEV void sbz_wb_dfdt(int lane, const double* __restrict__ B, const double* __restrict__ dB, const double* __restrict__ Cs, double k, UI u, double* out) {")
         out += lines
-        out.append("    double r = 0.0;")
-        out += [f"    r = (lane == {j}) ? wd{j} : r;" for j in range(nb)]
-        out.append("    return r;")
+        out.append("    if (lane == 0) {")
+        out += [f"        out[{c}] = wd{j};" for j, c in enumerate(bcomp)]
+        out.append("    }")
         out.append("}")
         # W of the border block, computed once by the whole wavefront (one CSE over all entries,
         # so subexpressions shared between rows are evaluated once) and staged through LDS, from
