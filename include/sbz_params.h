@@ -53,4 +53,11 @@
 #define SBZ_P_HESW     52  /* RECFAST helium switch: 6 (all corrections) or 0        baryons.jl:91-114 */
 #define SBZ_P_HSW      53  /* RECFAST hydrogen switch: 1 (K fudge fit) or 0          baryons.jl:76-84 */
 
+/* Derived constants, filled by the library itself (sbz_background_solve, sbz_bg_upload) from the
+ * entries above; callers leave them at any value. */
+#define SBZ_P_RE1F     54  /* (1 + z_re1)^n1 */
+#define SBZ_P_RE1K     55  /* tanh rate 1/(n1 (1 + z_re1)^(n1-1) Δz1)                baryons.jl:131-134 */
+#define SBZ_P_RE2F     56  /* (1 + z_re2)^n2 */
+#define SBZ_P_RE2K     57  /* 1/(n2 (1 + z_re2)^(n2-1) Δz2) */
+
 #endif

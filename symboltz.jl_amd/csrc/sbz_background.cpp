@@ -201,6 +201,7 @@ extern "C" int sbz_background_solve(double* params, double rtol, double atol, in
         return SBZ_ERR_ARG;
     }
     double* P = params;
+    sbz_params_derive(P);
     BgSys sys{P};
     const int order = 5;
     const int maxiters = 100000;

@@ -925,14 +925,22 @@ struct HV {
 };
 extern "C" int sbz_probe_n(void) { return SBZ_N; }
 extern "C" int sbz_probe_nnz(void) { return SBZ_NNZ_LU; }
-extern "C" void sbz_probe_bg(const double* P, const double* kt, int nk, double tau, double* B, double* dB) {
+struct HP {  // parameter vector with its derived entries filled
+    double p[SBZ_NPAR];
+    explicit HP(const double* P) { for (int i = 0; i < SBZ_NPAR; ++i) p[i] = P[i]; sbz_params_derive(p); }
+};
+extern "C" void sbz_probe_bg(const double* P0, const double* kt, int nk, double tau, double* B, double* dB) {
+    HP hp(P0);
+    const double* P = hp.p;
     int idx = sbz_find_knot(kt, nk, tau);
     Dual Bd[SBZ_NB];
     sbz_pt_background<Dual>(P, kt, nk, idx, Dual(tau, 1.0), Bd);
     for (int i = 0; i < SBZ_NB; ++i) { B[i] = Bd[i].v; dB[i] = Bd[i].d; }
 }
-extern "C" void sbz_probe_src(const double* P, const double* kt, int nk, double k, double tau, const double* u,
+extern "C" void sbz_probe_src(const double* P0, const double* kt, int nk, double k, double tau, const double* u,
                               double* S, double* work) {
+    HP hp(P0);
+    const double* P = hp.p;
     double Cs[25]; sbz_consts(P, Cs);
     int idx = sbz_find_knot(kt, nk, tau);
     for (int i = 0; i < SBZ_N; ++i) work[i] = u[i];

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "sbz_internal.h"
+#include "sbz_thermo.h"
 
 #define HIPCHK(x)                                                                       \
     do {                                                                                \
@@ -213,6 +214,7 @@ extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const doubl
     std::memcpy(s.knots.data() + n_knots, u, sizeof(double) * 5 * n_knots);
     std::memcpy(s.knots.data() + 6 * (size_t)n_knots, du, sizeof(double) * 5 * n_knots);
     std::memcpy(s.params, params, sizeof(s.params));
+    sbz_params_derive(s.params);
     s.valid = true;
     return SBZ_OK;
 }

@@ -136,14 +136,21 @@ SBZ_HD T sbz_alphaHe_fit(T Tb, double q, double p) {
     return q / (s2 * sbz_pow(1.0 + s2, 1.0 - p) * sbz_pow(1.0 + s1, 1.0 + p));
 }
 
+// Fill the derived entries of a parameter vector (SBZ_P_RE*): parameter-only powers of the
+// reionization terms, evaluated once per set instead of at every background evaluation.
+SBZ_HD void sbz_params_derive(double* P) {
+    const double z1 = P[SBZ_P_ZRE1], n1 = P[SBZ_P_NRE1], z2 = P[SBZ_P_ZRE2], n2 = P[SBZ_P_NRE2];
+    P[SBZ_P_RE1F] = pow(1.0 + z1, n1);
+    P[SBZ_P_RE1K] = 1.0 / (n1 * pow(1.0 + z1, n1 - 1.0) * P[SBZ_P_DZRE1]);
+    P[SBZ_P_RE2F] = pow(1.0 + z2, n2);
+    P[SBZ_P_RE2K] = 1.0 / (n2 * pow(1.0 + z2, n2 - 1.0) * P[SBZ_P_DZRE2]);
+}
+
+// smoothifelse(f(z_re) - f(z), 0, Xemax; k = 1/(n (1+z_re)^(n-1) Δz)), f(z) = (1+z)^n, with
+// fre = f(z_re) and kk precomputed (sbz_params_derive); opz = 1 + z = 1/a    baryons.jl:128-138
 template <class T>
-SBZ_HD T sbz_reion_xe(const double* P, T a, double zre, double dz, double n, double xemax) {
-    // smoothifelse(f(z_re) - f(z), 0, Xemax; k = 1/(n (1+z_re)^(n-1) Δz)),
-    // f(z) = (1+z)^n                                                          baryons.jl:128-138
-    T opz = 1.0 / a;  // 1 + z
-    double fre = pow(1.0 + zre, n);
-    T fz = (n == 1.5) ? sbz_sqrt(opz) * opz : sbz_pow(opz, n);
-    double kk = 1.0 / (n * pow(1.0 + zre, n - 1.0) * dz);
+SBZ_HD T sbz_reion_xe(T opz, double fre, double kk, double n, double xemax) {
+    T fz = (n == 1.5) ? sbz_sqrt(opz) * opz : (n == 1.0) ? opz : sbz_pow(opz, n);
     return sbz_smoothifelse(fre - fz, T(0.0), T(xemax), kk);
 }
 
@@ -165,8 +172,9 @@ SBZ_HD void sbz_thermo(const double* P, T a, T XH, T XHe, T dT, const SbzDens<T>
     th.XHepp = 2.0 * R * fHe / den / (1.0 + sbz_sqrt(1.0 + 4.0 * R * fHe / (den * den)));
     th.Xe = XH + fHe * XHe + th.XHepp;                       // baryons.jl:69
     if (P[SBZ_P_REION] != 0.0) {
-        th.Xe = th.Xe + sbz_reion_xe(P, a, P[SBZ_P_ZRE1], P[SBZ_P_DZRE1], P[SBZ_P_NRE1], 1.0 + fHe);
-        th.Xe = th.Xe + sbz_reion_xe(P, a, P[SBZ_P_ZRE2], P[SBZ_P_DZRE2], P[SBZ_P_NRE2], fHe);
+        const T opz = 1.0 / a;
+        th.Xe = th.Xe + sbz_reion_xe(opz, P[SBZ_P_RE1F], P[SBZ_P_RE1K], P[SBZ_P_NRE1], 1.0 + fHe);
+        th.Xe = th.Xe + sbz_reion_xe(opz, P[SBZ_P_RE2F], P[SBZ_P_RE2K], P[SBZ_P_NRE2], fHe);
     }
     th.ne = th.Xe * th.nH;                                   // baryons.jl:192
     th.kappadot = -(a / H0SI) * th.ne * (sigmaT * c_);       // D(_κ)           baryons.jl:175
