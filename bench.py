@@ -1,11 +1,13 @@
 """Benchmark of the perturbation-stage hot path on one MI355X.
 
 A step is one pass of the hot path over one batch: every k-mode of the workload integrated with
-Rodas5P from τ_ini(k) to τ0 in one persistent launch, followed by the P(k) reduction.  The
+Rodas5P from τ_ini(k) to τ0 in one persistent launch, with the CMB source functions evaluated on
+a 2000-point conformal-time grid during the integration, followed by the P(k) reduction.  The
 default workload is the 1024-mode massive-neutrino configuration ("cfg1": photon and massless-ν
 lmax 16, one massive ν of 0.06 eV with 4 momentum bins and lmax 10, n = 100 unknowns per mode,
-k log-spaced in [1e-5, 10]/Mpc, rtol = atol = 1e-5).  Inputs (background tables, instances) are
-resident in device memory before the timed region.
+k log-spaced in [1e-5, 10]/Mpc, rtol = atol = 1e-5; sources on cosgrid(max(τ_bg0, 0.01), τ0)).
+Inputs (background tables, instances) are resident in device memory before the timed region;
+outputs stay there.
 
 Prints ONE JSON line with the metric, the roofline of the dominant kernel (algorithmic FP64
 flops counted by the emitter × the solver's own step counts, over the kernel's HIP-event time)
@@ -65,7 +67,22 @@ def main():
                     help="wave: one wavefront per k-mode (default); lane: one lane per k-mode")
     ap.add_argument("--cpu-sample", type=int, default=32, help="oracle baseline: every N-th mode")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sources", action="store_true", help="P(k) only: skip the CMB sources")
+    ap.add_argument("--n-tau", type=int, default=2000, help="points of the source time grid")
+    ap.add_argument("--pmc-copy", type=float, default=0.0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.pmc_copy > 0:
+        import torch
+        n = int(args.pmc_copy * 2**20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.pmc_copy, "bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described in "
               f"DESIGN.md and not measured here.", file=sys.stderr)
@@ -73,7 +90,7 @@ def main():
 
     from symboltz import _lib
     from symboltz.problem import CosmologyProblem, solvebg
-    from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_LANE
+    from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_LANE, SAVE_SOURCES
 
     if _lib.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
@@ -87,7 +104,12 @@ def main():
     t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), bg.t[0], bg.t[-1])
     algo = ALGO_WAVE if args.algo == "wave" else ALGO_LANE
     kname = "sbz_wave_kernel" if args.algo == "wave" else "sbz_rodas5p_kernel"
-    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes, algo=algo)
+    taus = None
+    if not args.no_sources and args.workload == "cfg1":   # cosgrid(a, b; length) (src/utils.jl:285) over [max(τ_bg0, 0.01), τ0]
+        a0, b0 = max(bg.t[0], 1e-2), bg.t[-1]
+        taus = a0 + (b0 - a0) * (1.0 - np.cos(np.pi * np.linspace(0.0, 0.5, args.n_tau)))
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes, algo=algo, saveat=taus,
+                   save=SAVE_SOURCES if taus is not None else 0)
     log(f"workload {args.workload}: n={gm.n}, {len(ks)} modes, background {t_bg*1e3:.1f} ms ({len(bg.t)} knots)")
     for w in range(args.warmup):
         ms = plan.run()
@@ -102,6 +124,7 @@ def main():
     for s, ms in enumerate(kernel_ms):
         log(f"step {s}: kernel {ms:.1f} ms")
     res = plan.fetch()
+    src = plan.sources() if taus is not None else None
     ms_per_step = elapsed / args.steps * 1e3
     n_modes = len(ks)
     value = n_modes / (elapsed / args.steps)
@@ -122,17 +145,25 @@ def main():
         cores = os.cpu_count() or 1
         cores = min(cores, 16)
         tc = time.perf_counter()
-        o = O.solve(ks[sel], t0[sel], nthreads=cores)
+        o = O.solve(ks[sel], t0[sel], nthreads=cores, saveat=taus, sources=taus is not None)
         tc = time.perf_counter() - tc
         po = np.array([O.p0(k) * O.delta_m(k, bg.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks[sel])])
         parity = float(np.max(np.abs(pk[sel] / po - 1)))
+        sp = ""
+        if taus is not None:   # per mode and source: max |S_gpu - S_cpu| / max |S_cpu| over the grid
+            ref = o["src"]
+            sc = np.abs(ref).max(axis=1, keepdims=True) + 1e-300
+            src_parity = float(np.max(np.abs(src[sel] - ref) / sc))
+            sp = f"; sources max |ΔS|/max|S| = {src_parity:.2e}"
         cpu = {"value": len(sel) / tc, "unit": "k-mode solves/s", "cores": cores, "kind": "port",
-               "sample": f"every {args.cpu_sample}th mode of the workload ({len(sel)} modes, same k spread), "
-                         f"{tc:.1f} s on {cores} host threads; max |P_gpu/P_cpu - 1| = {parity:.2e}"}
-        log(f"cpu baseline: {cpu['value']:.1f} solves/s, parity {parity:.2e}")
+               "sample": f"every {args.cpu_sample}th mode of the workload ({len(sel)} modes, same k spread, "
+                         f"{'with' if taus is not None else 'without'} the sources), {tc:.1f} s on {cores} host "
+                         f"threads; max |P_gpu/P_cpu - 1| = {parity:.2e}{sp}"}
+        log(f"cpu baseline: {cpu['value']:.1f} solves/s, parity {parity:.2e}{sp}")
 
     out = {
-        "metric": "k-mode solves/s at rtol 1e-5 (plain GPU Rodas5P, P(k) at z=0)",
+        "metric": "k-mode solves/s at rtol 1e-5 (plain GPU Rodas5P, P(k) at z=0"
+                  + ("" if taus is None else f" + CMB sources on {len(taus)} τ") + ")",
         "value": value,
         "unit": "k-mode solves/s",
         "n_gpus": 1,
@@ -144,7 +175,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "description": desc, "algo": args.algo, "n_modes": n_modes, "n_unknowns": gm.n,
+        "config": {"workload": args.workload, "description": desc, "algo": args.algo,
+                   "n_tau_sources": 0 if taus is None else len(taus), "n_modes": n_modes, "n_unknowns": gm.n,
                    "background_ms": t_bg * 1e3, "all_success": ok},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,

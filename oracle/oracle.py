@@ -52,6 +52,7 @@ def lib():
         L.ora_solve.argtypes = [_ip, _dp] + sp + [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                   ctypes.c_long, _dp, ctypes.c_int, _dp, _dp, _ip, _i64p,
                                                   ctypes.c_int]
+        L.ora_solve_sources.argtypes = L.ora_solve.argtypes + [_dp]
         _L = L
     return _L
 
@@ -130,7 +131,7 @@ class Oracle:
     def p0(self, k):
         return self.L.ora_p0(_p(self.P), k)
 
-    def solve(self, ks, tstart, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, nthreads=None):
+    def solve(self, ks, tstart, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, nthreads=None, sources=False):
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         tstart = np.ascontiguousarray(np.broadcast_to(tstart, ks.shape), dtype=np.float64)
         nm = len(ks)
@@ -144,10 +145,11 @@ class Oracle:
         stats = np.zeros((nm, 6), dtype=np.int64)
         if nthreads is None:
             nthreads = os.cpu_count() or 1
-        self.L.ora_solve(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), _p(ks), _p(tstart), nm, rtol, atol,
-                         maxiters, _p(saveat), ns, _p(usave), _p(ufinal), _p(ret, ctypes.c_int),
-                         _p(stats, ctypes.c_int64), nthreads)
-        return dict(u=ufinal, usave=usave if ns else None, retcode=ret, stats=stats)
+        src = np.zeros((nm, ns, 3)) if (sources and ns) else None
+        self.L.ora_solve_sources(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), _p(ks), _p(tstart), nm, rtol,
+                                 atol, maxiters, _p(saveat), ns, _p(usave), _p(ufinal), _p(ret, ctypes.c_int),
+                                 _p(stats, ctypes.c_int64), nthreads, _p(src) if src is not None else None)
+        return dict(u=ufinal, usave=usave if ns else None, retcode=ret, stats=stats, src=src)
 
     def spectrum_matter(self, ks, kt_ini=1e-2, tau_ini_max=1e-4, **kw):
         """P(k) at τ0 as spectrum_matter(prob, ks) (fourier.jl:79-83,90-97)."""

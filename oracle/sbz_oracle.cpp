@@ -813,10 +813,25 @@ void ora_sources(const int* md, const double* P, const double* kt, const double*
 
 // Solve n_modes k-modes with `nthreads` worker threads (dynamic scheduling, one task per mode as
 // the reference's @spawn per k, src/solve.jl:566).  stats: 6 int64 per mode.
+// With src non-null, also the CMB sources (S_T, S_E, S_ψ) at every saveat time from the saved
+// states (src[mode][save][3]), inside the same per-mode task.
+void ora_solve_sources(const int* md, const double* P, const double* kt, const double* ku, const double* kdu,
+                       int nk, const double* ks, const double* tstart, int n_modes, double rtol, double atol,
+                       long maxiters, const double* saveat, int nsave, double* usave, double* ufinal, int* retcode,
+                       int64_t* stats, int nthreads, double* src);
+
 void ora_solve(const int* md, const double* P, const double* kt, const double* ku, const double* kdu, int nk,
                const double* ks, const double* tstart, int n_modes, double rtol, double atol, long maxiters,
                const double* saveat, int nsave, double* usave, double* ufinal, int* retcode, int64_t* stats,
                int nthreads) {
+    ora_solve_sources(md, P, kt, ku, kdu, nk, ks, tstart, n_modes, rtol, atol, maxiters, saveat, nsave, usave,
+                      ufinal, retcode, stats, nthreads, nullptr);
+}
+
+void ora_solve_sources(const int* md, const double* P, const double* kt, const double* ku, const double* kdu,
+                       int nk, const double* ks, const double* tstart, int n_modes, double rtol, double atol,
+                       long maxiters, const double* saveat, int nsave, double* usave, double* ufinal, int* retcode,
+                       int64_t* stats, int nthreads, double* src) {
     Model m = make_model(md);
     Spline s{kt, ku, kdu, nk};
     Opts o{rtol, atol, maxiters};
@@ -831,6 +846,10 @@ void ora_solve(const int* md, const double* P, const double* kt, const double* k
                                     usave ? usave + (size_t)i * nsave * m.n : nullptr, ufinal + (size_t)i * m.n, st);
             int64_t v[6] = {st.naccept, st.nreject, st.nf, st.njac, st.nlu, st.nsolve};
             std::memcpy(stats + 6 * (size_t)i, v, sizeof(v));
+            if (src && usave)
+                for (int j = 0; j < nsave; ++j)
+                    sources(m, P, s, ks[i], saveat[j], usave + ((size_t)i * nsave + j) * m.n,
+                            src + ((size_t)i * nsave + j) * 3);
         }
     };
     if (nthreads <= 1) {
