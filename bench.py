@@ -35,6 +35,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of the same workload (scaled by a copy-kernel calibration there), or None."""
+    if args.algo != "wave" or args.workload != "cfg1" or args.no_sources or args.n_tau != 2000:
+        return None
+    f = ROOT / "profiles" / "sbz_wave_kernel_cfg1_hbm.md"
+    if not f.exists():
+        return None
+    import re
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", f.read_text())
+    return (int(m.group(1)), str(f.relative_to(ROOT))) if m else None
+
+
 def workload(name):
     from symboltz import models
     from symboltz.constants import k_dimensionless
@@ -45,7 +58,8 @@ def workload(name):
         pars = dict(base, Neff=2.046, N=1, m_eV=0.06)
         ks = k_dimensionless(np.logspace(-5, 1, 1024), base["h"])
         desc = ("LCDM + 1 massive nu (0.06 eV, 4 bins, lmax 10), photon/massless-nu lmax 16, n=100; "
-                "1024 k log-spaced in [1e-5, 10]/Mpc; rtol=atol=1e-5; P(k) at z=0")
+                "1024 k log-spaced in [1e-5, 10]/Mpc; rtol=atol=1e-5; P(k) at z=0 and CMB sources "
+                "(S_T, S_E, S_psi) on a 2000-point cosgrid of conformal time")
     elif name == "cfg0":
         M = models.LCDM(lmax=10, h=False)
         pars = dict(base, Neff=3.046)
@@ -65,7 +79,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--algo", choices=["wave", "lane"], default="wave",
                     help="wave: one wavefront per k-mode (default); lane: one lane per k-mode")
-    ap.add_argument("--cpu-sample", type=int, default=32, help="oracle baseline: every N-th mode")
+    ap.add_argument("--cpu-sample", type=int, default=1, help="oracle baseline: every N-th mode (1: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sources", action="store_true", help="P(k) only: skip the CMB sources")
     ap.add_argument("--n-tau", type=int, default=2000, help="points of the source time grid")
@@ -107,7 +121,9 @@ def main():
     taus = None
     if not args.no_sources and args.workload == "cfg1":   # cosgrid(a, b; length) (src/utils.jl:285) over [max(τ_bg0, 0.01), τ0]
         a0, b0 = max(bg.t[0], 1e-2), bg.t[-1]
-        taus = a0 + (b0 - a0) * (1.0 - np.cos(np.pi * np.linspace(0.0, 0.5, args.n_tau)))
+        x = np.linspace(0.0, 0.5, args.n_tau)
+        cospi = np.where(x == 0.5, 0.0, np.cos(np.pi * x))   # cospi(1/2) = 0 exactly, as Julia's cospi
+        taus = a0 + (b0 - a0) * (1.0 - cospi)
     plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes, algo=algo, saveat=taus,
                    save=SAVE_SOURCES if taus is not None else 0)
     log(f"workload {args.workload}: n={gm.n}, {len(ks)} modes, background {t_bg*1e3:.1f} ms ({len(bg.t)} knots)")
@@ -156,7 +172,8 @@ def main():
             src_parity = float(np.max(np.abs(src[sel] - ref) / sc))
             sp = f"; sources max |ΔS|/max|S| = {src_parity:.2e}"
         cpu = {"value": len(sel) / tc, "unit": "k-mode solves/s", "cores": cores, "kind": "port",
-               "sample": f"every {args.cpu_sample}th mode of the workload ({len(sel)} modes, same k spread, "
+               "sample": f"{'all' if args.cpu_sample == 1 else f'every {args.cpu_sample}th'} mode(s) of the "
+                         f"workload ({len(sel)} modes, "
                          f"{'with' if taus is not None else 'without'} the sources), {tc:.1f} s on {cores} host "
                          f"threads; max |P_gpu/P_cpu - 1| = {parity:.2e}{sp}"}
         log(f"cpu baseline: {cpu['value']:.1f} solves/s, parity {parity:.2e}{sp}")
@@ -184,6 +201,10 @@ def main():
                      "flops_per_launch": flops, "flops_per_step": gm.em.flops_step, "steps_per_launch": steps_total},
         "cpu_baseline": cpu,
     }
+    tr = measured_traffic(args)
+    if tr is not None:
+        out["roofline"]["traffic"] = tr[0]
+        out["roofline"]["traffic_source"] = tr[1]
     print(json.dumps(out), flush=True)
 
 
