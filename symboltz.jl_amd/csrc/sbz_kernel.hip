@@ -426,7 +426,7 @@ struct WaveLDS {
     double B[7][SBZ_NB];           // background at the distinct stage times (0..5) and a spare slot (6)
     double dB[2][SBZ_NB];          // d/dτ of the background at t_n (0) and at the spare time (1)
     double Cs[25];                 // per-set constants (momentum nodes, weights, d ln f0/d ln x, c_s²)
-    double M[2][SBZ_NBORD * SBZ_WSTRIDE];  // border block: W staging (0), then its L (0) and U (1) factors
+    double M[3][SBZ_NBORD * SBZ_WSTRIDE];  // border block: A, α, β, p, q staging (0); L (1) and U (2) factors
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
     double ddkd;
@@ -546,12 +546,24 @@ SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB,
     }
 }
 
-// Wave-wide sum, identical in every lane: the butterfly adds in a lane-dependent order, so lane
-// 0's result is broadcast — every control decision taken on it (accept/reject, dt) must be uniform.
+// DPP lane permutation of a double within rows of 16 lanes (all 64 lanes must be active).
+template <int CTRL>
+SBZ_DEV double sbz_dpp(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(x & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+
+// Wave-wide sum, identical in every lane (called with all 64 lanes active): butterflies within
+// each row of 16 lanes (quad_perm xor 1, xor 2, row_half_mirror, row_mirror), then the four row
+// sums combined through readlane — every control decision taken on it must be uniform.
 SBZ_DEV double sbz_wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return sbz_readlane(v, 0);
+    v += sbz_dpp<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += sbz_dpp<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += sbz_dpp<0x141>(v);   // row_half_mirror
+    v += sbz_dpp<0x140>(v);   // row_mirror
+    return (sbz_readlane(v, 0) + sbz_readlane(v, 16)) + (sbz_readlane(v, 32) + sbz_readlane(v, 48));
 }
 
 extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) {
@@ -572,6 +584,8 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
     double* const DT = L.v[4];
     double* const US = L.v[5];
     double* const TMP = L.v[14];
+    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += 64) L.M[0][i] = 0.0;   // structural zeros, never written
+    __syncthreads();
 
     for (;;) {
         if (c.lane == 0) L.mode = (int)atomicAdd(a.queue, 1u);
@@ -659,6 +673,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
         }
         double row[SBZ_NBORD];
         double cpl = 0.0;
+        SbzWb wb;                  // low-rank (Woodbury) data of the border solve
         double cinv[SBZ_MCHAIN];   // reciprocal modified pivots of this lane's chain
         double csk = 0.0;          // s k of this lane's chain at t_n
         int bidx = kidx;   // knot cursor of this lane's stage-time background
@@ -747,10 +762,26 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
             }
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
-            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl);
+            double al, be;
+            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
             double rpo;   // reciprocal pivot of this lane's row
             bool ok = sbz_wb_lu(c.lane, row, rpo);
-            sbz_wb_store(c.lane, row, rpo, L.M[0], L.M[1]);
+            sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
+            // Woodbury data: z = A^-1 [α β] and C^-1 with C = I - [p q]^T z (2 x 2)
+            sbz_wb_tri2(c.lane, L.M[1], L.M[2], al, be);
+            wb.z1 = c.isB ? al : 0.0;
+            wb.z2 = c.isB ? be : 0.0;
+            {
+                const double c11 = 1.0 - sbz_wave_sum(wb.p * wb.z1), c12 = -sbz_wave_sum(wb.p * wb.z2);
+                const double c21 = -sbz_wave_sum(wb.q * wb.z1), c22 = 1.0 - sbz_wave_sum(wb.q * wb.z2);
+                const double det = c11 * c22 - c12 * c21;
+                ok = ok && (det != 0.0) && (det == det) && (fabs(det) < 1e300);
+                const double idet = 1.0 / det;
+                wb.c11 = c22 * idet;
+                wb.c12 = -c12 * idet;
+                wb.c21 = -c21 * idet;
+                wb.c22 = c11 * idet;
+            }
             ok = __all(ok || !c.isB);
             sbz_wave_apply(c, B0, L.dB[0], L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
             st_j++;
@@ -808,7 +839,9 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                     {
                         double b = c.isB ? Ks[c.bcomp] : 0.0;
                         if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                        const double x = sbz_wb_solve(c.lane, L.M[0], L.M[1], b);
+                        sbz_wb_tri(c.lane, L.M[1], L.M[2], b);   // y = A^-1 b
+                        const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
+                        const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                         if (c.isB) {
                             Ks[c.bcomp] = x;
                             L.X[c.lane] = x;

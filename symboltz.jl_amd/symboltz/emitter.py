@@ -78,10 +78,15 @@ class System:
         self.dB = {nm: sp.Symbol(f"dB_{nm}", real=True) for nm in B_NAMES}
         self.C = {nm: sp.Symbol(f"C_{nm}", real=True) for nm in C_NAMES}
         self.k = sp.Symbol("k", positive=True)
+        # Φ' and Ψ as placeholders: f_ph is linear in them, so every row splits into a local part
+        # plus α Φ' + β Ψ, with Φ' and Ψ themselves linear functionals of u (Woodbury in the kernel)
+        self.DPHI = sp.Symbol("DPHI", real=True)
+        self.PSI = sp.Symbol("PSI", real=True)
+        self.f_ph = self._rhs(ph=True)
         self.f = self._rhs()
 
     # -- equations ---------------------------------------------------------------------------
-    def _rhs(self):
+    def _rhs(self, ph=False):
         M, L, u, B, C, k = self.M, self.L, self.u, self.B, self.C, self.k
         R = sp.Rational
         pi = sp.pi
@@ -114,8 +119,10 @@ class System:
         if M.de == "w0wa":
             dX, thX = u[L["X"]], u[L["X"] + 1]
             drho += dX * B["rx"]
-        Psi = Phi - 12 * pi * a**2 * Pi / k**2                               # gravity.jl:39
-        dPhi = -R(4, 3) * pi * a**2 / H * drho - k**2 / (3 * H) * Phi - H * Psi  # gravity.jl:38
+        Psi_e = Phi - 12 * pi * a**2 * Pi / k**2                             # gravity.jl:39
+        dPhi_e = -R(4, 3) * pi * a**2 / H * drho - k**2 / (3 * H) * Phi - H * Psi_e  # gravity.jl:38
+        Psi, dPhi = (self.PSI, self.DPHI) if ph else (Psi_e, dPhi_e)
+        self.Psi_e, self.dPhi_e = Psi_e, dPhi_e
         f[L["Phi"]] = dPhi
         # photons                                                           photons.jl:30-33,56-59
         Lg = M.lmax_g
@@ -164,7 +171,8 @@ class System:
             ca2 = w - B["wdx"] / (3 * H * (1 + w))
             f[L["X"]] = -(1 + w) * (thX - 3 * dPhi) - 3 * H * (cs2 - w) * dX - 9 * (H / k) ** 2 * (1 + w) * (cs2 - ca2) * thX
             f[L["X"] + 1] = -H * (1 - 3 * cs2) * thX + cs2 / (1 + w) * k**2 * dX + k**2 * Psi
-        self.Psi, self.dPhi = Psi, dPhi
+        if not ph:
+            self.Psi, self.dPhi = Psi, dPhi
         return f
 
     # -- derived structure ------------------------------------------------------------------
@@ -613,22 +621,56 @@ class Emitted:
         out.append(f"#define SBZ_FLOPS_STEP {self.flops_step}")
         return "\n".join(out) + "\n"
 
+    def _check_lowrank(self, border, Jh, alpha, beta, pvec, qvec):
+        """J = ∂h/∂u + α pᵀ + β qᵀ on the border block, checked at a random point."""
+        import random
+        exprs = [e for e in self.J.values()] + list(Jh.values()) + list(alpha.values()) + list(beta.values()) + \
+            list(pvec.values()) + list(qvec.values())
+        syms = set()
+        for e in exprs:
+            syms |= sp.sympify(e).free_symbols
+        rng = random.Random(12345)
+        vals = {x: sp.Float(rng.uniform(0.5, 2.0)) for x in syms}
+        for i in border:
+            for j in border:
+                full = float(sp.sympify(self.J.get((i, j), 0)).xreplace(vals))
+                rec = float(sp.sympify(Jh.get((i, j), 0) + alpha[i] * pvec[j] + beta[i] * qvec[j]).xreplace(vals))
+                if abs(full - rec) > 1e-9 * (1.0 + abs(full)):
+                    raise ValueError(f"low-rank split of the border Jacobian fails at ({i}, {j})")
+
     def _emit_wave(self):
         """Structure and straight-line code of the wavefront-per-mode kernel (see sbz_kernel.hip)."""
         M, sysm, n = self.M, self.sys, self.n
         chains, border = wave_structure(M)
         bset = set(border)
-        # border elimination order: minimum degree on the border block of W
-        bpat = {(i, j) for (i, j) in self.J if i in bset and j in bset}
-        loc = {c: r for r, c in enumerate(border)}
-        lpat = {(loc[i], loc[j]) for (i, j) in bpat}
         nb = len(border)
         if nb > 64 or len(chains) > 64:
             raise ValueError("wave kernel supports at most 64 border unknowns and 64 chains")
+        # Border rows split as f_i = h_i(u) + α_i Φ' + β_i Ψ with Φ' = p·u, Ψ = q·u, so the border
+        # block of W is A - α pᵀ - β qᵀ with A = I/(γh) - ∂h/∂u block diagonal (Woodbury).
+        u = sysm.u
+        DP, PS = sysm.DPHI, sysm.PSI
+        alpha = {c: sp.diff(sysm.f_ph[c], DP) for c in border}
+        beta = {c: sp.diff(sysm.f_ph[c], PS) for c in border}
+        hrow = {c: sysm.f_ph[c].subs({DP: 0, PS: 0}) for c in border}
+        pvec = {c: sp.diff(sysm.dPhi_e, u[c]) for c in range(n)}
+        qvec = {c: sp.diff(sysm.Psi_e, u[c]) for c in range(n)}
+        if any(pvec[c] != 0 or qvec[c] != 0 for c in range(n) if c not in bset):
+            raise ValueError("Φ' and Ψ must depend on border unknowns only")
+        Jh = {}
+        for i in border:
+            for j in border:
+                d = sp.diff(hrow[i], u[j])
+                if d != 0:
+                    Jh[(i, j)] = d
+        self._check_lowrank(border, Jh, alpha, beta, pvec, qvec)
+        bpat = set(Jh.keys())
+        loc = {c: r for r, c in enumerate(border)}
+        lpat = {(loc[i], loc[j]) for (i, j) in bpat}
         order = min_degree_order(nb, lpat | {(j, i) for (i, j) in lpat})
         lane_of = {border[r]: ln for ln, r in enumerate(order)}     # comp -> lane
         bcomp = [border[r] for r in order]                         # lane -> comp
-        # pattern in lane space; eliminate lanes in order 0..nb-1
+        # pattern of A in lane space; eliminate lanes in order 0..nb-1
         Lpat = {(lane_of[i], lane_of[j]) for (i, j) in bpat}
         filled, lower, upper = symbolic_lu(nb, Lpat, list(range(nb)))
         chains = sorted(chains, key=lambda ch: -len(ch[3]))   # equal lengths on adjacent lanes
@@ -683,51 +725,54 @@ class Emitted:
         out += [f"        out[{c}] = wd{j};" for j, c in enumerate(bcomp)]
         out.append("    }")
         out.append("}")
-        # W of the border block, computed once by the whole wavefront (one CSE over all entries,
-        # so subexpressions shared between rows are evaluated once) and staged through LDS, from
-        # where every lane loads its own row.  Column nb of a row holds its coupling W[p][c0] to
-        # the bottom of an attached chain; the chain's Schur complement xc = W[c0][p]/d'_0 is then
-        # folded into the diagonal.
+        # A, α, β, p, q of the border, computed once per step by the whole wavefront (one CSE over
+        # all entries) and staged through LDS (structural zeros are never written and stay zero),
+        # from where every lane loads its own row.  Row r: A[r][0..nb), then its coupling to the
+        # bottom of an attached chain (whose Schur complement xc = W[c0][p]/d'_0 is folded into
+        # the diagonal), α_r, β_r, p_r, q_r.
         att_bottom = {lane_of[ch[4]]: ch[3][0] for ch in chains}
-        ws = nb + 1 + (nb + 1) % 2          # odd row stride in doubles: conflict-free row loads
-        wassign = []
+        ws = nb + 5 + (nb + 5 + 1) % 2       # odd row stride in doubles: conflict-free row loads
+        ig = sp.Symbol("ig")
+        items = []
         for (i, j) in sorted(filled):
-            ci, cj = bcomp[i], bcomp[j]
-            e = -self.J.get((ci, cj), sp.Integer(0))
-            if ci == cj:
-                e = sp.Symbol("ig") + e
-            wassign.append((i, j, e))
-        items = [(f"sW[{i * ws + j}]", e) for (i, j, e) in wassign if e != 0]
-        for i in sorted(att_bottom):
-            items.append((f"sW[{i * ws + nb}]", -self.J.get((bcomp[i], att_bottom[i]), sp.Integer(0))))
+            e = -Jh.get((bcomp[i], bcomp[j]), sp.Integer(0)) + (ig if i == j else 0)
+            if e != 0:
+                items.append((f"sW[{i * ws + j}]", e))
+        for i in range(nb):
+            c = bcomp[i]
+            if i in att_bottom:
+                items.append((f"sW[{i * ws + nb}]", -sp.diff(hrow[c], u[att_bottom[i]])))
+            for off, e in ((1, alpha[c]), (2, beta[c]), (3, pvec[c]), (4, qvec[c])):
+                if e != 0:
+                    items.append((f"sW[{i * ws + nb + off}]", e))
         lines, _ = _emit_block(items, "tw", interleave=True)
         out.append(f"#define SBZ_WSTRIDE {ws}")
-        out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* sW, double* row, double& cpl) {")
+        out.append("struct SbzWb {   // per-lane low-rank data of the border solve")
+        out.append("    double p, q, z1, z2;   // p_r, q_r; (A^-1 α)_r, (A^-1 β)_r")
+        out.append("    double c11, c12, c21, c22;   // (I - [p q]^T A^-1 [α β])^-1, uniform")
+        out.append("};")
+        out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* sW, double* row, double& cpl, double& al, double& be, SbzWb& wb) {")
         out.append("    if (lane == 0) {")
         out += ["    " + ln for ln in lines]
         out.append("    }")
         out.append("    __syncthreads();")
-        zero = {(i, j) for (i, j, e) in wassign if e == 0}
-        out.append("    const int r = lane < SBZ_NBORD ? lane : 0;")
+        out.append("    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;")
         out.append("    const double* w = sW + r * SBZ_WSTRIDE;")
+        out.append("    const bool own = lane < SBZ_NBORD;")
         for j in range(nb):
-            col = [i for (i, jj) in filled if jj == j and (i, j) not in zero]
-            if len(col) == 0:
-                out.append(f"    row[{j}] = 0.0;")
-            elif len(col) == nb:
-                out.append(f"    row[{j}] = w[{j}];")
-            else:
-                m = sum(1 << i for i in col)
-                out.append(f"    row[{j}] = ((0x{m:x}ull >> r) & 1ull) ? w[{j}] : 0.0;")
-        am = sum(1 << i for i in att_bottom)
-        out.append(f"    cpl = ((0x{am:x}ull >> r) & 1ull) ? w[SBZ_NBORD] : 0.0;")
+            out.append(f"    row[{j}] = w[{j}];")
+        out.append("    cpl = own ? w[SBZ_NBORD] : 0.0;")
+        out.append("    al = own ? w[SBZ_NBORD + 1] : 0.0;")
+        out.append("    be = own ? w[SBZ_NBORD + 2] : 0.0;")
+        out.append("    wb.p = own ? w[SBZ_NBORD + 3] : 0.0;")
+        out.append("    wb.q = own ? w[SBZ_NBORD + 4] : 0.0;")
         for i in sorted(att_bottom):   # selects on values, never on row positions: row stays in registers
             out.append(f"    row[{i}] -= (lane == {i}) ? cpl * xc : 0.0;")
-        out.append("    __syncthreads();   // sW is reused by the next step")
+        out.append("    __syncthreads();   // sW is rewritten by the next step")
         out.append("}")
-        # row-parallel LU of the border block (lane r owns row r, pivots in lane order), level
-        # scheduled: pivots whose rows and columns do not depend on each other are eliminated in
-        # the same round; pivot rows are broadcast with readlane (no LDS round trip)
+        # row-parallel LU of A (lane r owns row r, pivots in lane order), level scheduled: pivots
+        # whose rows and columns do not depend on each other are eliminated in the same round;
+        # pivot rows are broadcast with readlane (no LDS round trip)
         lev = [0] * nb
         for j in range(nb):
             deps = [q for q in range(j) if (j, q) in filled or (q, j) in filled]
@@ -773,33 +818,47 @@ class Emitted:
         st.append("    __syncthreads();")
         st.append("}")
         out += st
-        # row-parallel triangular solves, level scheduled: lane r holds b_r, x_j is broadcast with
-        # readlane and every lane subtracts its (possibly zero) factor entry times x_j
-        so = ["SBZ_DEV double sbz_wb_solve(int lane, const double* __restrict__ sL, const double* __restrict__ sU, double b) {",
-              "    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;",
-              "    const double* lr = sL + r * SBZ_WSTRIDE;",
-              "    const double* ur = sU + r * SBZ_WSTRIDE;"]
-        for L_ in range(nlev):
-            piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
-            if not piv:
-                continue
-            so.append("    {")
-            for j in piv:
-                so.append(f"        const double x{j} = sbz_readlane(b, {j});")
-            so.append("        b -= " + " + ".join(f"lr[{j}] * x{j}" for j in piv) + ";")
-            so.append("    }")
-        for L_ in range(nlevb):
-            piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]
-            if not piv:
-                continue
-            so.append("    {")
-            for j in piv:
-                so.append(f"        const double x{j} = sbz_readlane(b, {j}) * sU[{j} * SBZ_WSTRIDE + SBZ_NBORD];")
-            so.append("        b -= " + " + ".join(f"ur[{j}] * x{j}" for j in piv) + ";")
-            so.append("    }")
-        so.append("    return b * ur[SBZ_NBORD];")
-        so.append("}")
-        out += so
+
+        # A^-1 applied to NR right-hand sides at once (lane r holds entry r of each), level
+        # scheduled: x_j is broadcast with readlane and every lane subtracts its (possibly zero)
+        # factor entry times x_j
+        def tri(name, nr):
+            args = ", ".join(f"double& b{q}" for q in range(nr))
+            so = [f"SBZ_DEV void {name}(int lane, const double* __restrict__ sL, const double* __restrict__ sU, {args}) {{",
+                  "    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;",
+                  "    const double* lr = sL + r * SBZ_WSTRIDE;",
+                  "    const double* ur = sU + r * SBZ_WSTRIDE;"]
+            for L_ in range(nlev):
+                piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
+                if not piv:
+                    continue
+                so.append("    {")
+                for q in range(nr):
+                    for j in piv:
+                        so.append(f"        const double x{q}_{j} = sbz_readlane(b{q}, {j});")
+                for q in range(nr):
+                    so.append(f"        b{q} -= " + " + ".join(f"lr[{j}] * x{q}_{j}" for j in piv) + ";")
+                so.append("    }")
+            for L_ in range(nlevb):
+                piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]
+                if not piv:
+                    continue
+                so.append("    {")
+                for j in piv:
+                    so.append(f"        const double rp{j} = sU[{j} * SBZ_WSTRIDE + SBZ_NBORD];")
+                for q in range(nr):
+                    for j in piv:
+                        so.append(f"        const double x{q}_{j} = sbz_readlane(b{q}, {j}) * rp{j};")
+                for q in range(nr):
+                    so.append(f"        b{q} -= " + " + ".join(f"ur[{j}] * x{q}_{j}" for j in piv) + ";")
+                so.append("    }")
+            so.append("    const double rpr = ur[SBZ_NBORD];")
+            for q in range(nr):
+                so.append(f"    b{q} *= rpr;")
+            so.append("}")
+            return so
+        out += tri("sbz_wb_tri", 1)
+        out += tri("sbz_wb_tri2", 2)
         return out
 
 

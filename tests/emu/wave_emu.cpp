@@ -63,6 +63,21 @@ static double shfl_xor_d(double v, int off, int s) {
     barrier_at(s);
     return xd[g][cur ^ off];
 }
+// DPP moves used by the kernel: quad_perm (ctrl < 0x100), row_mirror (0x140), row_half_mirror (0x141)
+static int mov_dpp(int v, int ctrl, int s) {
+    const int g = gen[cur] ^= 1;
+    xi[g][cur] = v;
+    barrier_at(s);
+    int src;
+    if (ctrl < 0x100) src = (cur & ~3) | ((ctrl >> (2 * (cur & 3))) & 3);
+    else if (ctrl == 0x140) src = (cur & ~15) | (15 - (cur & 15));
+    else if (ctrl == 0x141) src = (cur & ~7) | (7 - (cur & 7));
+    else {
+        std::fprintf(stderr, "wave_emu: unsupported DPP control 0x%x\n", ctrl);
+        std::abort();
+    }
+    return (int)xi[g][src];
+}
 static int all(int p, int s) {
     const int g = gen[cur] ^= 1;
     xi[g][cur] = p ? 1 : 0;
@@ -118,6 +133,7 @@ static int run(void (*fn)(void*), void* arg) {
 #define __builtin_amdgcn_readlane(v, l) emu::readlane_i((v), (l), __LINE__)
 #define __shfl_xor(v, off, w) emu::shfl_xor_d((v), (off), __LINE__)
 #define __all(p) emu::all((p), __LINE__)
+#define __builtin_amdgcn_mov_dpp(v, ctrl, rm, bm, bc) emu::mov_dpp((v), (ctrl), __LINE__)
 static inline unsigned atomicAdd(unsigned* p, unsigned v) {
     const unsigned o = *p;
     *p = o + v;
