@@ -828,6 +828,11 @@ class Emitted:
                   "    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;",
                   "    const double* lr = sL + r * SBZ_WSTRIDE;",
                   "    const double* ur = sU + r * SBZ_WSTRIDE;"]
+            # every factor entry of a pass is loaded before the pass starts (the empty asm is a
+            # compiler barrier for memory accesses), so the rounds never wait on LDS latency
+            fwd = [j for j in range(nb) if lower[j]]
+            so += [f"    const double l{j} = lr[{j}];" for j in fwd]
+            so.append('    asm volatile("" ::: "memory");')
             for L_ in range(nlev):
                 piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
                 if not piv:
@@ -837,22 +842,23 @@ class Emitted:
                     for j in piv:
                         so.append(f"        const double x{q}_{j} = sbz_readlane(b{q}, {j});")
                 for q in range(nr):
-                    so.append(f"        b{q} -= " + " + ".join(f"lr[{j}] * x{q}_{j}" for j in piv) + ";")
+                    so.append(f"        b{q} -= " + " + ".join(f"l{j} * x{q}_{j}" for j in piv) + ";")
                 so.append("    }")
+            bwd = [j for j in range(nb) if ucol[j]]
+            so += [f"    const double u{j} = ur[{j}], rp{j} = sU[{j} * SBZ_WSTRIDE + SBZ_NBORD];" for j in bwd]
+            so.append("    const double rpr = ur[SBZ_NBORD];")
+            so.append('    asm volatile("" ::: "memory");')
             for L_ in range(nlevb):
                 piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]
                 if not piv:
                     continue
                 so.append("    {")
-                for j in piv:
-                    so.append(f"        const double rp{j} = sU[{j} * SBZ_WSTRIDE + SBZ_NBORD];")
                 for q in range(nr):
                     for j in piv:
                         so.append(f"        const double x{q}_{j} = sbz_readlane(b{q}, {j}) * rp{j};")
                 for q in range(nr):
-                    so.append(f"        b{q} -= " + " + ".join(f"ur[{j}] * x{q}_{j}" for j in piv) + ";")
+                    so.append(f"        b{q} -= " + " + ".join(f"u{j} * x{q}_{j}" for j in piv) + ";")
                 so.append("    }")
-            so.append("    const double rpr = ur[SBZ_NBORD];")
             for q in range(nr):
                 so.append(f"    b{q} *= rpr;")
             so.append("}")
