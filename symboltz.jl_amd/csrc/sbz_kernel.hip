@@ -42,7 +42,6 @@ SBZ_DEV double sbz_readlane(double v, int l) {
 // is 64 consecutive doubles (512 B, fully coalesced).
 #define SBZ_WAVE 64
 #define SBZ_OWN ((SBZ_N + 63) / 64)   // state components per lane in the wave kernel
-#define SBZ_NB 25
 struct SV {
     double* p;
     SBZ_DEV double& operator[](int i) const { return p[i * SBZ_WAVE]; }
@@ -131,6 +130,15 @@ SBZ_DEV void sbz_pt_background(const double* __restrict__ P, const double* __res
 #pragma unroll
     for (int i = 0; i < 8; ++i) B[16 + i] = (i < SBZ_NX) ? d.E[i] : T(1.0);
     B[24] = s[4] - P[SBZ_P_KAPPA0];  // κ = _κ - κ0                              baryons.jl:177
+    // reciprocals, once per evaluation, for the wavefront kernel's division-free expressions
+    B[SBZ_B_IHC] = 1.0 / d.Hc;
+    B[SBZ_B_IIRHOP] = 1.0 / (d.Irho + d.IP / 3.0);
+    B[SBZ_B_IIRHO] = 1.0 / d.Irho;
+    B[SBZ_B_IRB] = 1.0 / d.rho_b;
+    B[SBZ_B_IWX1] = (P[SBZ_P_DE] == 1.0) ? 1.0 / (1.0 + d.w_de) : T(0.0);
+    B[SBZ_B_ITAU] = 1.0 / tau;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) B[SBZ_B_IE0 + i] = (i < SBZ_NX) ? 1.0 / d.E[i] : T(1.0);
 }
 
 SBZ_DEV void sbz_bg_plain(const double* P, const double* kt, int nk, int& idx, double tau, double* B) {
@@ -190,8 +198,15 @@ SBZ_DEV void sbz_consts(const double* P, double* Cs) {
         Cs[i] = x;
         Cs[8 + i] = (i < SBZ_NX) ? P[SBZ_P_W + i] : 0.0;
         Cs[16 + i] = -x / (1.0 + exp(-x));  // d ln f0 / d ln x     neutrinos.jl:70
+        Cs[SBZ_C_IX0 + i] = 1.0 / x;
     }
     Cs[24] = P[SBZ_P_CS2X];
+    Cs[SBZ_C_IK] = 0.0;   // set per mode (sbz_consts_k)
+    Cs[SBZ_C_IK2] = 0.0;
+}
+SBZ_DEV void sbz_consts_k(double* Cs, double k) {
+    Cs[SBZ_C_IK] = 1.0 / k;
+    Cs[SBZ_C_IK2] = 1.0 / (k * k);
 }
 
 // Rodas5P tables in constant memory: the stage index is uniform across a wavefront, so these
@@ -231,7 +246,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
         const double k = a.inst_k[inst];
         const double t1 = kt[nk - 1];
         double t = a.inst_t0[inst];
-        double Cs[25];
+        double Cs[SBZ_NC];
         sbz_consts(P, Cs);
         double B[SBZ_NB], dB[SBZ_NB];
         int kidx = sbz_find_knot(kt, nk, t);
@@ -425,7 +440,7 @@ struct WaveLDS {
     double v[15][SBZ_N];           // 0,1 U (ping-pong), 2,3 F (ping-pong), 4 DT, 5 US, 6..13 K1..K8, 14 tmp
     double B[7][SBZ_NB];           // background at the distinct stage times (0..5) and a spare slot (6)
     double dB[2][SBZ_NB];          // d/dτ of the background at t_n (0) and at the spare time (1)
-    double Cs[25];                 // per-set constants (momentum nodes, weights, d ln f0/d ln x, c_s²)
+    double Cs[SBZ_NC];             // per-mode constants (momentum nodes, weights, d ln f0/d ln x, c_s², 1/x, 1/k)
     double M[3][SBZ_NBORD * SBZ_WSTRIDE];  // border block: A, α, β, p, q staging (0); L (1) and U (2) factors
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
@@ -458,18 +473,18 @@ struct SbzChainRow {
 };
 SBZ_DEV SbzChainRow sbz_chain_row(const WaveCtx& c, const double* B, const double* Cs, double k) {
     SbzChainRow r;
-    r.sk = (c.ckind == 2) ? k * sbz_pick8(Cs, c.cbin) / sbz_pick8(B + 16, c.cbin) : k;
+    r.sk = (c.ckind == 2) ? k * sbz_pick8(Cs, c.cbin) * sbz_pick8(B + SBZ_B_IE0, c.cbin) : k;
     r.dd = (c.ckind == 0) ? B[3] : 0.0;
-    r.top = r.dd - (c.cL + 1.0) / B[0];
+    r.top = r.dd - (c.cL + 1.0) * B[SBZ_B_ITAU];
     return r;
 }
 // the same coefficients differentiated in τ at fixed u
 SBZ_DEV SbzChainRow sbz_chain_drow(const WaveCtx& c, const double* B, const double* dB, const double* Cs, double k) {
     SbzChainRow r;
-    const double E = sbz_pick8(B + 16, c.cbin);
-    r.sk = (c.ckind == 2) ? -k * sbz_pick8(Cs, c.cbin) * sbz_pick8(dB + 16, c.cbin) / (E * E) : 0.0;
+    const double iE = sbz_pick8(B + SBZ_B_IE0, c.cbin), it = B[SBZ_B_ITAU];
+    r.sk = (c.ckind == 2) ? -k * sbz_pick8(Cs, c.cbin) * sbz_pick8(dB + 16, c.cbin) * (iE * iE) : 0.0;
     r.dd = (c.ckind == 0) ? dB[3] : 0.0;
-    r.top = r.dd + (c.cL + 1.0) / (B[0] * B[0]);
+    r.top = r.dd + (c.cL + 1.0) * (it * it);
     return r;
 }
 #define SBZ_RL(e) ((3.0 + (e)) / (2.0 * (3.0 + (e)) + 1.0))    // l/(2l+1) at l = 3 + e
@@ -609,10 +624,11 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
         double* osave = a.out_save ? a.out_save + (size_t)inst * a.n_save * SBZ_N : nullptr;
         double* osrc = a.out_src ? a.out_src + (size_t)inst * a.n_save * 3 : nullptr;
         if (c.lane == 0) {
-            double Cs[25], Bx[SBZ_NB];
+            double Cs[SBZ_NC], Bx[SBZ_NB];
             sbz_consts(P, Cs);
+            sbz_consts_k(Cs, k);
 #pragma unroll
-            for (int q = 0; q < 25; ++q) L.Cs[q] = Cs[q];
+            for (int q = 0; q < SBZ_NC; ++q) L.Cs[q] = Cs[q];
             int kk = kidx;
             sbz_bg_plain(P, kt, nk, kk, t, Bx);
 #pragma unroll
@@ -938,7 +954,7 @@ extern "C" __global__ void sbz_pk_kernel(const double* __restrict__ params, cons
     const double* kt = knots + knot_off[set];
     int nk = knot_n[set];
     double k = inst_k[i];
-    double Cs[25], B[SBZ_NB];
+    double Cs[SBZ_NC], B[SBZ_NB];
     sbz_consts(P, Cs);
     int idx = nk - 2;
     sbz_bg_plain(P, kt, nk, idx, kt[nk - 1], B);
@@ -957,6 +973,7 @@ struct HV {
     double& operator[](int i) const { return p[i]; }
 };
 extern "C" int sbz_probe_n(void) { return SBZ_N; }
+extern "C" int sbz_probe_nb(void) { return SBZ_NB; }
 extern "C" int sbz_probe_nnz(void) { return SBZ_NNZ_LU; }
 struct HP {  // parameter vector with its derived entries filled
     double p[SBZ_NPAR];
@@ -974,32 +991,32 @@ extern "C" void sbz_probe_src(const double* P0, const double* kt, int nk, double
                               double* S, double* work) {
     HP hp(P0);
     const double* P = hp.p;
-    double Cs[25]; sbz_consts(P, Cs);
+    double Cs[SBZ_NC]; sbz_consts(P, Cs);
     int idx = sbz_find_knot(kt, nk, tau);
     for (int i = 0; i < SBZ_N; ++i) work[i] = u[i];
     sbz_sources_at(P, kt, nk, idx, Cs, k, tau, HV{work}, HV{work + SBZ_N}, HV{work + 2 * SBZ_N},
                    HV{work + 3 * SBZ_N}, S);
 }
 extern "C" void sbz_probe_f(const double* B, const double* P, double k, const double* u, double* du) {
-    double Cs[25]; sbz_consts(P, Cs);
+    double Cs[SBZ_NC]; sbz_consts(P, Cs);
     sbz_f(B, Cs, k, HV{(double*)u}, HV{du});
 }
 extern "C" void sbz_probe_w(const double* B, const double* P, double k, double ig, double* W) {
-    double Cs[25]; sbz_consts(P, Cs);
+    double Cs[SBZ_NC]; sbz_consts(P, Cs);
     sbz_w(B, Cs, k, ig, HV{W});
 }
 extern "C" int sbz_probe_lu(double* W) { return sbz_lu(HV{W}) ? 1 : 0; }
 extern "C" void sbz_probe_solve(const double* W, double* b) { sbz_solve(HV{(double*)W}, HV{b}); }
 extern "C" void sbz_probe_dfdt(const double* B, const double* dB, const double* P, double k, const double* u, double* out) {
-    double Cs[25]; sbz_consts(P, Cs);
+    double Cs[SBZ_NC]; sbz_consts(P, Cs);
     sbz_dfdt(B, dB, Cs, k, HV{(double*)u}, HV{out});
 }
 extern "C" void sbz_probe_ic(const double* B, const double* P, double k, double* u) {
-    double Cs[25]; sbz_consts(P, Cs);
+    double Cs[SBZ_NC]; sbz_consts(P, Cs);
     sbz_ic(B, Cs, k, HV{u});
 }
 extern "C" double sbz_probe_dm(const double* B, const double* P, double k, const double* u) {
-    double Cs[25]; sbz_consts(P, Cs);
+    double Cs[SBZ_NC]; sbz_consts(P, Cs);
     return sbz_dm(B, Cs, k, HV{(double*)u});
 }
 #endif

@@ -35,12 +35,35 @@ CSRC = Path(__file__).resolve().parent.parent / "csrc"
 # Background vector layout B[] (device: sbz_pt_background in csrc/sbz_kernel.h)
 B_NAMES = ["tau", "a", "Hc", "kd", "cs2", "rg", "rn", "rc", "rb", "rh", "ph", "Irho", "IP", "rx", "wx", "wdx"] + \
           [f"E{i}" for i in range(8)] + ["kappa"]
+# reciprocals evaluated once per background evaluation (the wavefront kernel's expressions use them
+# instead of divisions): 1/ℋ, 1/(Iρ + I_P/3), 1/Iρ, 1/ρ_b, 1/(1 + w_X), 1/τ, 1/E_i
+B_INV = ["iHc", "iIrhoP", "iIrho", "irb", "iwx1", "itau"] + [f"iE{i}" for i in range(8)]
+B_NAMES = B_NAMES + B_INV
 NB = len(B_NAMES)
 BIDX = {n: i for i, n in enumerate(B_NAMES)}
 
 # Per-set constants vector C[] (device: computed once per mode from the parameter vector)
-C_NAMES = [f"x{i}" for i in range(8)] + [f"w{i}" for i in range(8)] + [f"dl{i}" for i in range(8)] + ["cs2X"]
+C_NAMES = [f"x{i}" for i in range(8)] + [f"w{i}" for i in range(8)] + [f"dl{i}" for i in range(8)] + ["cs2X"] + \
+          [f"ix{i}" for i in range(8)] + ["ik", "ik2"]   # 1/x_i; 1/k and 1/k² (per mode)
 CIDX = {n: i for i, n in enumerate(C_NAMES)}
+
+
+def _inv_map(sysm):
+    """base expression -> symbol of its precomputed reciprocal (B_INV, 1/x_i, 1/k)."""
+    B, C, k = sysm.B, sysm.C, sysm.k
+    m = {B["Hc"]: B["iHc"], B["Irho"] + B["IP"] / 3: B["iIrhoP"], B["Irho"]: B["iIrho"], B["rb"]: B["irb"],
+         B["wx"] + 1: B["iwx1"], B["tau"]: B["itau"], k: C["ik"]}
+    for i in range(8):
+        m[B[f"E{i}"]] = B[f"iE{i}"]
+        m[C[f"x{i}"]] = C[f"ix{i}"]
+    return m
+
+
+def with_reciprocals(e, invm):
+    """e with every b**(-n) (b a key of invm, n a positive integer) replaced by invm[b]**n."""
+    e = sp.sympify(e)
+    return e.replace(lambda x: x.is_Pow and x.base in invm and x.exp.is_Integer and x.exp < 0,
+                     lambda x: invm[x.base] ** (-x.exp))
 
 
 def layout(M):
@@ -502,6 +525,13 @@ class Emitted:
         out.append(f"// Emitted by symboltz.emitter for model {self.M}")
         out.append(f"#define SBZ_N {n}")
         out.append(f"#define SBZ_NNZ_LU {self.nnz_LU}")
+        # background vector B[] and constants Cs[] layouts (printer indices; filled by the kernel)
+        out.append(f"#define SBZ_NB {NB}")
+        out.append(f"#define SBZ_NC {len(C_NAMES)}")
+        for nm in ["iHc", "iIrhoP", "iIrho", "irb", "iwx1", "itau", "iE0"]:
+            out.append(f"#define SBZ_B_{nm.upper()} {BIDX[nm]}")
+        for nm in ["ix0", "ik", "ik2"]:
+            out.append(f"#define SBZ_C_{nm.upper()} {CIDX[nm]}")
         out.append(f"#define SBZ_NX {self.M.nx if self.M.has_h else 0}")
         out.append(f"#define SBZ_HAS_W0WA {1 if self.M.de == 'w0wa' else 0}")
         # f
@@ -700,7 +730,8 @@ class Emitted:
             bch[lane_of[ch[4]]] = ci
         out.append("__constant__ int sbz_bord_chain[SBZ_NBORD] = {" + ", ".join(map(str, bch)) + "};")
         # border RHS and ∂f/∂τ: every lane evaluates all border rows, keeps its own
-        assign = [(f"const double wf{j}", sysm.f[c]) for j, c in enumerate(bcomp)]
+        invm = _inv_map(sysm)
+        assign = [(f"const double wf{j}", with_reciprocals(sysm.f[c], invm)) for j, c in enumerate(bcomp)]
         lines, _ = _emit_block(assign, "tbf")
         out.append("template <class UI>")
         out.append("SBZ_DEV void sbz_wb_f(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u, double* out) {")
@@ -716,7 +747,7 @@ class Emitted:
             for nm in B_NAMES:
                 if fi.has(sysm.B[nm]):
                     e += sp.diff(fi, sysm.B[nm]) * sysm.dB[nm]
-            dassign.append((f"const double wd{j}", e))
+            dassign.append((f"const double wd{j}", with_reciprocals(e, invm)))
         lines, _ = _emit_block(dassign, "tbd")
         out.append("template <class UI>")
         out.append("SBZ_DEV void sbz_wb_dfdt(int lane, const double* __restrict__ B, const double* __restrict__ dB, const double* __restrict__ Cs, double k, UI u, double* out) {")
@@ -745,6 +776,7 @@ class Emitted:
             for off, e in ((1, alpha[c]), (2, beta[c]), (3, pvec[c]), (4, qvec[c])):
                 if e != 0:
                     items.append((f"sW[{i * ws + nb + off}]", e))
+        items = [(lhs, with_reciprocals(e, invm)) for lhs, e in items]
         lines, _ = _emit_block(items, "tw", interleave=True)
         out.append(f"#define SBZ_WSTRIDE {ws}")
         out.append("struct SbzWb {   // per-lane low-rank data of the border solve")

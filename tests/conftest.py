@@ -62,6 +62,7 @@ class Probe:
         L.sbz_probe_src.argtypes = [dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp, dp]
         self.L = L
         self.n = L.sbz_probe_n()
+        self.nb = L.sbz_probe_nb()
         self.nnz = L.sbz_probe_nnz()
 
     @staticmethod
@@ -69,7 +70,7 @@ class Probe:
         return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
     def bg(self, S, tau):
-        B, dB = np.zeros(25), np.zeros(25)
+        B, dB = np.zeros(self.nb), np.zeros(self.nb)
         self.L.sbz_probe_bg(self.p(S.P), self.p(S.knots), len(S.t), tau, self.p(B), self.p(dB))
         return B, dB
 
