@@ -131,14 +131,16 @@ SBZ_DEV void sbz_pt_background(const double* __restrict__ P, const double* __res
     for (int i = 0; i < 8; ++i) B[16 + i] = (i < SBZ_NX) ? d.E[i] : T(1.0);
     B[24] = s[4] - P[SBZ_P_KAPPA0];  // κ = _κ - κ0                              baryons.jl:177
     // reciprocals, once per evaluation, for the wavefront kernel's division-free expressions
-    B[SBZ_B_IHC] = 1.0 / d.Hc;
-    B[SBZ_B_IIRHOP] = 1.0 / (d.Irho + d.IP / 3.0);
-    B[SBZ_B_IIRHO] = 1.0 / d.Irho;
-    B[SBZ_B_IRB] = 1.0 / d.rho_b;
-    B[SBZ_B_IWX1] = (P[SBZ_P_DE] == 1.0) ? 1.0 / (1.0 + d.w_de) : T(0.0);
-    B[SBZ_B_ITAU] = 1.0 / tau;
+    // (values only: the emitted ∂f/∂τ is differentiated before the reciprocals are substituted,
+    // so no derivative of a reciprocal is ever read)
+    B[SBZ_B_IHC] = T(1.0 / dval(d.Hc));
+    B[SBZ_B_IIRHOP] = T(1.0 / dval(d.Irho + d.IP / 3.0));
+    B[SBZ_B_IIRHO] = T(1.0 / dval(d.Irho));
+    B[SBZ_B_IRB] = T(1.0 / dval(d.rho_b));
+    B[SBZ_B_IWX1] = T((P[SBZ_P_DE] == 1.0) ? 1.0 / (1.0 + dval(d.w_de)) : 0.0);
+    B[SBZ_B_ITAU] = T(1.0 / dval(tau));
 #pragma unroll
-    for (int i = 0; i < 8; ++i) B[SBZ_B_IE0 + i] = (i < SBZ_NX) ? 1.0 / d.E[i] : T(1.0);
+    for (int i = 0; i < 8; ++i) B[SBZ_B_IE0 + i] = T((i < SBZ_NX) ? 1.0 / dval(d.E[i]) : 1.0);
 }
 
 SBZ_DEV void sbz_bg_plain(const double* P, const double* kt, int nk, int& idx, double tau, double* B) {
@@ -570,6 +572,63 @@ SBZ_DEV double sbz_dpp(double v) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
 }
 
+// Rodas5P tableau as compile-time constants, so the stage combinations below fold them into
+// immediates (sbz_rodas5p.h; the __constant__ tables above stay for the lane-per-mode kernel).
+struct SbzR5 {
+    static constexpr double A[9][8] = {
+        {0}, {0}, {SBZ_R5_A21}, {SBZ_R5_A31, SBZ_R5_A32}, {SBZ_R5_A41, SBZ_R5_A42, SBZ_R5_A43},
+        {SBZ_R5_A51, SBZ_R5_A52, SBZ_R5_A53, SBZ_R5_A54},
+        {SBZ_R5_A61, SBZ_R5_A62, SBZ_R5_A63, SBZ_R5_A64, SBZ_R5_A65},
+        {SBZ_R5_A61, SBZ_R5_A62, SBZ_R5_A63, SBZ_R5_A64, SBZ_R5_A65, 1.0},
+        {SBZ_R5_A61, SBZ_R5_A62, SBZ_R5_A63, SBZ_R5_A64, SBZ_R5_A65, 1.0, 1.0}};
+    static constexpr double C[9][8] = {
+        {0}, {0}, {SBZ_R5_C21}, {SBZ_R5_C31, SBZ_R5_C32}, {SBZ_R5_C41, SBZ_R5_C42, SBZ_R5_C43},
+        {SBZ_R5_C51, SBZ_R5_C52, SBZ_R5_C53, SBZ_R5_C54},
+        {SBZ_R5_C61, SBZ_R5_C62, SBZ_R5_C63, SBZ_R5_C64, SBZ_R5_C65},
+        {SBZ_R5_C71, SBZ_R5_C72, SBZ_R5_C73, SBZ_R5_C74, SBZ_R5_C75, SBZ_R5_C76},
+        {SBZ_R5_C81, SBZ_R5_C82, SBZ_R5_C83, SBZ_R5_C84, SBZ_R5_C85, SBZ_R5_C86, SBZ_R5_C87}};
+    static constexpr double d[9] = {0, SBZ_R5_D1, SBZ_R5_D2, SBZ_R5_D3, SBZ_R5_D4, SBZ_R5_D5, 0, 0, 0};
+};
+
+// stage S input  US = u_n + Σ_{j<S} a_Sj K_j  for this lane's components
+template <int S>
+SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, double* US) {
+#pragma unroll
+    for (int r = 0; r < SBZ_OWN; ++r) {
+        const int i = lane + 64 * r;
+        if (i < SBZ_N) {
+            double acc = Un[i];
+#pragma unroll
+            for (int j = 1; j < S; ++j) acc += SbzR5::A[S][j - 1] * v[5 + j][i];
+            US[i] = acc;
+        }
+    }
+}
+// stage S right-hand side  K_S = f + h d_S ∂f/∂τ + Σ_{j<S} (c_Sj / h) K_j  (in place on K_S)
+template <int S>
+SBZ_DEV void sbz_stage_rhs(int lane, const double (*v)[SBZ_N], double dt, double idt, const double* DT, double* Ks) {
+#pragma unroll
+    for (int r = 0; r < SBZ_OWN; ++r) {
+        const int i = lane + 64 * r;
+        if (i < SBZ_N) {
+            double acc = Ks[i] + (dt * SbzR5::d[S]) * DT[i];
+#pragma unroll
+            for (int j = 1; j < S; ++j) acc += (SbzR5::C[S][j - 1] * idt) * v[5 + j][i];
+            Ks[i] = acc;
+        }
+    }
+}
+#define SBZ_STAGE_SWITCH(F, ...)                                                                  \
+    switch (s) {                                                                                  \
+    case 2: F<2>(__VA_ARGS__); break;                                                             \
+    case 3: F<3>(__VA_ARGS__); break;                                                             \
+    case 4: F<4>(__VA_ARGS__); break;                                                             \
+    case 5: F<5>(__VA_ARGS__); break;                                                             \
+    case 6: F<6>(__VA_ARGS__); break;                                                             \
+    case 7: F<7>(__VA_ARGS__); break;                                                             \
+    default: F<8>(__VA_ARGS__); break;                                                            \
+    }
+
 // Wave-wide sum, identical in every lane (called with all 64 lanes active): butterflies within
 // each row of 16 lanes (quad_perm xor 1, xor 2, row_half_mirror, row_mirror), then the four row
 // sums combined through readlane — every control decision taken on it must be uniform.
@@ -811,34 +870,13 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
                         for (int i = c.lane; i < SBZ_N; i += 64) Ks[i] = F0[i] + dt * SBZ_R5_D1 * DT[i];
                     } else {
                         double* in = (s <= 8) ? US : U1;
-                        if (s <= 8) {
-#pragma unroll
-                            for (int r = 0; r < SBZ_OWN; ++r) {
-                                const int i = c.lane + 64 * r;
-                                if (i < SBZ_N) {
-                                    double acc = Un[i];
-#pragma unroll
-                                    for (int j = 1; j < 8; ++j) acc += R5A[s][j - 1] * L.v[5 + j][i];
-                                    US[i] = acc;
-                                }
-                            }
-                        }
+                        if (s <= 8) SBZ_STAGE_SWITCH(sbz_stage_in, c.lane, L.v, Un, US)
                         __syncthreads();
                         sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
                         st_f++;
                         __syncthreads();
                         if (s == 9) break;
-                        const double dd = dt * R5d[s];
-#pragma unroll
-                        for (int r = 0; r < SBZ_OWN; ++r) {
-                            const int i = c.lane + 64 * r;
-                            if (i < SBZ_N) {
-                                double acc = Ks[i] + dd * DT[i];
-#pragma unroll
-                                for (int j = 1; j < 8; ++j) acc += (R5C[s][j - 1] * idt) * L.v[5 + j][i];
-                                Ks[i] = acc;
-                            }
-                        }
+                        SBZ_STAGE_SWITCH(sbz_stage_rhs, c.lane, L.v, dt, idt, DT, Ks)
                     }
                     __syncthreads();
                     // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution

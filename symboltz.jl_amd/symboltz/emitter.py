@@ -457,6 +457,21 @@ def count_ops(exprs):
     return flops, divs, special
 
 
+def _preload(lines):
+    """Hoist every u[i], B[i], dB[i], Cs[i] read of a block into locals loaded at its top, ahead of
+    a compiler barrier: the LDS reads are then all in flight at once instead of being issued one
+    dependency at a time."""
+    pat = re.compile(r"\b(u|B|dB|Cs)\[(\d+)\]")
+    seen = {}
+    for ln in lines:
+        for m in pat.finditer(ln):
+            seen.setdefault((m.group(1), int(m.group(2))), None)
+    loc = {"u": "pu", "B": "pb", "dB": "pd", "Cs": "pc"}
+    head = [f"    const double {loc[a]}{i} = {a}[{i}];" for (a, i) in seen]
+    body = [pat.sub(lambda m: f"{loc[m.group(1)]}{m.group(2)}", ln) for ln in lines]
+    return head + ['    asm volatile("" ::: "memory");'] + body
+
+
 def _emit_block(assign, tmp_prefix, opt=True, interleave=False):
     """CSE a list of (lhs_code, expr) and return C lines plus the op count.
 
@@ -733,6 +748,7 @@ class Emitted:
         invm = _inv_map(sysm)
         assign = [(f"const double wf{j}", with_reciprocals(sysm.f[c], invm)) for j, c in enumerate(bcomp)]
         lines, _ = _emit_block(assign, "tbf")
+        lines = _preload(lines)
         out.append("template <class UI>")
         out.append("SBZ_DEV void sbz_wb_f(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, UI u, double* out) {")
         out += lines
@@ -749,6 +765,7 @@ class Emitted:
                     e += sp.diff(fi, sysm.B[nm]) * sysm.dB[nm]
             dassign.append((f"const double wd{j}", with_reciprocals(e, invm)))
         lines, _ = _emit_block(dassign, "tbd")
+        lines = _preload(lines)
         out.append("template <class UI>")
         out.append("SBZ_DEV void sbz_wb_dfdt(int lane, const double* __restrict__ B, const double* __restrict__ dB, const double* __restrict__ Cs, double k, UI u, double* out) {")
         out += lines
