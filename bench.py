@@ -48,26 +48,54 @@ def measured_traffic(args):
     return (int(m.group(1)), str(f.relative_to(ROOT))) if m else None
 
 
-def workload(name):
+def lhs_sets(base, n_sets, seed=0):
+    """n_sets parameter sets from a seeded Latin hypercube over h, ω_b, ω_c, ln(1e10 As), ns and
+    τ_reion (the batched configuration); τ_reion is met by a root-find on z_re."""
+    from concurrent.futures import ThreadPoolExecutor
+    from scipy.stats import qmc
     from symboltz import models
-    from symboltz.constants import k_dimensionless
+    from symboltz.problem import z_re_for_optical_depth
+    lo = np.array([0.60, 0.020, 0.10, 2.9, 0.92, 0.04])
+    hi = np.array([0.75, 0.024, 0.14, 3.2, 1.00, 0.08])
+    X = qmc.scale(qmc.LatinHypercube(d=6, seed=seed).random(n_sets), lo, hi)
+    M = models.LCDM(lmax=10, h=False)
+    sets = []
+    for h, wb, wc, lnAs, ns, _ in X:
+        sets.append(dict(base, h=h, Omega_b0=wb / h**2, Omega_c0=wc / h**2, ln_As1e10=lnAs, ns=ns, Neff=3.046))
+    with ThreadPoolExecutor(max_workers=16) as ex:   # the background solves release the GIL
+        zs = list(ex.map(lambda a: z_re_for_optical_depth(M, a[0], a[1], xtol=1e-6), zip(sets, X[:, 5])))
+    for p, z in zip(sets, zs):
+        p["z_re"] = z
+    return M, sets
+
+
+def workload(name):
+    """(model, parameter sets, k in 1/Mpc, description) of a named workload."""
+    from symboltz import models
     base = {"h": 0.67, "Omega_b0": 0.0224 / 0.67**2, "Omega_c0": 0.120 / 0.67**2, "T0": 2.7255, "YHe": 0.245,
             "ln_As1e10": math.log(2.1e-9 * 1e10), "ns": 0.966, "z_re": 7.7}
+    if name == "cfg2":
+        M, sets = lhs_sets(base, 128)
+        ks = np.logspace(-5, 1, 512)
+        desc = ("LCDM massless nu lmax 10, n=38; 128 Latin-hypercube parameter sets (seed 0: h, omega_b, "
+                "omega_c, ln 1e10 As, ns, tau_reion via a z_re root-find) x 512 k log-spaced in [1e-5, 10]/Mpc "
+                "= 65536 instances in one launch; rtol=atol=1e-5; P(k) at z=0")
+        return M, sets, ks, desc
     if name == "cfg1":
         M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
         pars = dict(base, Neff=2.046, N=1, m_eV=0.06)
-        ks = k_dimensionless(np.logspace(-5, 1, 1024), base["h"])
+        ks = np.logspace(-5, 1, 1024)
         desc = ("LCDM + 1 massive nu (0.06 eV, 4 bins, lmax 10), photon/massless-nu lmax 16, n=100; "
                 "1024 k log-spaced in [1e-5, 10]/Mpc; rtol=atol=1e-5; P(k) at z=0 and CMB sources "
                 "(S_T, S_E, S_psi) on a 2000-point cosgrid of conformal time")
     elif name == "cfg0":
         M = models.LCDM(lmax=10, h=False)
         pars = dict(base, Neff=3.046)
-        ks = k_dimensionless(np.logspace(-4, 0, 100), base["h"])
+        ks = np.logspace(-4, 0, 100)
         desc = "LCDM massless nu lmax 10, n=38; 100 k log-spaced in [1e-4, 1]/Mpc; rtol=atol=1e-5"
     else:
         raise SystemExit(f"unknown workload {name}")
-    return M, pars, ks, desc
+    return M, [pars], ks, desc
 
 
 def main():
@@ -79,7 +107,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--algo", choices=["wave", "lane"], default="wave",
                     help="wave: one wavefront per k-mode (default); lane: one lane per k-mode")
-    ap.add_argument("--cpu-sample", type=int, default=1, help="oracle baseline: every N-th mode (1: all)")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="oracle baseline: every N-th instance (0: all for one set, every 16th for many sets)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sources", action="store_true", help="P(k) only: skip the CMB sources")
     ap.add_argument("--n-tau", type=int, default=2000, help="points of the source time grid")
@@ -108,14 +137,22 @@ def main():
 
     if _lib.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
-    M, pars, ks, desc = workload(args.workload)
-    prob = CosmologyProblem(M, pars)
-    t_bg = time.perf_counter()
-    bg = solvebg(prob)
-    t_bg = time.perf_counter() - t_bg
+    from symboltz.constants import k_dimensionless
+    M, sets, ks_mpc, desc = workload(args.workload)
     gm = GpuModel(M)
-    gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
-    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), bg.t[0], bg.t[-1])
+    t_bg = time.perf_counter()
+    bgs, inst_set, ks_l, t0_l = [], [], [], []
+    for si, pars in enumerate(sets):
+        bg = solvebg(CosmologyProblem(M, pars))
+        gm.upload_background(si, bg.t, bg.u, bg.du, bg.P)
+        k_s = k_dimensionless(ks_mpc, pars["h"])
+        bgs.append(bg)
+        inst_set.append(np.full(len(k_s), si, dtype=np.int32))
+        ks_l.append(k_s)
+        t0_l.append(np.clip(np.minimum(1e-2 / k_s, 1e-4), bg.t[0], bg.t[-1]))
+    t_bg = (time.perf_counter() - t_bg) / len(sets)
+    bg = bgs[0]
+    inst_set, ks, t0 = np.concatenate(inst_set), np.concatenate(ks_l), np.concatenate(t0_l)
     algo = ALGO_WAVE if args.algo == "wave" else ALGO_LANE
     kname = "sbz_wave_kernel" if args.algo == "wave" else "sbz_rodas5p_kernel"
     taus = None
@@ -124,9 +161,10 @@ def main():
         x = np.linspace(0.0, 0.5, args.n_tau)
         cospi = np.where(x == 0.5, 0.0, np.cos(np.pi * x))   # cospi(1/2) = 0 exactly, as Julia's cospi
         taus = a0 + (b0 - a0) * (1.0 - cospi)
-    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, lanes=args.lanes, algo=algo, saveat=taus,
+    plan = gm.plan(inst_set, ks, t0, lanes=args.lanes, algo=algo, saveat=taus,
                    save=SAVE_SOURCES if taus is not None else 0)
-    log(f"workload {args.workload}: n={gm.n}, {len(ks)} modes, background {t_bg*1e3:.1f} ms ({len(bg.t)} knots)")
+    log(f"workload {args.workload}: n={gm.n}, {len(sets)} set(s), {len(ks)} instances, background "
+        f"{t_bg*1e3:.1f} ms per set ({len(bg.t)} knots in set 0)")
     for w in range(args.warmup):
         ms = plan.run()
         plan.pk()
@@ -156,23 +194,34 @@ def main():
     parity = None
     if not args.no_cpu:
         from oracle import oracle
-        O = oracle.Oracle(M, bg.P, bg.t, bg.u, bg.du)
-        sel = np.arange(0, n_modes, args.cpu_sample)
+        step = args.cpu_sample if args.cpu_sample > 0 else (16 if len(sets) > 1 else 1)
+        sel = np.arange(0, n_modes, step)
         cores = os.cpu_count() or 1
         cores = min(cores, 16)
-        tc = time.perf_counter()
-        o = O.solve(ks[sel], t0[sel], nthreads=cores, saveat=taus, sources=taus is not None)
-        tc = time.perf_counter() - tc
-        po = np.array([O.p0(k) * O.delta_m(k, bg.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks[sel])])
+        tc = 0.0
+        po = np.zeros(len(sel))
+        o_src = []
+        for si in range(len(sets)):   # one oracle per parameter set, each on its sampled instances
+            m = sel[inst_set[sel] == si]
+            if len(m) == 0:
+                continue
+            b = bgs[si]
+            O = oracle.Oracle(M, b.P, b.t, b.u, b.du)
+            t_c = time.perf_counter()
+            o = O.solve(ks[m], t0[m], nthreads=cores, saveat=taus, sources=taus is not None)
+            tc += time.perf_counter() - t_c
+            po[np.searchsorted(sel, m)] = [O.p0(k) * O.delta_m(k, b.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks[m])]
+            if taus is not None:
+                o_src.append(o["src"])
         parity = float(np.max(np.abs(pk[sel] / po - 1)))
         sp = ""
         if taus is not None:   # per mode and source: max |S_gpu - S_cpu| / max |S_cpu| over the grid
-            ref = o["src"]
+            ref = np.concatenate(o_src)
             sc = np.abs(ref).max(axis=1, keepdims=True) + 1e-300
             src_parity = float(np.max(np.abs(src[sel] - ref) / sc))
             sp = f"; sources max |ΔS|/max|S| = {src_parity:.2e}"
         cpu = {"value": len(sel) / tc, "unit": "k-mode solves/s", "cores": cores, "kind": "port",
-               "sample": f"{'all' if args.cpu_sample == 1 else f'every {args.cpu_sample}th'} mode(s) of the "
+               "sample": f"{'all' if step == 1 else f'every {step}th'} instance(s) of the "
                          f"workload ({len(sel)} modes, "
                          f"{'with' if taus is not None else 'without'} the sources), {tc:.1f} s on {cores} host "
                          f"threads; max |P_gpu/P_cpu - 1| = {parity:.2e}{sp}"}
@@ -192,7 +241,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "description": desc, "algo": args.algo,
+        "config": {"workload": args.workload, "description": desc, "algo": args.algo, "n_sets": len(sets),
                    "n_tau_sources": 0 if taus is None else len(taus), "n_modes": n_modes, "n_unknowns": gm.n,
                    "background_ms": t_bg * 1e3, "all_success": ok},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

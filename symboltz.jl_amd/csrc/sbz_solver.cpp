@@ -279,6 +279,15 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->block = 64;
     // lane-per-mode: `lanes` concurrent modes; wave-per-mode: `lanes` concurrent wavefronts
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
+    if (opts->lanes <= 0 && p->algo == SBZ_ALGO_WAVE) {
+        // persistent grid: as many wavefronts as can be resident at once, each pulling instances
+        int per_cu = 0, ncu = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->k_wave, 64, 0) != hipSuccess || per_cu < 1)
+            per_cu = 4;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess || ncu < 1)
+            ncu = 256;
+        lanes = std::min(n_inst, per_cu * ncu);
+    }
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
     if (p->algo == SBZ_ALGO_LANE) lanes = (lanes + p->block - 1) / p->block * p->block;
     p->lanes = lanes;

@@ -67,11 +67,12 @@ def build_probe(M):
 
 def standard_models():
     """Models used by the tests, the smoke check and the benchmark."""
-    from .models import LCDM
+    from .models import LCDM, w0waCDM
     return [
         LCDM(lmax=5),                                     # reference test fixture (runtests.jl:14-15)
         LCDM(lmax=10, h=False),                           # ΛCDM with massless ν only, lmax 10
         LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4),     # photons/ν lmax 16, one massive ν (4 bins, lmax 10)
+        w0waCDM(lmax=6),                                  # w0wa dark-energy fluid with massive ν
     ]
 
 

@@ -211,3 +211,35 @@ def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=
     pk = plan.pk().reshape(len(par_sets), len(ks))
     plan.close()
     return pk, res, bgs
+
+
+def reionization_optical_depth(M, pars, reltol=1e-7, abstol=1e-7):
+    """Optical depth of the reionized electrons: κ(τ0) with the tanh reionization terms minus κ(τ0)
+    without them (both from full background solves; κ is the reference's _κ, baryons.jl:175-177).
+
+    Not in the reference, which has no τ_reion parameter; it serves parameter sets drawn in
+    τ_reion (see `z_re_for_optical_depth`)."""
+    from .models import P_REION
+    P = make_params(M, pars)
+    k_on = _lib.background_solve(P, reltol, abstol)[0][P_KAPPA0]
+    P[P_REION] = 0.0
+    k_off = _lib.background_solve(P, reltol, abstol)[0][P_KAPPA0]
+    return abs(k_on - k_off)
+
+
+def z_re_for_optical_depth(M, pars, tau_reion, z_lo=3.0, z_hi=30.0, xtol=1e-8, reltol=1e-7, abstol=1e-7):
+    """The hydrogen reionization redshift `z_re` whose reionization optical depth is `tau_reion`
+    (Brent's method on `reionization_optical_depth`, which grows monotonically with z_re).
+
+    Not in the reference (extension for parameter sets given in τ_reion)."""
+    from scipy.optimize import brentq
+    from .models import P_REION
+    P = make_params(M, dict(pars, z_re=z_lo))
+    P[P_REION] = 0.0
+    k_off = _lib.background_solve(P, reltol, abstol)[0][P_KAPPA0]
+
+    def g(z):
+        Pz = make_params(M, dict(pars, z_re=z))
+        return abs(_lib.background_solve(Pz, reltol, abstol)[0][P_KAPPA0] - k_off) - tau_reion
+
+    return brentq(g, z_lo, z_hi, xtol=xtol)

@@ -168,3 +168,17 @@ def test_cmb_sources_match_oracle(fixture5):
         for c in range(3):
             scale = np.abs(ref[:, c]).max() + 1e-300
             assert np.max(np.abs(src[i, :, c] - ref[:, c])) <= 1e-4 * scale, (k, c)
+
+
+def test_w0wa_model_matches_oracle():
+    say("case: w0waCDM lmax 6 (w0 = -0.9, wa = 0.1), 10 modes")
+    M = models.w0waCDM(lmax=6)
+    S = Setup(M, dict(models.parameters_Planck18(M), w0=-0.9, wa=0.1))
+    ks = 10 ** np.linspace(np.log10(0.1), np.log10(3000.0), 10)
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g = gpu_solve(S, ks, t0)
+    o = S.O.solve(ks, t0)
+    assert np.all(g["retcode"] == 0) and np.all(o["retcode"] == 0)
+    Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
+    assert np.max(np.abs(g["pk"] / Po - 1)) < 1e-4
+    assert rel_state_err(g["u"], o["u"]) < 1e-4

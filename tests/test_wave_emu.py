@@ -43,3 +43,16 @@ def test_wave_kernel_emulated_massive_nu_lmax16():
     o = S.O.solve(ks, t0)
     assert g["retcode"][0] == 0
     assert rel(g["u"], o["u"]) < 1e-4
+
+
+def test_wave_kernel_emulated_w0wa():
+    """The w0wa dark-energy fluid model (dark_energy.jl:43-64) through the wave kernel."""
+    M = models.w0waCDM(lmax=6)
+    S = Setup(M, dict(models.parameters_Planck18(M), w0=-0.9, wa=0.1))
+    E = WaveEmu(model_source(M).full_src)
+    ks = np.array([0.8, 40.0])
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g = E.solve(S.P, S.t, S.u, S.du, ks, t0)
+    o = S.O.solve(ks, t0)
+    assert np.all(g["retcode"] == 0) and np.all(o["retcode"] == 0)
+    assert rel(g["u"], o["u"]) < 1e-4
