@@ -38,7 +38,7 @@ def log(*a):
 def measured_traffic(args):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary of the same workload (scaled by a copy-kernel calibration there), or None."""
-    if args.algo != "wave" or args.workload != "cfg1" or args.no_sources or args.n_tau != 2000:
+    if args.algo not in ("wave", "auto") or args.workload != "cfg1" or args.no_sources or args.n_tau != 2000:
         return None
     f = ROOT / "profiles" / "sbz_wave_kernel_cfg1_hbm.md"
     if not f.exists():
@@ -105,8 +105,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg1")
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--algo", choices=["wave", "lane"], default="wave",
-                    help="wave: one wavefront per k-mode (default); lane: one lane per k-mode")
+    ap.add_argument("--algo", choices=["auto", "wave", "wave2", "lane"], default="auto",
+                    help="auto: timed choice of the wavefront variant (default); wave: one wavefront per "
+                         "k-mode, one resident per SIMD; wave2: two resident per SIMD; lane: one lane per k-mode")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="oracle baseline: every N-th instance (0: all for one set, every 16th for many sets)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -133,7 +134,7 @@ def main():
 
     from symboltz import _lib
     from symboltz.problem import CosmologyProblem, solvebg
-    from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_LANE, SAVE_SOURCES
+    from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_WAVE2, ALGO_LANE, ALGO_AUTO, SAVE_SOURCES
 
     if _lib.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
@@ -153,8 +154,7 @@ def main():
     t_bg = (time.perf_counter() - t_bg) / len(sets)
     bg = bgs[0]
     inst_set, ks, t0 = np.concatenate(inst_set), np.concatenate(ks_l), np.concatenate(t0_l)
-    algo = ALGO_WAVE if args.algo == "wave" else ALGO_LANE
-    kname = "sbz_wave_kernel" if args.algo == "wave" else "sbz_rodas5p_kernel"
+    algo = {"auto": ALGO_AUTO, "wave": ALGO_WAVE, "wave2": ALGO_WAVE2, "lane": ALGO_LANE}[args.algo]
     taus = None
     if not args.no_sources and args.workload == "cfg1":   # cosgrid(a, b; length) (src/utils.jl:285) over [max(τ_bg0, 0.01), τ0]
         a0, b0 = max(bg.t[0], 1e-2), bg.t[-1]
@@ -163,8 +163,10 @@ def main():
         taus = a0 + (b0 - a0) * (1.0 - cospi)
     plan = gm.plan(inst_set, ks, t0, lanes=args.lanes, algo=algo, saveat=taus,
                    save=SAVE_SOURCES if taus is not None else 0)
+    chosen = {0: "wave", ALGO_WAVE: "wave", ALGO_WAVE2: "wave2", ALGO_LANE: "lane"}[plan.algo]
+    kname = {"wave": "sbz_wave_kernel", "wave2": "sbz_wave_kernel_o2", "lane": "sbz_rodas5p_kernel"}[chosen]
     log(f"workload {args.workload}: n={gm.n}, {len(sets)} set(s), {len(ks)} instances, background "
-        f"{t_bg*1e3:.1f} ms per set ({len(bg.t)} knots in set 0)")
+        f"{t_bg*1e3:.1f} ms per set ({len(bg.t)} knots in set 0); kernel variant {chosen}")
     for w in range(args.warmup):
         ms = plan.run()
         plan.pk()
@@ -241,7 +243,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "description": desc, "algo": args.algo, "n_sets": len(sets),
+        "config": {"workload": args.workload, "description": desc, "algo": chosen, "n_sets": len(sets),
                    "n_tau_sources": 0 if taus is None else len(taus), "n_modes": n_modes, "n_unknowns": gm.n,
                    "background_ms": t_bg * 1e3, "all_success": ok},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -76,6 +76,7 @@ typedef struct {
 /* Kernel variants. */
 #define SBZ_ALGO_WAVE 1  /* one 64-lane wavefront per mode: chains in lanes, row-parallel border LU (default) */
 #define SBZ_ALGO_LANE 2  /* one lane per mode, state in a strided global workspace */
+#define SBZ_ALGO_WAVE2 3 /* SBZ_ALGO_WAVE compiled for two resident wavefronts per SIMD (throughput) */
 
 /* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number
  * of concurrently resident modes (0: one per instance).  save: SBZ_SAVE_* flags (0 with saveat
@@ -123,6 +124,10 @@ int sbz_plan_fetch_sources(sbz_plan* p, double* out_src);
  * (spectrum_matter, src/observables/fourier.jl:79-101). */
 int sbz_plan_pk(sbz_plan* p, double* pk);
 int sbz_plan_destroy(sbz_plan* p);
+/* Resident capacity of a kernel variant on the model's device: wavefronts (SBZ_ALGO_WAVE,
+ * SBZ_ALGO_WAVE2) or lanes (SBZ_ALGO_LANE) that run at once.  No reference counterpart: the
+ * autotuner (symboltz/autotune.py) uses it to decide whether higher occupancy can pay off. */
+int sbz_model_capacity(sbz_model* m, int algo, int* capacity);
 
 /* One-shot: create a plan, run, fetch, destroy. */
 int sbz_solve_batch(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0, int n_inst,

@@ -640,7 +640,7 @@ SBZ_DEV double sbz_wave_sum(double v) {
     return (sbz_readlane(v, 0) + sbz_readlane(v, 16)) + (sbz_readlane(v, 32) + sbz_readlane(v, 48));
 }
 
-extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) {
+SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     __shared__ WaveLDS L;
     WaveCtx c;
     c.lane = threadIdx.x;
@@ -979,6 +979,13 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a)
         __syncthreads();
     }
 }
+
+// The same integrator compiled twice: for one resident wavefront per SIMD (all 512 registers
+// per lane, fewest spills: lowest latency per mode) and for two (half the registers, a few more
+// spills: the second wavefront hides the first's latency when there are more instances than
+// SIMDs).  SBZ_ALGO_WAVE / SBZ_ALGO_WAVE2 select them.
+extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body(a); }
+extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body(a); }
 
 // P(k, τ0) = P0(k) Δ_m(τ0)² for each instance from its final state (fourier.jl:14-52,90-97).
 extern "C" __global__ void sbz_pk_kernel(const double* __restrict__ params, const double* __restrict__ knots,

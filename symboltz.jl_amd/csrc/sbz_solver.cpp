@@ -35,7 +35,7 @@ struct SbzSetTable {
 struct sbz_model {
     int n = 0, nnz_lu = 0, nw = 0;
     hipModule_t module = nullptr;
-    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr;
+    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -170,6 +170,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
     }
     if (hipModuleGetFunction(&m->k_solve, m->module, "sbz_rodas5p_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_wave, m->module, "sbz_wave_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_wave2, m->module, "sbz_wave_kernel_o2") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk, m->module, "sbz_pk_kernel") != hipSuccess) {
         sbz_set_error("code object lacks the solver kernels");
         (void)hipModuleUnload(m->module);
@@ -235,6 +236,20 @@ static hipError_t upload(T** dst, const T* src, size_t n) {
     return hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
 }
 
+extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
+    sbz_clear_error();
+    if (!m || !capacity || (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_LANE)) {
+        sbz_set_error("sbz_model_capacity: invalid argument");
+        return SBZ_ERR_ARG;
+    }
+    hipFunction_t kf = (algo == SBZ_ALGO_LANE) ? m->k_solve : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+    int per_cu = 0, ncu = 0;
+    HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 64, 0));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
+    *capacity = per_cu * ncu * (algo == SBZ_ALGO_LANE ? 64 : 1);
+    return SBZ_OK;
+}
+
 extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0,
                                int n_inst, const sbz_opts* opts, const double* saveat, int n_save, sbz_plan** out) {
     sbz_clear_error();
@@ -270,7 +285,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->atol = opts->atol;
     p->maxiters = opts->maxiters > 0 ? opts->maxiters : 100000;
     p->save_flags = n_save > 0 ? (opts->save ? opts->save : SBZ_SAVE_STATES) : 0;
-    if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE) {
+    if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2) {
         sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
         delete p;
         return SBZ_ERR_ARG;
@@ -279,11 +294,12 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->block = 64;
     // lane-per-mode: `lanes` concurrent modes; wave-per-mode: `lanes` concurrent wavefronts
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
-    if (opts->lanes <= 0 && p->algo == SBZ_ALGO_WAVE) {
+    if (opts->lanes <= 0 && p->algo != SBZ_ALGO_LANE) {
         // persistent grid: as many wavefronts as can be resident at once, each pulling instances
         int per_cu = 0, ncu = 0;
-        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m->k_wave, 64, 0) != hipSuccess || per_cu < 1)
-            per_cu = 4;
+        hipFunction_t kw = (p->algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kw, 64, 0) != hipSuccess || per_cu < 1)
+            per_cu = (p->algo == SBZ_ALGO_WAVE2) ? 8 : 4;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess || ncu < 1)
             ncu = 256;
         lanes = std::min(n_inst, per_cu * ncu);
@@ -374,7 +390,7 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     a.ws = p->d_ws;
     a.S = p->lanes;
     void* args[] = {&a};
-    hipFunction_t kf = (p->algo == SBZ_ALGO_LANE) ? m->k_solve : m->k_wave;
+    hipFunction_t kf = (p->algo == SBZ_ALGO_LANE) ? m->k_solve : (p->algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
     unsigned grid = (p->algo == SBZ_ALGO_LANE) ? (unsigned)(p->lanes / p->block) : (unsigned)p->lanes;
     HIPCHK(hipEventRecord(p->ev0, m->stream));
     HIPCHK(hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, args, nullptr));

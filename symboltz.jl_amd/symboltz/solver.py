@@ -64,6 +64,8 @@ class _Opts(ctypes.Structure):
 
 ALGO_WAVE = 1
 ALGO_LANE = 2
+ALGO_WAVE2 = 3
+ALGO_AUTO = -1   # host-side: symboltz.autotune picks WAVE or WAVE2
 
 
 SAVE_STATES = 1
@@ -116,7 +118,14 @@ class GpuModel:
         self.n_sets = max(self.n_sets, set_index + 1)
 
     def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0, save=0, algo=0):
-        return Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save, algo)
+        """algo: 0 (library default, one wavefront per SIMD), ALGO_WAVE, ALGO_WAVE2, ALGO_LANE or
+        ALGO_AUTO (timed choice between the two wavefront variants, symboltz.autotune)."""
+        if algo == ALGO_AUTO:
+            from .autotune import choose
+            algo = choose(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, save)
+        p = Plan(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, lanes, save, algo)
+        p.algo = algo
+        return p
 
     def close(self):
         if getattr(self, "h", None):

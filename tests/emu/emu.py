@@ -19,14 +19,17 @@ _llp = ctypes.POINTER(ctypes.c_longlong)
 
 def build(full_src, sanitize=False):
     shim = (HERE / "wave_emu.cpp").read_text()
-    tag = hashlib.sha1((full_src + shim + str(sanitize)).encode()).hexdigest()[:16]
+    tag = hashlib.sha1((full_src + shim + str(sanitize) + Path(__file__).read_text()).encode()).hexdigest()[:16]
     d = BUILD / tag
     lib = d / "libwave_emu.so"
     if lib.exists():
         return lib
     d.mkdir(parents=True, exist_ok=True)
     (d / "emu_kernel.inc").write_text(full_src)
-    cmd = ["g++", "-O1", "-g", "-std=c++17", "-shared", "-fPIC", "-w", "-I", str(d), str(HERE / "wave_emu.cpp"),
+    # -fno-gnu-unique: the kernel's LDS block is a static local of an inline function; without it
+    # the dynamic linker would share one such object between the libraries of different models
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-shared", "-fPIC", "-fno-gnu-unique", "-w", "-I", str(d),
+           str(HERE / "wave_emu.cpp"),
            "-o", str(lib)]
     if sanitize:
         cmd[1:1] = ["-fsanitize=address", "-fno-omit-frame-pointer"]

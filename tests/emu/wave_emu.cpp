@@ -22,7 +22,7 @@
 #define __host__
 #define __forceinline__ inline
 #define __restrict__ __restrict
-#define __launch_bounds__(x)
+#define __launch_bounds__(...)
 #define __constant__ static
 #define __shared__ static
 
