@@ -5,7 +5,8 @@ wavefront per SIMD, every register available: the lowest latency per mode) or SB
 per SIMD: the second hides the first's latency once there are more instances than SIMDs).  When
 the instances fit the one-per-SIMD capacity the choice is WAVE without measuring; otherwise each
 candidate is timed on a cost-stratified sample of the plan's own instances (every m-th of the
-instances sorted by k, sized to saturate the larger capacity) and the faster is kept.  Choices are
+instances sorted by k) large enough (8 × the larger capacity, or all) that its time is set by
+throughput rather than by its single longest mode, and the faster is kept.  Choices are
 cached per model, instance-count bucket and output kind.
 """
 import ctypes
@@ -40,7 +41,7 @@ def choose(gm, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=N
         _cache[key] = ALGO_WAVE
         return ALGO_WAVE
     order = np.argsort(-np.asarray(ks), kind="stable")
-    m = max(1, n // (2 * cap2))
+    m = max(1, n // (8 * cap2))
     sel = np.sort(order[::m])
     times = {}
     for algo in (ALGO_WAVE, ALGO_WAVE2):
