@@ -145,6 +145,29 @@ int sbz_los_integrate(const double* S, long long s_k, long long s_tau, const dou
                       const double* ks, int n_k, const double* y, const double* dy, int n_x, double dx, int n_l,
                       double* out, float* kernel_ms);
 
+/* ---- surrogate MLP (no reference counterpart) -------------------------------------------- */
+
+/* A multilayer perceptron for the learned surrogate of log P(k) (symboltz/surrogate.py): widths
+ * w[0..n_layers], GELU after every hidden layer, BF16 weights and activations in MFMA GEMMs with
+ * FP32 accumulation, FP32 master weights and Adam.  max_batch bounds every batch; weights are
+ * He-uniform from `seed`, biases zero.  Parameters are laid out as all weight matrices ([w_l][w_l+1]
+ * row-major, layer by layer) followed by all biases. */
+typedef struct sbz_mlp sbz_mlp;
+int sbz_mlp_create(int n_layers, const int* widths, int max_batch, unsigned long long seed, sbz_mlp** out);
+int sbz_mlp_destroy(sbz_mlp* m);
+int sbz_mlp_n_params(sbz_mlp* m, long long* n);
+int sbz_mlp_get_params(sbz_mlp* m, float* params);
+int sbz_mlp_set_params(sbz_mlp* m, const float* params);
+/* Mean-squared-error loss and its gradient on one batch (host x [B][w0], t [B][wL]); no update. */
+int sbz_mlp_grad(sbz_mlp* m, const float* x, const float* t, int batch, float* grad, float* loss);
+/* Training data kept on the device (host x [n][w0], t [n][wL]). */
+int sbz_mlp_set_data(sbz_mlp* m, const float* x, const float* t, long long n);
+/* n_batches Adam steps (lr; β = 0.9, 0.999; ε = 1e-8) over consecutive batches of the row list
+ * idx[n_batches*batch]; loss[n_batches] (may be null) receives each batch's loss. */
+int sbz_mlp_train(sbz_mlp* m, const int* idx, int n_batches, int batch, float lr, float* loss, float* kernel_ms);
+/* Inference y [n][wL] = net(x [n][w0]) on host buffers, in batches of max_batch. */
+int sbz_mlp_forward(sbz_mlp* m, const float* x, long long n, float* y, float* kernel_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-f
 def build_library(verbose=False):
     LIBDIR.mkdir(exist_ok=True)
     out = LIBDIR / "libsbz_hip.so"
-    srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_los.hip")]
+    srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_los.hip",
+                                    "sbz_mlp.hip")]
     cmd = ["hipcc", "-fPIC", "-shared", "-I", str(INCLUDE)] + HIP_FLAGS + srcs + ["-lhiprtc", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -73,6 +74,7 @@ def standard_models():
         LCDM(lmax=10, h=False),                           # ΛCDM with massless ν only, lmax 10
         LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4),     # photons/ν lmax 16, one massive ν (4 bins, lmax 10)
         w0waCDM(lmax=6),                                  # w0wa dark-energy fluid with massive ν
+        LCDM(lmax=6),                                     # the surrogate's model (3 massive ν)
     ]
 
 
