@@ -167,21 +167,29 @@ def main():
     ap.add_argument("--data", default=None, help="npz with X, logP, Xh, logPh (skips the generation)")
     ap.add_argument("--save-data", default=None)
     ap.add_argument("--infer-batch", type=int, default=65536)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--generate-only", action="store_true")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)   # noqa: E731
     if args.data:
         d = np.load(args.data)
-        X, Y, Xh, Yh = d["X"], d["logP"], d["Xh"], d["logPh"]
-        gen_s = None
+        X, Y, Xh, Yh = (d[k].astype(np.float64) for k in ("X", "logP", "Xh", "logPh"))
+        gen_s = {k: float(d[k]) for k in ("train_kernel_s", "train_wall_s", "heldout_kernel_s", "heldout_wall_s")
+                 if k in d.files} or None
     else:
         X, Xh = lhs(args.train, seed=1), lhs(args.heldout, seed=2)
         Y, gk, gw = solve_spectra(X, log=log)
         Yh, hk, hw = solve_spectra(Xh, log=log)
         gen_s = dict(train_kernel_s=gk, train_wall_s=gw, heldout_kernel_s=hk, heldout_wall_s=hw)
         if args.save_data:
-            np.savez(args.save_data, X=X, logP=Y, Xh=Xh, logPh=Yh)
+            np.savez(args.save_data, X=X.astype(np.float32), logP=Y.astype(np.float32), Xh=Xh.astype(np.float32),
+                     logPh=Yh.astype(np.float32), **{k: np.array(v) for k, v in gen_s.items()})
+        if args.generate_only:
+            print(json.dumps(dict(gen_s, train_sets=len(X), heldout_sets=len(Xh))), flush=True)
+            return
     S = Surrogate(n_members=args.members)
-    hist, train_s = S.fit(X, Y, epochs=args.epochs, log=log)
+    hist, train_s = S.fit(X, Y, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     mean, disagree, inbox, _ = S.predict(Xh)
     err = np.abs(np.exp(mean - Yh) - 1.0).max(axis=1)          # max |P_sur/P_solver - 1| per set
     # inference throughput at the requested batch (parameters drawn in the box)
