@@ -170,13 +170,24 @@ extern "C" __global__ void sbz_mlp_mse(const float* __restrict__ y, const float*
     if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
 }
 
-// bias gradient: column sums of the FP32 output gradient [M][N]
-extern "C" __global__ void sbz_mlp_colsum(const float* __restrict__ d, int M, int N, float* __restrict__ db) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
+// bias gradient: column sums of the FP32 output gradient [M][N].  A 256-thread block sums a
+// 64-column x 64-row tile (64 columns x 4 row slices, coalesced along the columns), reduces its
+// slices in LDS and adds the tile's sums atomically (db is zeroed first).
+extern "C" __global__ void __launch_bounds__(256) sbz_mlp_colsum(const float* __restrict__ d, int M, int N,
+                                                                float* __restrict__ db) {
+    __shared__ float part[4][64];
+    const int c = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + c, i0 = blockIdx.y * 64;
     float s = 0.f;
-    for (int i = 0; i < M; ++i) s += d[(long long)i * N + j];
-    db[j] = s;
+    if (j < N)
+        for (int i = i0 + sl; i < min(M, i0 + 64); i += 4) s += d[(long long)i * N + j];
+    part[sl][c] = s;
+    __syncthreads();
+    if (sl == 0 && j < N) atomicAdd(&db[j], (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]));
+}
+extern "C" __global__ void sbz_mlp_zero(float* __restrict__ x, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) x[e] = 0.f;
 }
 
 // FP32 copy of a BF16 gradient (the hidden layers' dZ are produced in BF16 for the GEMMs)
@@ -403,7 +414,8 @@ static int loss_and_grad(sbz_mlp* h, int B, float* loss_out) {
     for (int l = L - 1; l >= 0; --l) {
         const int K = h->w[l], N = h->w[l + 1];
         // db_l = column sums of dZ_l (FP32 copy in h->df)
-        hipLaunchKernelGGL(sbz_mlp_colsum, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->df, B, N,
+        hipLaunchKernelGGL(sbz_mlp_zero, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->grad + h->off_b[l], N);
+        hipLaunchKernelGGL(sbz_mlp_colsum, dim3((N + 63) / 64, (B + 63) / 64), dim3(256), 0, h->stream, h->df, B, N,
                            h->grad + h->off_b[l]);
         MCHK(hipGetLastError());
         // dW_l = X_lᵀ dZ_l
