@@ -145,6 +145,31 @@ int sbz_los_integrate(const double* S, long long s_k, long long s_tau, const dou
                       const double* ks, int n_k, const double* y, const double* dy, int n_x, double dx, int n_l,
                       double* out, float* kernel_ms);
 
+/* j_l(x) and j_l'(x) = l/(2l+1) j_{l-1} - (l+1)/(2l+1) j_{l+1} on x_i = i dx (i < n_x - 1) plus one
+ * duplicate padding row, computed on the device and returned in y, dy [n_x][n_l]: the table of
+ * SphericalBesselCache(ls) (src/observables/angular.jl:9-48).  ls strictly ascending. */
+int sbz_bessel_table(const int* ls, int n_l, int n_x, double dx, double* y, double* dy);
+
+/* The C_l stage of spectrum_cmb (src/observables/angular.jl:260-341) after the source solve, on
+ * the device from the coarse sources to C_l:
+ *  - source_kinterp with a ChebyshevInterpolator (src/observables/fourier.jl:232-242,520-532):
+ *    S_coarse [n_c][n_t][n_s] on the nodes y_c (barycentric weights w_c) to the fine wavenumbers
+ *    ks [n_f] whose transformed values are y_f;
+ *  - the Bessel table of the ascending ls [n_l] on x_i = i dx, i < n_x - 1 (angular.jl:9-48);
+ *  - los_integrate of every source over τs [n_t] (angular.jl:109-185), the Limber value for
+ *    l ≥ l_limber[s] (angular.jl:150-177), then Θ /= k for kind 0 (temperature) and
+ *    Θ *= √((l+2)(l+1)l(l-1))/k² for kind 1 (E polarization); kind 2 (lensing) is left as is
+ *    (angular.jl:300-311);
+ *  - C_l[m][l] = Σ_k wk[k] Θ_A(k, l) Θ_B(k, l) for each pair (A, B) = modes[2m], modes[2m+1],
+ *    where the caller's wk holds the k-quadrature weight times (2/π) k² P0(k) (angular.jl:198-223).
+ * The sources are Ss = [k S_T, k² S_E, S_ψ] as spectrum_cmb builds them (angular.jl:296-299).
+ * out_theta [n_s][n_f][n_l] may be NULL.  kernel_ms[5]: Bessel table, k interpolation, LOS,
+ * Limber and factors, k integral. */
+int sbz_cmb_spectra(const double* S_coarse, int n_c, int n_s, const double* taus, int n_t, const double* y_c,
+                    const double* w_c, const double* ks, const double* y_f, const double* wk, int n_f, const int* ls,
+                    int n_l, int n_x, double dx, const int* kind, const int* l_limber, const int* modes, int n_m,
+                    double* out_cl, double* out_theta, float* kernel_ms);
+
 /* ---- surrogate MLP (no reference counterpart) -------------------------------------------- */
 
 /* A multilayer perceptron for the learned surrogate of log P(k) (symboltz/surrogate.py): widths

@@ -13,6 +13,8 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 SRC = HERE / "sbz_oracle.cpp"
 LIB = HERE / "build" / "libsbz_oracle.so"
+CMB_SRC = HERE / "cmb_oracle.cpp"
+CMB_LIB = HERE / "build" / "libcmb_oracle.so"
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -21,12 +23,13 @@ _L = None
 
 
 def build(force=False):
-    """Compile the oracle with g++ (the recipe is also in oracle/Makefile)."""
+    """Compile the oracle libraries with g++ (the perturbation stage and the CMB stage)."""
     LIB.parent.mkdir(exist_ok=True)
-    if force or not LIB.exists() or LIB.stat().st_mtime < SRC.stat().st_mtime:
-        cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-shared", "-fPIC", "-pthread", str(SRC), "-o",
-               str(LIB)]
-        subprocess.run(cmd, check=True)
+    for src, out in ((SRC, LIB), (CMB_SRC, CMB_LIB)):
+        if force or not out.exists() or out.stat().st_mtime < src.stat().st_mtime:
+            cmd = ["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-shared", "-fPIC", "-pthread", str(src), "-o",
+                   str(out)]
+            subprocess.run(cmd, check=True)
     return LIB
 
 

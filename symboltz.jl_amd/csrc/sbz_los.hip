@@ -9,6 +9,8 @@
 // the table once per (k, τ) row: it is bound by HBM/L2 bandwidth, not arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "sbz_internal.h"
 
 #define LOS_CHUNK 256
@@ -72,9 +74,9 @@ extern "C" int sbz_los_integrate(const double* S, long long s_k, long long s_tau
         sbz_set_error("sbz_los_integrate: invalid argument");
         return SBZ_ERR_ARG;
     }
-    for (int i = 1; i < n_tau; ++i)
-        if (!(taus[i] > taus[i - 1])) {
-            sbz_set_error("sbz_los_integrate: τs must be sorted in ascending order");
+    for (int i = 0; i < n_tau; ++i)
+        if (!std::isfinite(taus[i]) || (i > 0 && !(taus[i] > taus[i - 1]))) {
+            sbz_set_error("sbz_los_integrate: τs must be finite and sorted in ascending order");
             return SBZ_ERR_ARG;
         }
     double kmax = 0.0;

@@ -3,37 +3,72 @@
 Restates `SphericalBesselCache` (src/observables/angular.jl:9-48): j_l(x) and j_l'(x) tabulated on
 a uniform x grid from 0 to xmax = 20 l_max with spacing close to 2π/15, padded with one duplicate
 point, read by cubic Hermite interpolation.  The table is stored [x][l] (l contiguous) as the
-reference does, which is also the coalesced order for the GPU kernel (one thread per l).
-Values come from scipy.special.spherical_jn (the reference uses Bessels.jl); j_l' uses the
-recurrence j_l' = l/(2l+1) j_{l-1} - (l+1)/(2l+1) j_{l+1} (angular.jl:60).
+reference does, which is also the coalesced order for the GPU kernels (threads walk l).
+
+The values are computed on the GPU (`sbz_bessel_table`, three-term recurrence with Miller's
+downward form where l exceeds x) the first time `y` or `dy` is read; the C_l pipeline
+(`symboltz.cmb.spectrum_cmb`) builds the same table on the device itself and never copies it.
+j_l' = l/(2l+1) j_{l-1} - (l+1)/(2l+1) j_{l+1} as at angular.jl:60.
 """
+import ctypes
 import math
 
 import numpy as np
-from scipy.special import spherical_jn
+
+from . import _lib
 
 
 class SphericalBesselCache:
     def __init__(self, ls, xmax=None, dx=2 * math.pi / 15):
         self.l = np.asarray(ls)
+        if self.l.ndim != 1 or len(self.l) == 0 or np.any(np.diff(self.l) <= 0) or self.l[0] < 0:
+            raise ValueError("ls must be a non-empty, strictly ascending vector of non-negative integers")
         if xmax is None:
             xmax = 20.0 * float(self.l[-1])
-        n = int(xmax / dx)
-        xs = np.linspace(0.0, xmax, n)
-        self.dx = xs[1] - xs[0]
-        self.x = np.concatenate([xs, xs[-1:]])
-        L = self.l.astype(np.float64)[None, :]
-        X = self.x[:, None]
-        self.y = np.ascontiguousarray(spherical_jn(self.l[None, :], X))
-        jm = np.where(self.l[None, :] > 0, spherical_jn(np.maximum(self.l - 1, 0)[None, :], X), 0.0)
-        jp = spherical_jn(self.l[None, :] + 1, X)
-        self.dy = np.ascontiguousarray(np.where(self.l[None, :] > 0, L / (2 * L + 1) * jm - (L + 1) / (2 * L + 1) * jp, -jp))
+        n = int(xmax / dx)                       # trunc(Int, (xmax - xmin) / dx) points from 0 to xmax
+        self.dx = xmax / (n - 1)                 # the resulting step, as the reference keeps it
+        self.n_x = n + 1                         # plus the duplicate padding point
+        self.xmax = xmax
+        self._y = self._dy = None
+
+    @property
+    def x(self):
+        x = np.arange(self.n_x, dtype=np.float64) * self.dx
+        x[-1] = x[-2]
+        return x
+
+    def _table(self):
+        if self._y is None:
+            L = _lib.lib()
+            dp = ctypes.POINTER(ctypes.c_double)
+            L.sbz_bessel_table.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           dp, dp]
+            ls = np.ascontiguousarray(self.l, dtype=np.int32)
+            y = np.zeros((self.n_x, len(ls)))
+            dy = np.zeros((self.n_x, len(ls)))
+            _lib.check(L.sbz_bessel_table(_lib.ptr(ls, ctypes.c_int), len(ls), self.n_x, self.dx, _lib.ptr(y),
+                                          _lib.ptr(dy)), "sbz_bessel_table")
+            self._y, self._dy = y, dy
+        return self._y, self._dy
+
+    @property
+    def y(self):
+        return self._table()[0]
+
+    @property
+    def dy(self):
+        return self._table()[1]
 
     def __call__(self, il, x):
         """Hermite-interpolated j_l(x) for the il-th tabulated l (angular.jl:40-50)."""
+        y, dy = self._table()
         w = np.asarray(x) / self.dx
-        i = np.minimum(w.astype(np.int64), len(self.x) - 2)
+        i = np.minimum(w.astype(np.int64), self.n_x - 2)
         w = w - i
         wm1 = w - 1.0
-        return ((1 + 2 * w) * wm1 * wm1 * self.y[i, il] + w * w * (3 - 2 * w) * self.y[i + 1, il] +
-                w * wm1 * (wm1 * self.dy[i, il] + w * self.dy[i + 1, il]) * self.dx)
+        return ((1 + 2 * w) * wm1 * wm1 * y[i, il] + w * w * (3 - 2 * w) * y[i + 1, il] +
+                w * wm1 * (wm1 * dy[i, il] + w * dy[i + 1, il]) * self.dx)
+
+    def __repr__(self):
+        return (f"jₗ(x) Hermite interpolation cache for {self.l[0]} ≤ l ≤ {self.l[-1]} and "
+                f"0.0 ≤ x ≤ {self.x[-1]} ({self.n_x * len(self.l) * 16 / 2**20:.1f} MiB)")

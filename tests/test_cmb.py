@@ -1,0 +1,108 @@
+"""CMB angular power spectra (src/observables/angular.jl:198-359; fourier.jl:419-532).
+
+CPU: the oracle pipeline against the reference's CLASS fixture test/class_Cl.dat with the
+reference's own gate (rtol 2e-3 norm-wise for TT, EE and ψψ, test/runtests.jl:879-886), the
+Chebyshev interpolator against polynomial exactness, and the spline quadrature weights against
+scipy's natural cubic spline integral.
+GPU: the product pipeline (`symboltz.cmb.spectrum_cmb`) against the same fixture and against the
+oracle pipeline; its device kernels against the oracle's on the same coarse sources."""
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+from conftest import GOLDEN
+from oracle import oracle_cmb as oc
+from symboltz.cmb import ChebyshevInterpolator, cosgrid, fk_inv_tanh, fk_tanh, lingrid, spline_integral_weights
+from symboltz.problem import CosmologyProblem, spectrum_primordial
+
+
+def fixture_ls():
+    d = np.loadtxt(GOLDEN / "class_Cl.dat")
+    lsc = d[:, 0]
+    ls = np.unique(np.round(np.exp(np.linspace(np.log(lsc[0]), np.log(lsc[-1]), 200))).astype(int))
+    return d, lsc, ls
+
+
+def normwise(a, b):
+    # Julia's isapprox(a, b; rtol): norm(a - b) ≤ rtol max(norm(a), norm(b))
+    return np.linalg.norm(a - b) / max(np.linalg.norm(a), np.linalg.norm(b))
+
+
+def test_spline_integral_weights_match_natural_spline_integral():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([[0.0], np.sort(rng.uniform(0.01, 100.0, 400))])
+    y = rng.standard_normal((len(x), 3))
+    ref = CubicSpline(x, y, bc_type="natural").integrate(x[0], x[-1])
+    assert np.max(np.abs(spline_integral_weights(x) @ y - ref)) < 1e-12 * np.abs(y).sum()
+    # exact for a straight line (natural spline reproduces it) on any grid
+    assert abs(spline_integral_weights(x) @ (2 * x + 1) - (x[-1] ** 2 + x[-1])) < 1e-9
+
+
+def test_chebyshev_interpolator_is_exact_on_polynomials():
+    c = ChebyshevInterpolator(1e-2, 1e4, 30, f=fk_tanh, finv=fk_inv_tanh)
+    assert c.xs[0] == 1e4 and c.xs[-1] == 1e-2 and np.all(np.diff(c.xs) < 0)
+    rng = np.random.default_rng(1)
+    coef = rng.standard_normal(31)
+    p = np.polynomial.Polynomial(coef, domain=[c.ys[-1], c.ys[0]])
+    for x in rng.uniform(1e-2, 1e4, 50):
+        assert abs(c(p(c.ys), x) - p(float(fk_tanh(x)))) < 1e-9 * np.abs(coef).sum()
+    # the oracle's interpolation (C++) agrees with the product's host evaluation
+    ch = oc.Cheb(1e-2, 1e4, 30, f=lambda k: np.tanh(k / 2000.0), finv=lambda y: 2000.0 * np.arctanh(y))
+    assert np.array_equal(ch.xs, c.xs) and np.array_equal(ch.ws, c.ws)
+    xs = rng.uniform(1e-2, 1e4, 20)
+    o = oc.kinterp(p(c.ys)[:, None], ch, xs)[:, 0]
+    assert np.max(np.abs(o - np.array([c(p(c.ys), x) for x in xs]))) < 1e-12 * np.abs(coef).sum()
+    # at a node the node value is returned
+    assert oc.kinterp(p(c.ys)[:, None], ch, [c.xs[5]])[0, 0] == p(c.ys)[5]
+
+
+def test_grids_follow_the_reference():
+    g = lingrid(0.01, 1e4, step=np.pi / 3.3)
+    assert g[0] == 0.01 and g[-1] == 1e4 and len(g) == int(np.ceil((1e4 - 0.01) / (np.pi / 3.3))) + 1
+    c = cosgrid(0.0, 1.0, length=300)
+    assert c[0] == 0.0 and c[-1] == 1.0 and np.all(np.diff(np.diff(c)) > 0)
+    assert oc.cosgrid(0.0, 1.0, 300)[-1] == 1.0
+
+
+def test_oracle_cmb_spectra_match_class_fixture(fixture5):
+    S = fixture5
+    d, lsc, ls = fixture_ls()
+    prob = CosmologyProblem(S.M, S.pars)
+    D, det = oc.spectrum_cmb(S.O, lambda k: spectrum_primordial(k, prob), ["TT", "EE", "ψψ"], ls, ls_fine=lsc)
+    for i, name in enumerate(["TT", "EE", "ψψ"]):
+        err = normwise(D[:, i], d[:, 1 + i])
+        print(f"oracle {name}: norm-wise {err:.2e}")
+        assert err < 2e-3, name
+
+
+@pytest.mark.gpu
+def test_gpu_cmb_spectra_match_class_fixture_and_oracle(fixture5):
+    from symboltz.bessel import SphericalBesselCache
+    from symboltz.cmb import spectrum_cmb
+    print("case: C_l TT/EE/ψψ for the CLASS fixture, 131 source modes, l ≤ 2500", flush=True)
+    S = fixture5
+    d, lsc, ls = fixture_ls()
+    prob = CosmologyProblem(S.M, S.pars)
+    jl = SphericalBesselCache(ls)
+    det, tm = {}, {}
+    D = spectrum_cmb(["TT", "EE", "ψψ"], prob, jl, lsc, normalization="Dl", timing=tm, details=det)
+    print({k: v for k, v in tm.items() if k.endswith("_ms")}, flush=True)
+    for i, name in enumerate(["TT", "EE", "ψψ"]):
+        err = normwise(D[:, i], d[:, 1 + i])
+        print(f"gpu {name}: norm-wise {err:.2e}")
+        assert err < 2e-3, name
+    # the same pipeline on the CPU oracle (its own source solve): the two differ only by the
+    # integration of the sources (same algorithm)
+    Do, dto = oc.spectrum_cmb(S.O, lambda k: spectrum_primordial(k, prob), ["TT", "EE", "ψψ"], ls, ls_fine=lsc)
+    for i in range(3):
+        assert normwise(D[:, i], Do[:, i]) < 1e-4
+    # kernels alone: the oracle's interpolation and LOS on the GPU's own coarse sources
+    Sf = oc.kinterp(det["S_coarse"], oc.Cheb(1e-2, 1e4, 130, f=lambda k: np.tanh(k / 2000.0),
+                                             finv=lambda y: 2000.0 * np.arctanh(y)), det["ks_fine"])
+    Th = oc.los(Sf, det["taus"], det["ks_fine"], ls, oc.bessel_table(ls), [oc.INT_MAX, oc.INT_MAX, 10])
+    L = ls.astype(np.float64)
+    Th[0] /= det["ks_fine"][:, None]
+    Th[1] *= np.sqrt((L + 2) * (L + 1) * L * (L - 1))[None, :] / det["ks_fine"][:, None] ** 2
+    for s in range(3):
+        scale = np.abs(Th[s]).max()
+        assert np.max(np.abs(det["Th"][s] - Th[s])) <= 1e-10 * scale, s
