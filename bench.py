@@ -98,6 +98,91 @@ def workload(name):
     return M, [pars], ks, desc
 
 
+def bench_cmb(args):
+    """C_l workload: spectrum_cmb([TT, EE, ψψ]) of the reference's CLASS-comparison model (ΛCDM
+    lmax 5; test/runtests.jl:14-16,879-886) for a batch of cosmologies: set 0 is Planck18 (checked
+    against test/class_Cl.dat), the others a seeded Latin hypercube over h, ω_b, ω_c, ln 1e10 As, ns.
+    D_l at every l in [2, 2500] from 152 cached l.  A step is the whole pipeline for the batch:
+    background solves (host), 131 source modes per set on 300 τ in one launch, the device C_l stage
+    per set (Bessel table, k interpolation to ~10,000 k, LOS, Limber, k integral) and the l spline."""
+    from scipy.stats import qmc
+    from symboltz import models
+    from symboltz.bessel import SphericalBesselCache
+    from symboltz.cmb import spectrum_cmb_sets
+    from symboltz.problem import CosmologyProblem, remake, spectrum_primordial
+    M = models.LCDM(lmax=5)
+    pars = models.parameters_Planck18(M)
+    base = CosmologyProblem(M, pars)
+    X = qmc.scale(qmc.LatinHypercube(d=5, seed=0).random(max(args.cmb_sets - 1, 1)),
+                  [0.60, 0.020, 0.10, 2.9, 0.92], [0.75, 0.024, 0.14, 3.2, 1.00])
+    probs = [base] + [remake(base, dict(h=h, Omega_b0=wb / h**2, Omega_c0=wc / h**2, ln_As1e10=la, ns=ns))
+                      for h, wb, wc, la, ns in X[:args.cmb_sets - 1]]
+    d = np.loadtxt(ROOT / "tests" / "golden" / "class_Cl.dat")
+    lsc = d[:, 0]
+    ls = np.unique(np.round(np.exp(np.linspace(np.log(lsc[0]), np.log(lsc[-1]), 200))).astype(int))
+    jl = SphericalBesselCache(ls)
+    modes = ["TT", "EE", "ψψ"]
+    for w in range(args.warmup):
+        spectrum_cmb_sets(modes, probs, jl, lsc, normalization="Dl")
+        log(f"warmup {w} done")
+    tms = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        tm, det = {}, []
+        D = spectrum_cmb_sets(modes, probs, jl, lsc, normalization="Dl", timing=tm, details=det)
+        tms.append(tm)
+        log(f"step {s}: " + ", ".join(f"{k} {v:.2f}" for k, v in tm.items() if k.endswith("_ms")))
+    elapsed = (time.perf_counter() - t_start) / args.steps
+    nw = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(a), np.linalg.norm(b)))
+    vs_class = {m: nw(D[0][:, i], d[:, 1 + i]) for i, m in enumerate(modes)}
+    log(f"set 0 vs class_Cl.dat: {vs_class}")
+    # LOS kernel: per (k, τ, l) it reads 4 table doubles (32 B) and does a Hermite read (12 flops)
+    # plus 2 flops per source
+    los_ms = float(np.mean([t["los_ms"] for t in tms])) / len(probs)
+    nk, nt, nl = len(det[0]["ks_fine"]), len(det[0]["taus"]), len(ls)
+    src_ms = float(np.mean([t["sources_ms"] for t in tms]))
+    gm = base.gpu
+    steps = int(np.sum(tms[-1]["steps"]))
+    flops = steps * gm.em.flops_step
+    achieved = flops / (src_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle, oracle_cmb
+        from symboltz.problem import solvebg
+        bg = solvebg(base)
+        O = oracle.Oracle(M, bg.P, bg.t, bg.u, bg.du)
+        t_c = time.perf_counter()
+        Do, _ = oracle_cmb.spectrum_cmb(O, lambda k: spectrum_primordial(k, base), modes, ls, ls_fine=lsc)
+        tc = time.perf_counter() - t_c
+        par = {m: nw(D[0][:, i], Do[:, i]) for i, m in enumerate(modes)}
+        cpu = {"value": 1.0 / tc, "unit": "C_l spectrum sets/s", "cores": oracle_cmb._threads(), "kind": "port",
+               "sample": f"set 0 (Planck18) once, {tc:.2f} s: oracle source solve, C++ interpolation and LOS, "
+                         f"scipy k integral; norm-wise |D_gpu - D_cpu| for set 0 = {par}"}
+        log(f"cpu baseline {tc:.2f} s per set; parity {par}")
+    out = {
+        "metric": "CMB C_l spectrum sets/s (TT, EE, psipsi at l = 2..2500, whole pipeline)",
+        "value": len(probs) / elapsed, "unit": "C_l spectrum sets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "cmb", "description": f"LCDM lmax 5; {len(probs)} cosmologies (Planck18 + LHS seed 0) x "
+                   f"{len(det[0]['ks_coarse'])} Chebyshev source modes x {nt} tau in one launch; ~{nk} fine k; "
+                   f"{nl} cached l; D_l at l = 2..2500",
+                   "n_sets": len(probs), "set0_vs_class_Cl_normwise": vs_class,
+                   "stage_ms_per_step": {k: float(np.mean([t[k] for t in tms])) for k in tms[0] if k.endswith("_ms")},
+                   "los_kernel": {"kernel": "sbz_cmb_los_kernel", "ms_per_set": los_ms,
+                                  "table_bytes_read_per_set": 32.0 * nk * nt * nl,
+                                  "table_read_GBs": 32.0 * nk * nt * nl / (los_ms * 1e-3) / 1e9,
+                                  "flops_per_set": 20.0 * nk * nt * nl,
+                                  "TFLOPs": 20.0 * nk * nt * nl / (los_ms * 1e-3) / 1e12}},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "source solve (wave kernel)",
+                     "kernel_ms": src_ms, "flops_per_launch": flops, "flops_per_step": gm.em.flops_step,
+                     "steps_per_launch": steps},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +198,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sources", action="store_true", help="P(k) only: skip the CMB sources")
     ap.add_argument("--n-tau", type=int, default=2000, help="points of the source time grid")
+    ap.add_argument("--cmb-sets", type=int, default=16, help="cosmologies per step of the cmb workload")
     ap.add_argument("--pmc-copy", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -138,6 +224,8 @@ def main():
 
     if _lib.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
+    if args.workload == "cmb":
+        return bench_cmb(args)
     from symboltz.constants import k_dimensionless
     M, sets, ks_mpc, desc = workload(args.workload)
     gm = GpuModel(M)
@@ -191,6 +279,17 @@ def main():
     ok = bool(np.all(res["retcode"] == 0))
     log(f"retcodes ok={ok}; steps/mode min {res['stats'][:,0].min()} max {res['stats'][:,0].max()} "
         f"total {steps_total}; flops/step {gm.em.flops_step}; {achieved:.3f} TFLOP/s")
+
+    # accuracy of the rtol = 1e-5 spectrum against the same GPU solver at rtol = atol = 1e-8 (outside
+    # the timed region)
+    # (high-k modes need ~15x the steps of rtol 1e-5, beyond the default maxiters of 1e5)
+    ref = gm.plan(inst_set, ks, t0, rtol=1e-8, atol=1e-8, maxiters=2_000_000, lanes=args.lanes, algo=plan.algo)
+    ref.run()
+    pk_ref = ref.pk()
+    ref_ok = bool(np.all(ref.fetch()["retcode"] == 0))
+    ref.close()
+    pk_err = float(np.max(np.abs(pk / pk_ref - 1)))
+    log(f"max |P(rtol 1e-5) / P(rtol 1e-8) - 1| = {pk_err:.2e} (reference run success: {ref_ok})")
 
     cpu = None
     parity = None
@@ -250,7 +349,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "description": desc, "algo": chosen, "n_sets": len(sets),
                    "n_tau_sources": 0 if taus is None else len(taus), "n_modes": n_modes, "n_unknowns": gm.n,
-                   "background_ms": t_bg * 1e3, "all_success": ok},
+                   "background_ms": t_bg * 1e3, "all_success": ok,
+                   "pk_max_rel_err_vs_rtol_1e-8": pk_err},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": kname, "kernel_ms": float(np.mean(kernel_ms)),

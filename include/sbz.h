@@ -72,6 +72,9 @@ typedef struct {
 /* What a plan stores at its saveat times. */
 #define SBZ_SAVE_STATES 1   /* full perturbation states [n_inst][n_save][n] */
 #define SBZ_SAVE_SOURCES 2  /* CMB sources S_T, S_E, S_ψ [n_inst][n_save][3] (cosmologies.jl:99-105) */
+#define SBZ_SAVEAT_PER_SET 4 /* saveat is [n_sets][n_save]: one time grid per parameter set (each set's
+                                own τ0), as source_grid with per-cosmology τs needs; no reference
+                                counterpart (the reference solves one cosmology per call) */
 
 /* Kernel variants. */
 #define SBZ_ALGO_WAVE 1  /* one 64-lane wavefront per mode: chains in lanes, row-parallel border LU (default) */

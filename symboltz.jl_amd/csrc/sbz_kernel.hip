@@ -64,8 +64,9 @@ struct SbzSolveArgs {
     int n_inst;
     double rtol, atol;
     long long maxiters;
-    const double* saveat;      // [n_save] shared output times (ascending) or null
+    const double* saveat;      // output times (ascending): [n_save] shared, or [n_sets][n_save] per set
     int n_save;
+    int saveat_stride;         // 0 (shared) or n_save (one row per parameter set)
     double* out_u;             // [n_inst][SBZ_N] final state
     double* out_save;          // [n_inst][n_save][SBZ_N] or null
     double* out_src;           // [n_inst][n_save][3] CMB sources (S_T, S_E, S_ψ) or null
@@ -242,6 +243,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
         if (qi >= (unsigned int)a.n_inst) break;
         const int inst = a.order ? a.order[qi] : (int)qi;
         const int set = a.inst_set[inst];
+        const double* saveat = a.saveat ? a.saveat + (size_t)set * a.saveat_stride : nullptr;
         const double* P = a.params + (size_t)set * SBZ_NPAR;
         const double* kt = a.knots + a.knot_off[set];
         const int nk = a.knot_n[set];
@@ -271,12 +273,12 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
         }
         if (!finite0) ret = SBZ_RET_INITIALFAILURE;
         double* osrc = a.out_src ? a.out_src + (size_t)inst * a.n_save * 3 : nullptr;
-        while ((osave || osrc) && isave < a.n_save && a.saveat[isave] <= t) {
+        while ((osave || osrc) && isave < a.n_save && saveat[isave] <= t) {
             if (osave)
                 for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = vec(su)[i];
             if (osrc && ret == SBZ_RET_SUCCESS) {
                 double S3[3];
-                sbz_sources_at(P, kt, nk, kidx, Cs, k, a.saveat[isave], vec(su), vec(WS_K1), vec(WS_K1 + 1),
+                sbz_sources_at(P, kt, nk, kidx, Cs, k, saveat[isave], vec(su), vec(WS_K1), vec(WS_K1 + 1),
                                vec(WS_K1 + 2), S3);
                 for (int c = 0; c < 3; ++c) osrc[(size_t)isave * 3 + c] = S3[c];
             }
@@ -384,9 +386,9 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
             sbz_bg_plain(P, kt, nk, kidx, tn, B);
             sbz_f(B, Cs, k, u1, f1);
             st_f++;
-            while ((osave || osrc) && isave < a.n_save && a.saveat[isave] <= tn) {
+            while ((osave || osrc) && isave < a.n_save && saveat[isave] <= tn) {
                 // cubic Hermite dense output over the accepted step
-                double th = (a.saveat[isave] - t) / dt;
+                double th = (saveat[isave] - t) / dt;
                 double h00 = (1 + 2 * th) * (1 - th) * (1 - th), h10 = th * (1 - th) * (1 - th);
                 double h01 = th * th * (3 - 2 * th), h11 = th * th * (th - 1);
                 SV ys = vec(WS_K1);
@@ -395,7 +397,7 @@ extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs
                     for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = ys[i];
                 if (osrc) {
                     double S3[3];
-                    sbz_sources_at(P, kt, nk, kidx, Cs, k, a.saveat[isave], ys, vec(WS_K1 + 1), vec(WS_K1 + 2),
+                    sbz_sources_at(P, kt, nk, kidx, Cs, k, saveat[isave], ys, vec(WS_K1 + 1), vec(WS_K1 + 2),
                                    vec(WS_K1 + 3), S3);
                     for (int c = 0; c < 3; ++c) osrc[(size_t)isave * 3 + c] = S3[c];
                 }
@@ -669,6 +671,7 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
         if (qi >= a.n_inst) break;
         const int inst = a.order ? a.order[qi] : qi;
         const int set = a.inst_set[inst];
+        const double* saveat = a.saveat ? a.saveat + (size_t)set * a.saveat_stride : nullptr;
         const double* P = a.params + (size_t)set * SBZ_NPAR;
         const double* kt = a.knots + a.knot_off[set];
         const int nk = a.knot_n[set];
@@ -757,9 +760,9 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
         bool have_step = false;
         for (;;) {
             // ---- saves in (tprev, t] (or at/before the start): Hermite dense output, sources
-            while ((osave || osrc) && isave < a.n_save && a.saveat[isave] <= t) {
+            while ((osave || osrc) && isave < a.n_save && saveat[isave] <= t) {
                 const int un = ub, u0 = ub ^ 1, fn = fb, f0 = fb ^ 1;  // current (n) and previous slots
-                const double th = have_step ? (a.saveat[isave] - tprev) / hprev : 1.0;
+                const double th = have_step ? (saveat[isave] - tprev) / hprev : 1.0;
                 const double h00 = (1 + 2 * th) * (1 - th) * (1 - th), h10 = th * (1 - th) * (1 - th);
                 const double h01 = th * th * (3 - 2 * th), h11 = th * th * (th - 1);
                 for (int i = c.lane; i < SBZ_N; i += 64) {
@@ -773,7 +776,7 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                     if (c.lane == 0) {
                         double Bt[SBZ_NB], dBt[SBZ_NB];
                         int kk = kidx;
-                        L.ddkd = sbz_bg_tay2(P, kt, nk, kk, a.saveat[isave], Bt, dBt);
+                        L.ddkd = sbz_bg_tay2(P, kt, nk, kk, saveat[isave], Bt, dBt);
 #pragma unroll
                         for (int q = 0; q < SBZ_NB; ++q) { L.B[6][q] = Bt[q]; L.dB[1][q] = dBt[q]; }
                     }

@@ -56,6 +56,7 @@ struct SbzSolveArgsHost {
     long long maxiters;
     const double* saveat;
     int n_save;
+    int saveat_stride;
     double* out_u;
     double* out_save;
     double* out_src;
@@ -68,7 +69,7 @@ struct SbzSolveArgsHost {
 
 struct sbz_plan {
     sbz_model* m = nullptr;
-    int n_inst = 0, n_sets = 0, n_save = 0, lanes = 0, block = 64;
+    int n_inst = 0, n_sets = 0, n_save = 0, saveat_stride = 0, lanes = 0, block = 64;
     double rtol = 1e-5, atol = 1e-5;
     long long maxiters = 100000;
     // device buffers
@@ -270,11 +271,15 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
             return SBZ_ERR_ARG;
         }
     }
-    for (int i = 1; i < n_save; ++i)
-        if (saveat[i] < saveat[i - 1]) {
-            sbz_set_error("sbz_plan_create: saveat must be ascending");
-            return SBZ_ERR_ARG;
-        }
+    // SBZ_SAVEAT_PER_SET: saveat holds one ascending row of n_save times per parameter set
+    const bool per_set = n_save > 0 && (opts->save & SBZ_SAVEAT_PER_SET);
+    const int n_rows = per_set ? n_sets : 1;
+    for (int r = 0; r < n_rows; ++r)
+        for (int i = 1; i < n_save; ++i)
+            if (!(saveat[(size_t)r * n_save + i] >= saveat[(size_t)r * n_save + i - 1])) {
+                sbz_set_error("sbz_plan_create: saveat must be ascending (row %d, entry %d)", r, i);
+                return SBZ_ERR_ARG;
+            }
     (void)hipSetDevice(m->device);
     sbz_plan* p = new sbz_plan();
     p->m = m;
@@ -284,7 +289,9 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->rtol = opts->rtol;
     p->atol = opts->atol;
     p->maxiters = opts->maxiters > 0 ? opts->maxiters : 100000;
-    p->save_flags = n_save > 0 ? (opts->save ? opts->save : SBZ_SAVE_STATES) : 0;
+    p->save_flags = n_save > 0 ? opts->save : 0;
+    if (n_save > 0 && !(p->save_flags & (SBZ_SAVE_STATES | SBZ_SAVE_SOURCES))) p->save_flags |= SBZ_SAVE_STATES;
+    p->saveat_stride = per_set ? n_save : 0;
     if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2) {
         sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
         delete p;
@@ -332,7 +339,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     UP(&p->d_k, inst_k, (size_t)n_inst);
     UP(&p->d_t0, inst_t0, (size_t)n_inst);
     UP(&p->d_order, order.data(), (size_t)n_inst);
-    UP(&p->d_saveat, saveat, (size_t)n_save);
+    UP(&p->d_saveat, saveat, (size_t)n_save * n_rows);
 #undef UP
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_out, sizeof(double) * std::max<size_t>((size_t)n_inst * m->n, 1));
     if (e == hipSuccess && (p->save_flags & SBZ_SAVE_STATES))
@@ -381,6 +388,7 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     a.maxiters = p->maxiters;
     a.saveat = p->n_save ? p->d_saveat : nullptr;
     a.n_save = p->n_save;
+    a.saveat_stride = p->saveat_stride;
     a.out_u = p->d_out;
     a.out_save = p->d_save;
     a.out_src = p->d_src;

@@ -26,7 +26,7 @@ from scipy.interpolate import CubicSpline
 from . import _lib
 from .bessel import SphericalBesselCache
 from .problem import solvebg, spectrum_primordial
-from .solver import SAVE_SOURCES
+from .solver import ALGO_AUTO, SAVE_SOURCES
 
 INT_MAX = 2**31 - 1
 _KIND = {"T": 0, "E": 1, "ψ": 2}
@@ -167,29 +167,52 @@ def _cmb_spectra_gpu(S_coarse, taus, kinterp, ks_fine, wk, jl, kinds, l_limber, 
 def source_grid(prob, taus, ks, bg=None, ptopts=None, timing=None):
     """CMB sources (S_T, S_E, S_ψ) at every τ for every k, [n_k][n_τ][3] (fourier.jl:267-291), from
     one GPU launch; modes start at the background start (ptivini = -Inf)."""
-    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
     if bg is None:
         bg = solvebg(prob)
-    taus = np.asarray(taus, dtype=np.float64)
-    if taus.min() < bg.t[0] or taus.max() > bg.t[-1]:
-        raise ValueError("input τs and computed background solution have different timespans")
-    gm = prob.gpu
-    gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
+    return source_grid_sets([prob], [taus], ks, [bg], ptopts, timing)[0]
+
+
+def source_grid_sets(probs, taus_sets, ks, bgs, ptopts=None, timing=None):
+    """`source_grid` for several cosmologies of one model in ONE persistent launch: set s is solved
+    at ks on its own time grid taus_sets[s]; returns [n_sets][n_k][n_τ][3]."""
+    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
+    key = probs[0].M.key()
+    if any(p.M.key() != key for p in probs):
+        raise ValueError("all problems of a batch must share one model")
+    taus_sets = [np.asarray(t, dtype=np.float64) for t in taus_sets]
+    n_t = len(taus_sets[0])
+    for t, bg in zip(taus_sets, bgs):
+        if len(t) != n_t:
+            raise ValueError("every set needs the same number of τ points")
+        if t.min() < bg.t[0] or t.max() > bg.t[-1]:
+            raise ValueError("input τs and computed background solution have different timespans")
+    gm = probs[0].gpu
+    for s, bg in enumerate(bgs):
+        gm.upload_background(s, bg.t, bg.u, bg.du, bg.P)
+    # one saveat row per set held by the model (rows of sets outside this batch are never read)
+    rows = np.repeat(taus_sets[0][None, :], gm.n_sets, axis=0)
+    for s, t in enumerate(taus_sets):
+        rows[s] = t
     ks = np.ascontiguousarray(ks, dtype=np.float64)
-    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, np.full(len(ks), bg.t[0]), rtol=ptopts["reltol"],
-                   atol=ptopts["abstol"], maxiters=ptopts["maxiters"], saveat=taus, save=SAVE_SOURCES)
+    n_s, n_k = len(probs), len(ks)
+    inst_set = np.repeat(np.arange(n_s, dtype=np.int32), n_k)
+    inst_k = np.tile(ks, n_s)
+    inst_t0 = np.repeat(np.array([bg.t[0] for bg in bgs]), n_k)
+    plan = gm.plan(inst_set, inst_k, inst_t0, rtol=ptopts["reltol"], atol=ptopts["abstol"],
+                   maxiters=ptopts["maxiters"], saveat=rows, save=SAVE_SOURCES,
+                   algo=ptopts.get("algo", ALGO_AUTO))
     ms = plan.run()
     res = plan.fetch()
     src = plan.sources()
     plan.close()
     if timing is not None:
         timing["sources_ms"] = ms
-        timing["steps"] = res["stats"][:, 0]
+        timing["steps"] = res["stats"][:, 0] + res["stats"][:, 1]    # accepted + rejected attempts
     bad = np.flatnonzero(res["retcode"] != 0)
     if len(bad):
-        raise RuntimeError(f"Perturbation (mode k = {ks[bad[0]]}) solution failed with return code "
+        raise RuntimeError(f"Perturbation (mode k = {inst_k[bad[0]]}) solution failed with return code "
                            f"{_lib.RETCODE_NAMES[int(res['retcode'][bad[0]])]}")
-    return src
+    return src.reshape(n_s, n_k, n_t, 3)
 
 
 def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_tau0=math.pi, xs=None, taucut=1e-2,
@@ -199,6 +222,19 @@ def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_
     `spectrum_cmb(modes, prob, jl[, ls]; normalization)` (angular.jl:260-359).  Spectra are of
     dimensionless temperature fluctuations; normalization "Dl" returns l(l+1) C_l / 2π.
     Returns [n_l][n_modes]."""
+    dets = [] if details is not None else None
+    out = spectrum_cmb_sets(modes, [prob], jl, ls, normalization, kinterp, dk_tau0, xs, taucut, l_limber, bgopts,
+                            ptopts, timing, dets)[0]
+    if details is not None:
+        details.update(dets[0])
+    return out
+
+
+def spectrum_cmb_sets(modes, probs, jl, ls=None, normalization="Cl", kinterp=None, dk_tau0=math.pi, xs=None,
+                      taucut=1e-2, l_limber=10, bgopts=None, ptopts=None, timing=None, details=None):
+    """`spectrum_cmb` for several cosmologies of one model: the source modes of every set run in one
+    persistent launch (each set on its own τ grid), then the device C_l stage runs per set.
+    Returns [n_sets][n_l][n_modes]; `details`, when a list, receives one dict per set."""
     if not isinstance(jl, SphericalBesselCache):
         raise TypeError("jl must be a SphericalBesselCache")
     modes = [str(m) for m in modes]
@@ -207,6 +243,11 @@ def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_
             raise ValueError(f"Unknown CMB power spectrum mode {m}")
     if normalization not in ("Cl", "Dl"):
         raise ValueError(f"Normalization {normalization} is not Cl or Dl")
+    if ls is not None:
+        ls = np.asarray(ls)
+        if (ls.min(), ls.max()) != (jl.l.min(), jl.l.max()):
+            raise ValueError(f"jl.l and ls have different extrema {(jl.l.min(), jl.l.max())} != "
+                             f"{(ls.min(), ls.max())}")
     joined = "".join(modes)
     present = [s for s in "TEψ" if s in joined]
     if kinterp is None:
@@ -215,49 +256,55 @@ def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_
         else:
             kinterp = ChebyshevInterpolator(1e-2, 2e3, 60)
     bgopts = dict(reltol=1e-7, abstol=1e-7) | (bgopts or {})
-    bg = solvebg(prob, bgopts["reltol"], bgopts["abstol"])
-    tau0 = bg.tau0
-    ks_fine = lingrid(kinterp.minimum(), kinterp.maximum(), step=dk_tau0 / tau0)
-    t = bg.t[bg.t >= taucut]
     xs = cosgrid(0.0, 1.0, length=300) if xs is None else np.asarray(xs, dtype=np.float64)
     if xs[0] != 0 or xs[-1] != 1:
         raise ValueError(f"xs begins with {xs[0]} and ends with {xs[-1]}, but should span [0, 1]")
-    taus = t[0] + (t[-1] - t[0]) * xs
-    if jl.x[-1] < ks_fine[-1] * taus[-1]:
-        raise ValueError("the Bessel cache ends below k_max τ0")
-
-    src = source_grid(prob, taus, kinterp.xs, bg, ptopts, timing)   # [n_c][n_τ][3]
+    if len(probs) > 1:
+        from concurrent.futures import ThreadPoolExecutor   # the host background solves release the GIL
+        with ThreadPoolExecutor(max_workers=min(16, len(probs))) as ex:
+            bgs = list(ex.map(lambda p: solvebg(p, bgopts["reltol"], bgopts["abstol"]), probs))
+    else:
+        bgs = [solvebg(probs[0], bgopts["reltol"], bgopts["abstol"])]
+    taus_sets = []
+    for bg in bgs:
+        t = bg.t[bg.t >= taucut]
+        taus = t[0] + (t[-1] - t[0]) * xs
+        taus_sets.append(taus)
     kc = kinterp.xs
+    src = source_grid_sets(probs, taus_sets, kc, bgs, ptopts, timing)   # [n_sets][n_c][n_τ][3]
     cols = [_KIND[s] for s in present]
-    S = src[:, :, cols].copy()
-    for i, s in enumerate(present):
-        if s == "T":
-            S[:, :, i] *= kc[:, None]
-        elif s == "E":
-            S[:, :, i] *= kc[:, None] ** 2
-    S[:, -1, :] = 0.0                          # χ = 0: weighted by j_l(0) = 0, may be non-finite
     kinds = [_KIND[s] for s in present]
     lim = [l_limber if s == "ψ" else INT_MAX for s in present]
-    P0 = spectrum_primordial(ks_fine, prob)
-    q = spline_integral_weights(np.concatenate([[0.0], ks_fine]))[1:]
-    wk = q * (2 / np.pi) * ks_fine**2 * P0
     pairs = [(present.index(m[0]), present.index(m[1])) for m in modes]
-    cl, th, ms = _cmb_spectra_gpu(S, taus, kinterp, ks_fine, wk, jl, kinds, lim, pairs,
-                                  want_theta=details is not None)
-    if timing is not None:
-        timing.update({f"{k}_ms": v for k, v in ms.items()})
-        timing["n_k_fine"] = len(ks_fine)
-    if details is not None:
-        details.update(taus=taus, ks_fine=ks_fine, ks_coarse=kc, Th=th, S_coarse=S, present=present)
     lc = np.asarray(jl.l, dtype=np.float64)
-    out = cl.T.copy()
-    if normalization == "Dl":
-        out *= (lc * (lc + 1) / (2 * np.pi))[:, None]
-    if ls is not None:
-        ls = np.asarray(ls)
-        if (ls.min(), ls.max()) != (jl.l.min(), jl.l.max()):
-            raise ValueError(f"jl.l and ls have different extrema {(jl.l.min(), jl.l.max())} != "
-                             f"{(ls.min(), ls.max())}")
-        lf = ls.astype(np.float64)
-        out = np.stack([CubicSpline(lc, out[:, i], bc_type="natural")(lf) for i in range(out.shape[1])], axis=1)
-    return out
+    outs = []
+    for si, (prob, bg, taus) in enumerate(zip(probs, bgs, taus_sets)):
+        ks_fine = lingrid(kinterp.minimum(), kinterp.maximum(), step=dk_tau0 / bg.tau0)
+        if jl.x[-1] < ks_fine[-1] * taus[-1]:
+            raise ValueError("the Bessel cache ends below k_max τ0")
+        S = src[si][:, :, cols].copy()
+        for i, s in enumerate(present):
+            if s == "T":
+                S[:, :, i] *= kc[:, None]
+            elif s == "E":
+                S[:, :, i] *= kc[:, None] ** 2
+        S[:, -1, :] = 0.0                      # χ = 0: weighted by j_l(0) = 0, may be non-finite
+        P0 = spectrum_primordial(ks_fine, prob)
+        q = spline_integral_weights(np.concatenate([[0.0], ks_fine]))[1:]
+        wk = q * (2 / np.pi) * ks_fine**2 * P0
+        cl, th, ms = _cmb_spectra_gpu(S, taus, kinterp, ks_fine, wk, jl, kinds, lim, pairs,
+                                      want_theta=details is not None)
+        if timing is not None:
+            for k, v in ms.items():
+                timing[f"{k}_ms"] = timing.get(f"{k}_ms", 0.0) + v
+            timing["n_k_fine"] = len(ks_fine)
+        if details is not None:
+            details.append(dict(taus=taus, ks_fine=ks_fine, ks_coarse=kc, Th=th, S_coarse=S, present=present))
+        out = cl.T.copy()
+        if normalization == "Dl":
+            out *= (lc * (lc + 1) / (2 * np.pi))[:, None]
+        if ls is not None:
+            lf = ls.astype(np.float64)
+            out = np.stack([CubicSpline(lc, out[:, i], bc_type="natural")(lf) for i in range(out.shape[1])], axis=1)
+        outs.append(out)
+    return np.stack(outs)

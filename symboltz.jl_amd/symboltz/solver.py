@@ -70,6 +70,7 @@ ALGO_AUTO = -1   # host-side: symboltz.autotune picks WAVE or WAVE2
 
 SAVE_STATES = 1
 SAVE_SOURCES = 2
+SAVEAT_PER_SET = 4   # saveat given as [n_sets][n_save], one time grid per parameter set
 
 
 def _declare(L):
@@ -148,8 +149,16 @@ class Plan:
         self.t0 = np.ascontiguousarray(t0, dtype=np.float64)
         self.n_inst = len(self.k)
         self.saveat = np.ascontiguousarray(saveat if saveat is not None else np.zeros(0), dtype=np.float64)
-        self.n_save = len(self.saveat)
-        opts = _Opts(rtol, atol, int(maxiters), int(lanes), int(save), int(algo))
+        save = int(save) & (SAVE_STATES | SAVE_SOURCES)
+        flags = save
+        if self.saveat.ndim == 2:   # one row per uploaded parameter set
+            if self.saveat.shape[0] != gm.n_sets:
+                raise ValueError(f"per-set saveat has {self.saveat.shape[0]} rows for {gm.n_sets} parameter sets")
+            flags |= SAVEAT_PER_SET
+            self.n_save = self.saveat.shape[1]
+        else:
+            self.n_save = len(self.saveat)
+        opts = _Opts(rtol, atol, int(maxiters), int(lanes), flags, int(algo))
         self.save = save if (save or self.n_save == 0) else SAVE_STATES
         h = ctypes.c_void_p()
         _lib.check(L.sbz_plan_create(gm.h, _lib.ptr(self.set, ctypes.c_int), _lib.ptr(self.k), _lib.ptr(self.t0),

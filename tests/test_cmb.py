@@ -106,3 +106,29 @@ def test_gpu_cmb_spectra_match_class_fixture_and_oracle(fixture5):
     for s in range(3):
         scale = np.abs(Th[s]).max()
         assert np.max(np.abs(det["Th"][s] - Th[s])) <= 1e-10 * scale, s
+
+
+@pytest.mark.gpu
+def test_gpu_cmb_batch_of_cosmologies_matches_single_runs(fixture5):
+    # several cosmologies in one source launch (each on its own τ grid) give the single-set results
+    from symboltz.bessel import SphericalBesselCache
+    from symboltz.cmb import spectrum_cmb, spectrum_cmb_sets
+    from symboltz.problem import remake
+    print("case: C_l batch of 3 cosmologies (lmax 5), 152 l", flush=True)
+    S = fixture5
+    d, lsc, ls = fixture_ls()
+    base = CosmologyProblem(S.M, S.pars)
+    probs = [base, remake(base, {"h": 0.70, "Omega_c0": 0.25}), remake(base, {"ns": 0.94, "Omega_b0": 0.047})]
+    jl = SphericalBesselCache(ls)
+    Db = spectrum_cmb_sets(["TT", "EE", "TE", "ψψ"], probs, jl, normalization="Dl")
+    for i, p in enumerate(probs):
+        Ds = spectrum_cmb(["TT", "EE", "TE", "ψψ"], p, jl, normalization="Dl")
+        assert np.max(np.abs(Db[i] - Ds)) <= 1e-12 * np.abs(Ds).max(), i
+    # a varied cosmology against the oracle pipeline
+    from symboltz import _lib
+    from oracle import oracle
+    P, t, u, du, _ = _lib.background_solve(probs[1].P)
+    O = oracle.Oracle(S.M, P, t, u, du)
+    Do, _ = oc.spectrum_cmb(O, lambda k: spectrum_primordial(k, probs[1]), ["TT", "EE", "TE", "ψψ"], ls)
+    for i in range(4):
+        assert normwise(Db[1][:, i], Do[:, i]) < 1e-4, i
