@@ -153,6 +153,19 @@ int sbz_los_integrate(const double* S, long long s_k, long long s_tau, const dou
  * SphericalBesselCache(ls) (src/observables/angular.jl:9-48).  ls strictly ascending. */
 int sbz_bessel_table(const int* ls, int n_l, int n_x, double dx, double* y, double* dy);
 
+/* A C_l context for one ascending l list: the Bessel table of sbz_bessel_table is built on the
+ * device at creation and kept there; sbz_cmb_compute then runs the stage below for one cosmology
+ * per call on the context's own stream (work buffers grow as needed and are reused).  In
+ * sbz_cmb_compute kernel_ms[0] is 0 (the table already exists); sbz_cmb_spectra is the one-shot
+ * create + compute + destroy.  Calls on one context must not overlap. */
+typedef struct sbz_cmb sbz_cmb;
+int sbz_cmb_create(const int* ls, int n_l, int n_x, double dx, sbz_cmb** out);
+int sbz_cmb_compute(sbz_cmb* h, const double* S_coarse, int n_c, int n_s, const double* taus, int n_t,
+                    const double* y_c, const double* w_c, const double* ks, const double* y_f, const double* wk,
+                    int n_f, const int* kind, const int* l_limber, const int* modes, int n_m, double* out_cl,
+                    double* out_theta, float* kernel_ms);
+int sbz_cmb_destroy(sbz_cmb* h);
+
 /* The C_l stage of spectrum_cmb (src/observables/angular.jl:260-341) after the source solve, on
  * the device from the coarse sources to C_l:
  *  - source_kinterp with a ChebyshevInterpolator (src/observables/fourier.jl:232-242,520-532):

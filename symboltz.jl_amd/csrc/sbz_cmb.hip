@@ -23,6 +23,7 @@
 
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "sbz_internal.h"
@@ -295,26 +296,6 @@ extern "C" __global__ void sbz_cl_reduce_kernel(const double* __restrict__ part,
 
 namespace {
 
-struct DevBuf {
-    std::vector<void*> ptrs;
-    ~DevBuf() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <class T>
-    hipError_t alloc(T** p, size_t n) {
-        hipError_t e = hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
-        if (e == hipSuccess) ptrs.push_back((void*)*p);
-        return e;
-    }
-};
-
-struct Events {
-    std::vector<hipEvent_t> ev;
-    ~Events() {
-        for (auto e : ev) (void)hipEventDestroy(e);
-    }
-};
-
 int check_ls(const int* ls, int n_l, const char* who) {
     for (int i = 0; i < n_l; ++i)
         if (ls[i] < 0 || (i > 0 && ls[i] <= ls[i - 1])) {
@@ -323,6 +304,30 @@ int check_ls(const int* ls, int n_l, const char* who) {
         }
     return SBZ_OK;
 }
+
+// grow-only device buffer
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t b) {
+        if (b <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
 
 }  // namespace
 
@@ -335,6 +340,26 @@ int check_ls(const int* ls, int n_l, const char* who) {
         }                                                                                         \
     } while (0)
 
+// A C_l context: the Bessel table of one l list stays on the device across calls, work buffers
+// only grow, and every launch goes to the context's own stream.
+struct sbz_cmb {
+    int n_l = 0, n_x = 0, lmax = 0;
+    double dx = 0.0;
+    float bessel_ms = 0.0f;      // the table kernel's time at creation
+    std::vector<int> ls;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    DBuf ls_d, y, dy;            // the table
+    DBuf S, Sf, Th, part, cl;    // per call
+    DBuf small;                  // nodes, weights, grids, small int arrays
+    ~sbz_cmb() {
+        for (DBuf* b : {&ls_d, &y, &dy, &S, &Sf, &Th, &part, &cl, &small}) b->release();
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 static int bessel_launch(const int* d_ls, int n_l, int lmax, int n_x, double dx, double* d_y, double* d_dy,
                          hipStream_t st) {
     hipLaunchKernelGGL(sbz_bessel_kernel, dim3((n_x + 255) / 256), dim3(256), 0, st, d_ls, n_l, lmax, n_x, dx, d_y,
@@ -343,23 +368,214 @@ static int bessel_launch(const int* d_ls, int n_l, int lmax, int n_x, double dx,
     return SBZ_OK;
 }
 
+extern "C" int sbz_cmb_create(const int* ls, int n_l, int n_x, double dx, sbz_cmb** out) {
+    sbz_clear_error();
+    if (!ls || !out || n_l < 1 || n_x < 3 || !(dx > 0) || !std::isfinite(dx)) {
+        sbz_set_error("sbz_cmb_create: invalid argument");
+        return SBZ_ERR_ARG;
+    }
+    if (int st = check_ls(ls, n_l, "sbz_cmb_create")) return st;
+    sbz_cmb* h = new sbz_cmb();
+    h->n_l = n_l;
+    h->n_x = n_x;
+    h->dx = dx;
+    h->lmax = ls[n_l - 1];
+    h->ls.assign(ls, ls + n_l);
+    auto fail = [&](hipError_t e, const char* what) {
+        sbz_set_error("sbz_cmb_create: %s: %s", what, hipGetErrorString(e));
+        delete h;
+        return SBZ_ERR_HIP;
+    };
+    hipError_t e = hipStreamCreate(&h->stream);
+    if (e != hipSuccess) return fail(e, "stream");
+    for (auto& v : h->ev)
+        if ((e = hipEventCreate(&v)) != hipSuccess) return fail(e, "event");
+    const size_t tb = sizeof(double) * (size_t)n_x * n_l;
+    if ((e = h->ls_d.reserve(sizeof(int) * n_l)) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = h->y.reserve(tb)) != hipSuccess || (e = h->dy.reserve(tb)) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = hipMemcpyAsync(h->ls_d.p, ls, sizeof(int) * n_l, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return fail(e, "upload");
+    if ((e = hipEventRecord(h->ev[0], h->stream)) != hipSuccess) return fail(e, "event");
+    hipLaunchKernelGGL(sbz_bessel_kernel, dim3((n_x + 255) / 256), dim3(256), 0, h->stream, h->ls_d.as<int>(), n_l,
+                       h->lmax, n_x, dx, h->y.as<double>(), h->dy.as<double>());
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e, "launch");
+    if ((e = hipEventRecord(h->ev[1], h->stream)) != hipSuccess) return fail(e, "event");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return fail(e, "sbz_bessel_kernel");
+    (void)hipEventElapsedTime(&h->bessel_ms, h->ev[0], h->ev[1]);
+    *out = h;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_cmb_destroy(sbz_cmb* h) {
+    delete h;
+    return SBZ_OK;
+}
+
 extern "C" int sbz_bessel_table(const int* ls, int n_l, int n_x, double dx, double* y, double* dy) {
     sbz_clear_error();
-    if (!ls || !y || !dy || n_l < 1 || n_x < 2 || !(dx > 0)) {
+    if (!y || !dy) {
         sbz_set_error("sbz_bessel_table: invalid argument");
         return SBZ_ERR_ARG;
     }
-    if (int st = check_ls(ls, n_l, "sbz_bessel_table")) return st;
-    DevBuf b;
-    int* d_ls;
-    double *d_y, *d_dy;
-    CCHK(b.alloc(&d_ls, n_l));
-    CCHK(b.alloc(&d_y, (size_t)n_x * n_l));
-    CCHK(b.alloc(&d_dy, (size_t)n_x * n_l));
-    CCHK(hipMemcpy(d_ls, ls, sizeof(int) * n_l, hipMemcpyHostToDevice));
-    if (int st = bessel_launch(d_ls, n_l, ls[n_l - 1], n_x, dx, d_y, d_dy, 0)) return st;
-    CCHK(hipMemcpy(y, d_y, sizeof(double) * n_x * n_l, hipMemcpyDeviceToHost));
-    CCHK(hipMemcpy(dy, d_dy, sizeof(double) * n_x * n_l, hipMemcpyDeviceToHost));
+    sbz_cmb* h = nullptr;
+    if (int st = sbz_cmb_create(ls, n_l, n_x, dx, &h)) return st;
+    int st = SBZ_OK;
+    hipError_t e = hipMemcpy(y, h->y.p, sizeof(double) * n_x * n_l, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(dy, h->dy.p, sizeof(double) * n_x * n_l, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        sbz_set_error("sbz_bessel_table: copy: %s", hipGetErrorString(e));
+        st = SBZ_ERR_HIP;
+    }
+    sbz_cmb_destroy(h);
+    return st;
+}
+
+extern "C" int sbz_cmb_compute(sbz_cmb* h, const double* S_coarse, int n_c, int n_s, const double* taus, int n_t,
+                               const double* y_c, const double* w_c, const double* ks, const double* y_f,
+                               const double* wk, int n_f, const int* kind, const int* l_limber, const int* modes,
+                               int n_m, double* out_cl, double* out_theta, float* kernel_ms) {
+    sbz_clear_error();
+    if (!h || !S_coarse || !taus || !y_c || !w_c || !ks || !y_f || !wk || !kind || !l_limber || !modes || !out_cl ||
+        n_c < 1 || n_c > SBZ_CMB_MAXC || n_s < 1 || n_s > SBZ_CMB_MAXS || n_t < 2 || n_f < 1 || n_m < 1) {
+        sbz_set_error("sbz_cmb_compute: invalid argument (at most %d nodes and %d source kinds)", SBZ_CMB_MAXC,
+                      SBZ_CMB_MAXS);
+        return SBZ_ERR_ARG;
+    }
+    const int n_l = h->n_l, n_x = h->n_x;
+    const double dx = h->dx;
+    if (n_f > 65535) {
+        sbz_set_error("sbz_cmb_compute: at most 65535 fine wavenumbers per call (got %d)", n_f);
+        return SBZ_ERR_ARG;
+    }
+    for (int i = 0; i < n_t; ++i)
+        if (!std::isfinite(taus[i]) || (i > 0 && !(taus[i] > taus[i - 1]))) {
+            sbz_set_error("sbz_cmb_compute: τs must be finite and sorted in ascending order");
+            return SBZ_ERR_ARG;
+        }
+    for (int i = 0; i < n_c; ++i)
+        if (!std::isfinite(y_c[i]) || !std::isfinite(w_c[i])) {
+            sbz_set_error("sbz_cmb_compute: non-finite interpolation node or weight %d", i);
+            return SBZ_ERR_ARG;
+        }
+    double kmax = 0.0;
+    for (int i = 0; i < n_f; ++i) {
+        if (!(ks[i] > 0.0) || !(ks[i] < 1e300) || !std::isfinite(y_f[i]) || !std::isfinite(wk[i])) {
+            sbz_set_error("sbz_cmb_compute: wavenumber %d (or its transform or weight) is not positive and finite", i);
+            return SBZ_ERR_ARG;
+        }
+        kmax = fmax(kmax, ks[i]);
+    }
+    const double chimax = taus[n_t - 1] - taus[0];
+    if ((n_x - 2) * dx < kmax * chimax * (1.0 - 1e-12)) {
+        sbz_set_error("sbz_cmb_compute: Bessel table ends at x = %g < k_max χ_max = %g", (n_x - 2) * dx,
+                      kmax * chimax);
+        return SBZ_ERR_ARG;
+    }
+    for (int s = 0; s < n_s; ++s)
+        if (kind[s] < 0 || kind[s] > 2) {
+            sbz_set_error("sbz_cmb_compute: source kind %d of source %d is not 0 (T), 1 (E) or 2 (lensing)", kind[s],
+                          s);
+            return SBZ_ERR_ARG;
+        }
+    for (int mo = 0; mo < 2 * n_m; ++mo)
+        if (modes[mo] < 0 || modes[mo] >= n_s) {
+            sbz_set_error("sbz_cmb_compute: mode entry %d names source %d of %d", mo, modes[mo], n_s);
+            return SBZ_ERR_ARG;
+        }
+    const long long nS = (long long)n_c * n_t * n_s;
+    for (long long e = 0; e < nS; ++e)
+        if (!std::isfinite(S_coarse[e])) {
+            sbz_set_error("sbz_cmb_compute: non-finite source value at flat index %lld", e);
+            return SBZ_ERR_ARG;
+        }
+    // first l index at or above each source's Limber multipole
+    int il_lim[SBZ_CMB_MAXS] = {n_l, n_l, n_l, n_l};
+    for (int s = 0; s < n_s; ++s) {
+        int i = 0;
+        while (i < n_l && h->ls[i] < l_limber[s]) ++i;
+        il_lim[s] = i;
+    }
+    const int n_chunk = (n_f + SBZ_CL_KCHUNK - 1) / SBZ_CL_KCHUNK;
+    const int m = n_t * n_s;
+    // small arrays packed into one upload: yc wc | yf ks wk | taus w chi | ints (lim kind modes)
+    std::vector<double> sm;
+    sm.reserve(2 * n_c + 3 * n_f + 3 * n_t + SBZ_CMB_MAXS * 2 + 2 * n_m + 8);
+    const size_t o_yc = sm.size();
+    sm.insert(sm.end(), y_c, y_c + n_c);
+    const size_t o_wc = sm.size();
+    sm.insert(sm.end(), w_c, w_c + n_c);
+    const size_t o_yf = sm.size();
+    sm.insert(sm.end(), y_f, y_f + n_f);
+    const size_t o_ks = sm.size();
+    sm.insert(sm.end(), ks, ks + n_f);
+    const size_t o_wk = sm.size();
+    sm.insert(sm.end(), wk, wk + n_f);
+    const size_t o_taus = sm.size();
+    sm.insert(sm.end(), taus, taus + n_t);
+    // trapezoid weights (angular.jl:124-129) and χ = τ0 - τ
+    const size_t o_w = sm.size();
+    sm.resize(sm.size() + n_t);
+    sm[o_w] = 0.5 * (taus[1] - taus[0]);
+    for (int i = 1; i < n_t - 1; ++i) sm[o_w + i] = 0.5 * (taus[i + 1] - taus[i - 1]);
+    sm[o_w + n_t - 1] = 0.5 * (taus[n_t - 1] - taus[n_t - 2]);
+    const size_t o_chi = sm.size();
+    for (int i = 0; i < n_t; ++i) sm.push_back(taus[n_t - 1] - taus[i]);
+    const size_t o_int = sm.size();   // ints stored after the doubles
+    std::vector<int> iv;
+    for (int s = 0; s < n_s; ++s) iv.push_back(il_lim[s]);
+    for (int s = 0; s < n_s; ++s) iv.push_back(kind[s]);
+    for (int q = 0; q < 2 * n_m; ++q) iv.push_back(modes[q]);
+    sm.resize(o_int + (iv.size() + 1) / 2);
+    std::memcpy(sm.data() + o_int, iv.data(), sizeof(int) * iv.size());
+
+    CCHK(h->small.reserve(sizeof(double) * sm.size()));
+    CCHK(h->S.reserve(sizeof(double) * nS));
+    CCHK(h->Sf.reserve(sizeof(double) * (size_t)n_f * m));
+    CCHK(h->Th.reserve(sizeof(double) * (size_t)n_s * n_f * n_l));
+    CCHK(h->part.reserve(sizeof(double) * (size_t)n_chunk * n_m * n_l));
+    CCHK(h->cl.reserve(sizeof(double) * (size_t)n_m * n_l));
+    hipStream_t st = h->stream;
+    double* dsm = h->small.as<double>();
+    const int* d_lim = (const int*)(dsm + o_int);
+    const int* d_kind = d_lim + n_s;
+    const int* d_modes = d_kind + n_s;
+    CCHK(hipMemcpyAsync(dsm, sm.data(), sizeof(double) * sm.size(), hipMemcpyHostToDevice, st));
+    CCHK(hipMemcpyAsync(h->S.p, S_coarse, sizeof(double) * nS, hipMemcpyHostToDevice, st));
+    CCHK(hipEventRecord(h->ev[0], st));
+    CCHK(hipEventRecord(h->ev[1], st));   // the table is built at creation
+    hipLaunchKernelGGL(sbz_kinterp_kernel, dim3((n_f + SBZ_KI_KB - 1) / SBZ_KI_KB), dim3(256), 0, st,
+                       h->S.as<double>(), n_c, m, dsm + o_yc, dsm + o_wc, dsm + o_yf, n_f, h->Sf.as<double>());
+    CCHK(hipGetLastError());
+    CCHK(hipEventRecord(h->ev[2], st));
+    {
+        const int bl = n_l >= 256 ? 256 : ((n_l + 63) / 64) * 64;
+        hipLaunchKernelGGL(sbz_cmb_los_kernel, dim3((n_l + bl - 1) / bl, n_f), dim3(bl), 0, st, h->Sf.as<double>(),
+                           n_s, dsm + o_w, dsm + o_chi, n_t, dsm + o_ks, n_f, h->y.as<double>(), h->dy.as<double>(),
+                           n_x, dx, n_l, il_lim[0], il_lim[1], il_lim[2], il_lim[3], h->Th.as<double>());
+        CCHK(hipGetLastError());
+    }
+    CCHK(hipEventRecord(h->ev[3], st));
+    {
+        const long long n = (long long)n_f * n_l;
+        hipLaunchKernelGGL(sbz_cmb_post_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           h->Sf.as<double>(), n_s, dsm + o_taus, dsm + o_chi, n_t, dsm + o_ks, n_f, h->ls_d.as<int>(),
+                           n_l, d_lim, d_kind, h->Th.as<double>());
+        CCHK(hipGetLastError());
+    }
+    CCHK(hipEventRecord(h->ev[4], st));
+    hipLaunchKernelGGL(sbz_cl_kernel, dim3((n_l + 63) / 64, n_chunk), dim3(256), 0, st, h->Th.as<double>(), n_f, n_l,
+                       dsm + o_wk, d_modes, n_m, h->part.as<double>());
+    CCHK(hipGetLastError());
+    hipLaunchKernelGGL(sbz_cl_reduce_kernel, dim3((n_m * n_l + 255) / 256), dim3(256), 0, st, h->part.as<double>(),
+                       n_chunk, n_m * n_l, h->cl.as<double>());
+    CCHK(hipGetLastError());
+    CCHK(hipEventRecord(h->ev[5], st));
+    CCHK(hipMemcpyAsync(out_cl, h->cl.p, sizeof(double) * n_m * n_l, hipMemcpyDeviceToHost, st));
+    if (out_theta)
+        CCHK(hipMemcpyAsync(out_theta, h->Th.p, sizeof(double) * n_s * n_f * n_l, hipMemcpyDeviceToHost, st));
+    CCHK(hipStreamSynchronize(st));
+    if (kernel_ms)
+        for (int i = 0; i < 5; ++i) CCHK(hipEventElapsedTime(&kernel_ms[i], h->ev[i], h->ev[i + 1]));
     return SBZ_OK;
 }
 
@@ -367,151 +583,12 @@ extern "C" int sbz_cmb_spectra(const double* S_coarse, int n_c, int n_s, const d
                                const double* w_c, const double* ks, const double* y_f, const double* wk, int n_f,
                                const int* ls, int n_l, int n_x, double dx, const int* kind, const int* l_limber,
                                const int* modes, int n_m, double* out_cl, double* out_theta, float* kernel_ms) {
-    sbz_clear_error();
-    if (!S_coarse || !taus || !y_c || !w_c || !ks || !y_f || !wk || !ls || !kind || !l_limber || !modes || !out_cl ||
-        n_c < 1 || n_c > SBZ_CMB_MAXC || n_s < 1 || n_s > SBZ_CMB_MAXS || n_t < 2 || n_f < 1 || n_l < 1 || n_x < 2 ||
-        !(dx > 0) || n_m < 1) {
-        sbz_set_error("sbz_cmb_spectra: invalid argument (at most %d nodes and %d source kinds)", SBZ_CMB_MAXC,
-                      SBZ_CMB_MAXS);
-        return SBZ_ERR_ARG;
-    }
-    if (int st = check_ls(ls, n_l, "sbz_cmb_spectra")) return st;
-    if (n_f > 65535) {
-        sbz_set_error("sbz_cmb_spectra: at most 65535 fine wavenumbers per call (got %d)", n_f);
-        return SBZ_ERR_ARG;
-    }
-    for (int i = 0; i < n_t; ++i)
-        if (!std::isfinite(taus[i]) || (i > 0 && !(taus[i] > taus[i - 1]))) {
-            sbz_set_error("sbz_cmb_spectra: τs must be finite and sorted in ascending order");
-            return SBZ_ERR_ARG;
-        }
-    for (int i = 0; i < n_c; ++i)
-        if (!std::isfinite(y_c[i]) || !std::isfinite(w_c[i])) {
-            sbz_set_error("sbz_cmb_spectra: non-finite interpolation node or weight %d", i);
-            return SBZ_ERR_ARG;
-        }
-    for (int i = 0; i < n_f; ++i)
-        if (!std::isfinite(y_f[i]) || !std::isfinite(wk[i])) {
-            sbz_set_error("sbz_cmb_spectra: non-finite transformed wavenumber or weight %d", i);
-            return SBZ_ERR_ARG;
-        }
-    double kmax = 0.0;
-    for (int i = 0; i < n_f; ++i) {
-        if (!(ks[i] > 0.0) || !(ks[i] < 1e300)) {
-            sbz_set_error("sbz_cmb_spectra: wavenumber %d is not positive and finite", i);
-            return SBZ_ERR_ARG;
-        }
-        kmax = fmax(kmax, ks[i]);
-    }
-    const double chimax = taus[n_t - 1] - taus[0];
-    if ((n_x - 2) * dx < kmax * chimax * (1.0 - 1e-12)) {
-        sbz_set_error("sbz_cmb_spectra: Bessel table ends at x = %g < k_max χ_max = %g", (n_x - 2) * dx,
-                      kmax * chimax);
-        return SBZ_ERR_ARG;
-    }
-    for (int s = 0; s < n_s; ++s)
-        if (kind[s] < 0 || kind[s] > 2) {
-            sbz_set_error("sbz_cmb_spectra: source kind %d of source %d is not 0 (T), 1 (E) or 2 (lensing)", kind[s],
-                          s);
-            return SBZ_ERR_ARG;
-        }
-    for (int mo = 0; mo < 2 * n_m; ++mo)
-        if (modes[mo] < 0 || modes[mo] >= n_s) {
-            sbz_set_error("sbz_cmb_spectra: mode entry %d names source %d of %d", mo, modes[mo], n_s);
-            return SBZ_ERR_ARG;
-        }
-    const long long nS = (long long)n_c * n_t * n_s;
-    for (long long e = 0; e < nS; ++e)
-        if (!std::isfinite(S_coarse[e])) {
-            sbz_set_error("sbz_cmb_spectra: non-finite source value at flat index %lld", e);
-            return SBZ_ERR_ARG;
-        }
-    // first l index at or above each source's Limber multipole
-    int il_lim[SBZ_CMB_MAXS] = {n_l, n_l, n_l, n_l};
-    for (int s = 0; s < n_s; ++s) {
-        int i = 0;
-        while (i < n_l && ls[i] < l_limber[s]) ++i;
-        il_lim[s] = i;
-    }
-    std::vector<double> w(n_t), chi(n_t);
-    w[0] = 0.5 * (taus[1] - taus[0]);
-    for (int i = 1; i < n_t - 1; ++i) w[i] = 0.5 * (taus[i + 1] - taus[i - 1]);
-    w[n_t - 1] = 0.5 * (taus[n_t - 1] - taus[n_t - 2]);
-    for (int i = 0; i < n_t; ++i) chi[i] = taus[n_t - 1] - taus[i];
-    const int n_chunk = (n_f + SBZ_CL_KCHUNK - 1) / SBZ_CL_KCHUNK;
-
-    DevBuf b;
-    double *d_S, *d_yc, *d_wc, *d_yf, *d_ks, *d_wk, *d_taus, *d_w, *d_chi, *d_y, *d_dy, *d_Sf, *d_Th, *d_part, *d_cl;
-    int *d_ls, *d_lim, *d_kind, *d_modes;
-    const int m = n_t * n_s;
-    CCHK(b.alloc(&d_S, (size_t)nS));
-    CCHK(b.alloc(&d_yc, n_c));
-    CCHK(b.alloc(&d_wc, n_c));
-    CCHK(b.alloc(&d_yf, n_f));
-    CCHK(b.alloc(&d_ks, n_f));
-    CCHK(b.alloc(&d_wk, n_f));
-    CCHK(b.alloc(&d_taus, n_t));
-    CCHK(b.alloc(&d_w, n_t));
-    CCHK(b.alloc(&d_chi, n_t));
-    CCHK(b.alloc(&d_y, (size_t)n_x * n_l));
-    CCHK(b.alloc(&d_dy, (size_t)n_x * n_l));
-    CCHK(b.alloc(&d_Sf, (size_t)n_f * m));
-    CCHK(b.alloc(&d_Th, (size_t)n_s * n_f * n_l));
-    CCHK(b.alloc(&d_part, (size_t)n_chunk * n_m * n_l));
-    CCHK(b.alloc(&d_cl, (size_t)n_m * n_l));
-    CCHK(b.alloc(&d_ls, n_l));
-    CCHK(b.alloc(&d_lim, n_s));
-    CCHK(b.alloc(&d_kind, n_s));
-    CCHK(b.alloc(&d_modes, 2 * n_m));
-    CCHK(hipMemcpy(d_S, S_coarse, sizeof(double) * nS, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_yc, y_c, sizeof(double) * n_c, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_wc, w_c, sizeof(double) * n_c, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_yf, y_f, sizeof(double) * n_f, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_ks, ks, sizeof(double) * n_f, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_wk, wk, sizeof(double) * n_f, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_taus, taus, sizeof(double) * n_t, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_w, w.data(), sizeof(double) * n_t, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_chi, chi.data(), sizeof(double) * n_t, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_ls, ls, sizeof(int) * n_l, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_lim, il_lim, sizeof(int) * n_s, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_kind, kind, sizeof(int) * n_s, hipMemcpyHostToDevice));
-    CCHK(hipMemcpy(d_modes, modes, sizeof(int) * 2 * n_m, hipMemcpyHostToDevice));
-
-    Events E;
-    E.ev.resize(6, nullptr);
-    for (auto& e : E.ev) CCHK(hipEventCreate(&e));
-    CCHK(hipEventRecord(E.ev[0], 0));
-    if (int st = bessel_launch(d_ls, n_l, ls[n_l - 1], n_x, dx, d_y, d_dy, 0)) return st;
-    CCHK(hipEventRecord(E.ev[1], 0));
-    hipLaunchKernelGGL(sbz_kinterp_kernel, dim3((n_f + SBZ_KI_KB - 1) / SBZ_KI_KB), dim3(256), 0, 0, d_S, n_c, m, d_yc,
-                       d_wc, d_yf, n_f, d_Sf);
-    CCHK(hipGetLastError());
-    CCHK(hipEventRecord(E.ev[2], 0));
-    {
-        const int bl = n_l >= 256 ? 256 : ((n_l + 63) / 64) * 64;
-        hipLaunchKernelGGL(sbz_cmb_los_kernel, dim3((n_l + bl - 1) / bl, n_f), dim3(bl), 0, 0, d_Sf, n_s, d_w, d_chi,
-                           n_t, d_ks, n_f, d_y, d_dy, n_x, dx, n_l, il_lim[0], il_lim[1], il_lim[2], il_lim[3], d_Th);
-        CCHK(hipGetLastError());
-    }
-    CCHK(hipEventRecord(E.ev[3], 0));
-    {
-        const long long n = (long long)n_f * n_l;
-        hipLaunchKernelGGL(sbz_cmb_post_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, d_Sf, n_s, d_taus,
-                           d_chi, n_t, d_ks, n_f, d_ls, n_l, d_lim, d_kind, d_Th);
-        CCHK(hipGetLastError());
-    }
-    CCHK(hipEventRecord(E.ev[4], 0));
-    hipLaunchKernelGGL(sbz_cl_kernel, dim3((n_l + 63) / 64, n_chunk), dim3(256), 0, 0, d_Th, n_f, n_l, d_wk, d_modes,
-                       n_m, d_part);
-    CCHK(hipGetLastError());
-    hipLaunchKernelGGL(sbz_cl_reduce_kernel, dim3((n_m * n_l + 255) / 256), dim3(256), 0, 0, d_part, n_chunk, n_m * n_l,
-                       d_cl);
-    CCHK(hipGetLastError());
-    CCHK(hipEventRecord(E.ev[5], 0));
-    CCHK(hipEventSynchronize(E.ev[5]));
-    if (kernel_ms)
-        for (int i = 0; i < 5; ++i) CCHK(hipEventElapsedTime(&kernel_ms[i], E.ev[i], E.ev[i + 1]));
-    CCHK(hipMemcpy(out_cl, d_cl, sizeof(double) * n_m * n_l, hipMemcpyDeviceToHost));
-    if (out_theta) CCHK(hipMemcpy(out_theta, d_Th, sizeof(double) * n_s * n_f * n_l, hipMemcpyDeviceToHost));
-    return SBZ_OK;
+    sbz_cmb* h = nullptr;
+    int st = sbz_cmb_create(ls, n_l, n_x, dx, &h);
+    if (st == SBZ_OK)
+        st = sbz_cmb_compute(h, S_coarse, n_c, n_s, taus, n_t, y_c, w_c, ks, y_f, wk, n_f, kind, l_limber, modes, n_m,
+                             out_cl, out_theta, kernel_ms);
+    if (st == SBZ_OK && kernel_ms) kernel_ms[0] = h->bessel_ms;
+    if (h) sbz_cmb_destroy(h);
+    return st;
 }

@@ -30,6 +30,36 @@ class SphericalBesselCache:
         self.n_x = n + 1                         # plus the duplicate padding point
         self.xmax = xmax
         self._y = self._dy = None
+        self._ctx = None
+        self.context_ms = 0.0
+
+    def context(self):
+        """Device C_l context (`sbz_cmb_create`): the table built once on the GPU and kept there."""
+        if self._ctx is None:
+            import time
+            L = _lib.lib()
+            L.sbz_cmb_create.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(ctypes.c_void_p)]
+            L.sbz_cmb_destroy.argtypes = [ctypes.c_void_p]
+            ls = np.ascontiguousarray(self.l, dtype=np.int32)
+            h = ctypes.c_void_p()
+            t = time.perf_counter()
+            _lib.check(L.sbz_cmb_create(_lib.ptr(ls, ctypes.c_int), len(ls), self.n_x, self.dx, ctypes.byref(h)),
+                       "sbz_cmb_create")
+            self.context_ms = (time.perf_counter() - t) * 1e3
+            self._ctx = h
+        return self._ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None) is not None:
+            _lib.lib().sbz_cmb_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @property
     def x(self):

@@ -137,9 +137,10 @@ def spline_integral_weights(x):
 def _cmb_spectra_gpu(S_coarse, taus, kinterp, ks_fine, wk, jl, kinds, l_limber, modes, want_theta=False):
     L = _lib.lib()
     dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
-    L.sbz_cmb_spectra.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_int,
-                                  ip, ctypes.c_int, ctypes.c_int, ctypes.c_double, ip, ip, ip, ctypes.c_int, dp, dp,
-                                  ctypes.POINTER(ctypes.c_float)]
+    L.sbz_cmb_compute.argtypes = [ctypes.c_void_p, dp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, dp, dp, dp,
+                                  dp, ctypes.c_int, ip, ip, ip, ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_float)]
+    new_ctx = jl._ctx is None
+    h = jl.context()
     S = np.ascontiguousarray(S_coarse, dtype=np.float64)
     n_c, n_t, n_s = S.shape
     taus = np.ascontiguousarray(taus, dtype=np.float64)
@@ -148,20 +149,20 @@ def _cmb_spectra_gpu(S_coarse, taus, kinterp, ks_fine, wk, jl, kinds, l_limber, 
     yc = np.ascontiguousarray(kinterp.ys, dtype=np.float64)
     wc = np.ascontiguousarray(kinterp.ws, dtype=np.float64)
     wk = np.ascontiguousarray(wk, dtype=np.float64)
-    ls = np.ascontiguousarray(jl.l, dtype=np.int32)
     kinds = np.ascontiguousarray(kinds, dtype=np.int32)
     lim = np.ascontiguousarray(l_limber, dtype=np.int32)
     md = np.ascontiguousarray(modes, dtype=np.int32).ravel()
     n_m = len(md) // 2
-    cl = np.zeros((n_m, len(ls)))
-    th = np.zeros((n_s, len(ks), len(ls))) if want_theta else None
+    cl = np.zeros((n_m, len(jl.l)))
+    th = np.zeros((n_s, len(ks), len(jl.l))) if want_theta else None
     ms = (ctypes.c_float * 5)()
-    _lib.check(L.sbz_cmb_spectra(_lib.ptr(S), n_c, n_s, _lib.ptr(taus), n_t, _lib.ptr(yc), _lib.ptr(wc), _lib.ptr(ks),
-                                 _lib.ptr(yf), _lib.ptr(wk), len(ks), _lib.ptr(ls, ctypes.c_int), len(ls), jl.n_x,
-                                 jl.dx, _lib.ptr(kinds, ctypes.c_int), _lib.ptr(lim, ctypes.c_int),
-                                 _lib.ptr(md, ctypes.c_int), n_m, _lib.ptr(cl),
-                                 _lib.ptr(th) if th is not None else None, ms), "sbz_cmb_spectra")
-    return cl, th, dict(zip(["bessel", "kinterp", "los", "limber", "kintegral"], list(ms)))
+    _lib.check(L.sbz_cmb_compute(h, _lib.ptr(S), n_c, n_s, _lib.ptr(taus), n_t, _lib.ptr(yc), _lib.ptr(wc),
+                                 _lib.ptr(ks), _lib.ptr(yf), _lib.ptr(wk), len(ks), _lib.ptr(kinds, ctypes.c_int),
+                                 _lib.ptr(lim, ctypes.c_int), _lib.ptr(md, ctypes.c_int), n_m, _lib.ptr(cl),
+                                 _lib.ptr(th) if th is not None else None, ms), "sbz_cmb_compute")
+    t = dict(zip(["bessel", "kinterp", "los", "limber", "kintegral"], list(ms)))
+    t["bessel"] = jl.context_ms if new_ctx else 0.0     # the table is built once per cache
+    return cl, th, t
 
 
 def source_grid(prob, taus, ks, bg=None, ptopts=None, timing=None):
