@@ -98,6 +98,55 @@ def workload(name):
     return M, [pars], ks, desc
 
 
+def bench_bg(args):
+    """Background workload: solvebg (src/solve.jl:427-435) for a Latin-hypercube batch of cosmologies
+    of the surrogate model (ΛCDM + 3 degenerate massive ν, lmax 6; 7 parameters incl. z_re and Σm_ν),
+    all sets in one GPU launch, one thread per set.  A step is one batched solve."""
+    from symboltz import _lib
+    from symboltz.surrogate import lhs, pars_of, surrogate_model
+    from symboltz.models import make_params
+    M = surrogate_model()
+    X = lhs(args.bg_sets, seed=1)
+    Ps = np.stack([make_params(M, pars_of(M, x)) for x in X])
+    for w in range(args.warmup):
+        _lib.background_solve_batch(Ps)
+    t_start = time.perf_counter()
+    kms = []
+    for s in range(args.steps):
+        P, knots, nrej, status, ms = _lib.background_solve_batch(Ps)
+        kms.append(ms)
+        log(f"step {s}: kernel {ms:.1f} ms, {int(np.sum(status != 0))} failed")
+    elapsed = (time.perf_counter() - t_start) / args.steps
+    ok = bool(np.all(status == 0))
+    cpu = None
+    if not args.no_cpu:
+        from concurrent.futures import ThreadPoolExecutor
+        sel = np.arange(0, len(Ps), max(1, len(Ps) // 512))
+        cores = min(16, os.cpu_count() or 1)
+        t_c = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=cores) as ex:
+            host = list(ex.map(lambda i: _lib.background_solve(Ps[i]), sel))
+        tc = time.perf_counter() - t_c
+        dt0 = max(abs(P[i, 23] / h[0][23] - 1) for i, h in zip(sel, host))
+        cpu = {"value": len(sel) / tc, "unit": "background solves/s", "cores": cores, "kind": "port",
+               "sample": f"{len(sel)} of the sets with the host build of the same solver on {cores} threads "
+                         f"({tc:.2f} s); max |tau0_gpu/tau0_cpu - 1| = {dt0:.1e}"}
+        log(f"cpu baseline {len(sel) / tc:.1f} solves/s; tau0 parity {dt0:.1e}")
+    kern = float(np.mean(kms))
+    out = {
+        "metric": "background solves/s (solvebg: Rodas5P on a, X_H+, X_He+, dT, kappa with the a = 1 event)",
+        "value": len(Ps) / elapsed, "unit": "background solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "bg", "description": f"{len(Ps)} Latin-hypercube sets (seed 1) of LCDM + 3 massive nu, "
+                   "rtol = atol = 1e-7, one thread per set", "n_sets": len(Ps), "all_success": ok,
+                   "kernel_ms": kern, "knots_mean": float(np.mean([len(k[0]) for k in knots]))},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_cmb(args):
     """C_l workload: spectrum_cmb([TT, EE, ψψ]) of the reference's CLASS-comparison model (ΛCDM
     lmax 5; test/runtests.jl:14-16,879-886) for a batch of cosmologies: set 0 is Planck18 (checked
@@ -199,6 +248,7 @@ def main():
     ap.add_argument("--no-sources", action="store_true", help="P(k) only: skip the CMB sources")
     ap.add_argument("--n-tau", type=int, default=2000, help="points of the source time grid")
     ap.add_argument("--cmb-sets", type=int, default=16, help="cosmologies per step of the cmb workload")
+    ap.add_argument("--bg-sets", type=int, default=16384, help="cosmologies per step of the bg workload")
     ap.add_argument("--pmc-copy", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -226,13 +276,18 @@ def main():
         raise SystemExit("bench.py: no GPU visible")
     if args.workload == "cmb":
         return bench_cmb(args)
+    if args.workload == "bg":
+        return bench_bg(args)
     from symboltz.constants import k_dimensionless
     M, sets, ks_mpc, desc = workload(args.workload)
     gm = GpuModel(M)
     t_bg = time.perf_counter()
     bgs, inst_set, ks_l, t0_l = [], [], [], []
+    if len(sets) > 1:   # all backgrounds at once on the GPU, one thread per set
+        from symboltz.problem import solvebg_sets
+        bg_all = solvebg_sets([CosmologyProblem(M, pars) for pars in sets])
     for si, pars in enumerate(sets):
-        bg = solvebg(CosmologyProblem(M, pars))
+        bg = bg_all[si] if len(sets) > 1 else solvebg(CosmologyProblem(M, pars))
         gm.upload_background(si, bg.t, bg.u, bg.du, bg.P)
         k_s = k_dimensionless(ks_mpc, pars["h"])
         bgs.append(bg)

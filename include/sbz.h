@@ -50,6 +50,16 @@ int sbz_device_count(int* count);
 int sbz_background_solve(double* params, double rtol, double atol, int max_knots, double* t,
                          double* u, double* du, int* n_knots, int* n_rejected);
 
+/* The same solve for n parameter sets at once on the GPU, one thread per set (solvebg of
+ * src/solve.jl:427-435 for each cosmology of a batch; the reference runs them one after the other on
+ * the CPU).  params [n][SBZ_NPAR] in/out; t [n][max_knots], u and du [n][max_knots][5];
+ * n_knots, n_rejected (may be NULL) and status [n] per set, status 0 on success, else 1 maximum
+ * iterations, 2 step size below minimum, 3 more steps than max_knots, 4 a = 1 not reached.
+ * Entries past n_knots[i] in a set's rows are left untouched.  Per-set failures do not fail the
+ * call (the first is described by sbz_last_error). */
+int sbz_background_solve_batch(double* params, int n, double rtol, double atol, int max_knots, double* t,
+                               double* u, double* du, int* n_knots, int* n_rejected, int* status, float* kernel_ms);
+
 /* Background right-hand side at one state (host; used by tests of the background stage). */
 int sbz_background_rhs(const double* params, const double* u, double* du);
 

@@ -85,3 +85,29 @@ def background_solve(P, rtol=1e-7, atol=1e-7, max_knots=20000):
     check(st, "sbz_background_solve")
     n = nk.value
     return P, t[:n].copy(), u[:n].copy(), du[:n].copy(), nrej.value
+
+
+BG_STATUS = {0: "Success", 1: "MaxIters", 2: "DtLessThanMin", 3: "MaxKnots", 4: "NoEnd"}
+
+
+def background_solve_batch(Ps, rtol=1e-7, atol=1e-7, max_knots=2048):
+    """The background solve of every row of Ps [n][SBZ_NPAR] on the GPU, one thread per set.
+    Returns (P_out [n][NPAR], list of (t, u, du) per set, n_rejected [n], status [n], kernel_ms)."""
+    L = lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.sbz_background_solve_batch.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             _dp, _dp, _dp, ip, ip, ip, ctypes.POINTER(ctypes.c_float)]
+    P = np.ascontiguousarray(Ps, dtype=np.float64).copy()
+    n = P.shape[0]
+    t = np.empty((n, max_knots))          # only each row's used prefix is written
+    u = np.empty((n, max_knots, 5))
+    du = np.empty((n, max_knots, 5))
+    nk = np.zeros(n, dtype=np.int32)
+    nr = np.zeros(n, dtype=np.int32)
+    stt = np.zeros(n, dtype=np.int32)
+    ms = ctypes.c_float(0.0)
+    check(L.sbz_background_solve_batch(ptr(P), n, rtol, atol, max_knots, ptr(t), ptr(u), ptr(du),
+                                       ptr(nk, ctypes.c_int), ptr(nr, ctypes.c_int), ptr(stt, ctypes.c_int),
+                                       ctypes.byref(ms)), "sbz_background_solve_batch")
+    knots = [(t[i, :nk[i]].copy(), u[i, :nk[i]].copy(), du[i, :nk[i]].copy()) for i in range(n)]
+    return P, knots, nr, stt, ms.value

@@ -23,6 +23,7 @@ def build_library(verbose=False):
     LIBDIR.mkdir(exist_ok=True)
     out = LIBDIR / "libsbz_hip.so"
     srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_los.hip", "sbz_cmb.hip",
+                                    "sbz_bgsolve.hip",
                                     "sbz_mlp.hip")]
     cmd = ["hipcc", "-fPIC", "-shared", "-I", str(INCLUDE)] + HIP_FLAGS + srcs + ["-lhiprtc", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)

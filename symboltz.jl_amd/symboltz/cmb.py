@@ -25,7 +25,7 @@ from scipy.interpolate import CubicSpline
 
 from . import _lib
 from .bessel import SphericalBesselCache
-from .problem import solvebg, spectrum_primordial
+from .problem import solvebg, solvebg_sets, spectrum_primordial
 from .solver import ALGO_AUTO, SAVE_SOURCES
 
 INT_MAX = 2**31 - 1
@@ -217,25 +217,30 @@ def source_grid_sets(probs, taus_sets, ks, bgs, ptopts=None, timing=None):
 
 
 def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_tau0=math.pi, xs=None, taucut=1e-2,
-                 l_limber=10, bgopts=None, ptopts=None, timing=None, details=None):
+                 l_limber=10, bgopts=None, ptopts=None, timing=None, details=None, bg=None):
     """Angular power spectra C_l^{AB} for `modes` such as ["TT", "EE", "TE", "ψψ"] at jl.l (or, with
     `ls`, splined to every l in `ls`, whose extrema must equal those of jl.l), as
     `spectrum_cmb(modes, prob, jl[, ls]; normalization)` (angular.jl:260-359).  Spectra are of
     dimensionless temperature fluctuations; normalization "Dl" returns l(l+1) C_l / 2π.
-    Returns [n_l][n_modes]."""
+    Returns [n_l][n_modes].  `bg`: a background solution to use instead of solving one."""
     dets = [] if details is not None else None
     out = spectrum_cmb_sets(modes, [prob], jl, ls, normalization, kinterp, dk_tau0, xs, taucut, l_limber, bgopts,
-                            ptopts, timing, dets)[0]
+                            ptopts, timing, dets, bgs=None if bg is None else [bg])[0]
     if details is not None:
         details.update(dets[0])
     return out
 
 
 def spectrum_cmb_sets(modes, probs, jl, ls=None, normalization="Cl", kinterp=None, dk_tau0=math.pi, xs=None,
-                      taucut=1e-2, l_limber=10, bgopts=None, ptopts=None, timing=None, details=None):
+                      taucut=1e-2, l_limber=10, bgopts=None, ptopts=None, timing=None, details=None, bgs=None):
     """`spectrum_cmb` for several cosmologies of one model: the source modes of every set run in one
     persistent launch (each set on its own τ grid), then the device C_l stage runs per set.
-    Returns [n_sets][n_l][n_modes]; `details`, when a list, receives one dict per set."""
+    Returns [n_sets][n_l][n_modes]; `details`, when a list, receives one dict per set.
+
+    The backgrounds come from `bgs` when given, else from one batched GPU solve (several sets) or
+    the host solve (one set).  The τ grid starts at the first background knot at or after τcut, as
+    in the reference, so two background solvers whose step sequences differ in rounding give
+    spectra that differ at the level of that grid's trapezoid error (~1e-4)."""
     if not isinstance(jl, SphericalBesselCache):
         raise TypeError("jl must be a SphericalBesselCache")
     modes = [str(m) for m in modes]
@@ -260,10 +265,11 @@ def spectrum_cmb_sets(modes, probs, jl, ls=None, normalization="Cl", kinterp=Non
     xs = cosgrid(0.0, 1.0, length=300) if xs is None else np.asarray(xs, dtype=np.float64)
     if xs[0] != 0 or xs[-1] != 1:
         raise ValueError(f"xs begins with {xs[0]} and ends with {xs[-1]}, but should span [0, 1]")
-    if len(probs) > 1:
-        from concurrent.futures import ThreadPoolExecutor   # the host background solves release the GIL
-        with ThreadPoolExecutor(max_workers=min(16, len(probs))) as ex:
-            bgs = list(ex.map(lambda p: solvebg(p, bgopts["reltol"], bgopts["abstol"]), probs))
+    if bgs is not None:
+        if len(bgs) != len(probs):
+            raise ValueError("one background solution per problem")
+    elif len(probs) > 1:   # every set's background at once on the GPU
+        bgs = solvebg_sets(probs, bgopts["reltol"], bgopts["abstol"])
     else:
         bgs = [solvebg(probs[0], bgopts["reltol"], bgopts["abstol"])]
     taus_sets = []

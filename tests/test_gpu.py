@@ -182,3 +182,50 @@ def test_w0wa_model_matches_oracle():
     Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
     assert np.max(np.abs(g["pk"] / Po - 1)) < 1e-4
     assert rel_state_err(g["u"], o["u"]) < 1e-4
+
+
+def test_gpu_background_batch_matches_host_solver():
+    # the batched GPU background solve (one thread per set) against the host solver, set by set
+    from scipy.stats import qmc
+    from symboltz import _lib
+    say("case: GPU background solve, 24 parameter sets (one failing)")
+    M = models.LCDM(lmax=5)
+    base = models.parameters_Planck18(M)
+    X = qmc.scale(qmc.LatinHypercube(d=5, seed=3).random(24), [0.60, 0.020, 0.10, 2.9, 0.92],
+                  [0.75, 0.024, 0.14, 3.2, 1.00])
+    sets = [dict(base, h=h, Omega_b0=wb / h**2, Omega_c0=wc / h**2, ln_As1e10=la, ns=ns) for h, wb, wc, la, ns in X]
+    sets[7]["Omega_c0"] = float("nan")      # fails (the reference warns "Background solution failed")
+    Ps = np.stack([models.make_params(M, p) for p in sets])
+    P, knots, nrej, status, ms = _lib.background_solve_batch(Ps)
+    say(f"batch kernel {ms:.1f} ms")
+    assert status[7] != 0 and np.all(np.delete(status, 7) == 0)
+    for i in range(24):
+        if i == 7:
+            continue
+        Ph, t, u, du, _ = _lib.background_solve(Ps[i])
+        tg, ug, dug = knots[i]
+        for slot in (models.P_TAU0, models.P_KAPPA0):
+            assert abs(P[i, slot] / Ph[slot] - 1) < 1e-8, (i, slot)
+        # τrec is the visibility maximum over the knots (solve.jl:181-195): equal up to the knot
+        # spacing there, which differs with the rounding of the step sequence
+        j = np.searchsorted(t, Ph[models.P_TAUREC])
+        assert abs(P[i, models.P_TAUREC] - Ph[models.P_TAUREC]) <= 2 * (t[j + 1] - t[j - 1]), i
+        # the step sequences differ by rounding (device and host math differ in the last bits), so
+        # the knots differ; the two Hermite splines (the background the perturbations read) agree
+        # at sampled times to the integration tolerance
+        ts = np.geomspace(t[0] * 10, t[-1] * 0.999, 60)
+        errs = []
+        for c in (0, 4):   # a and _κ carry exact knot derivatives
+            a = hermite(tg, ug[:, c], dug[:, c], ts)
+            b = hermite(t, u[:, c], du[:, c], ts)
+            errs.append(np.max(np.abs(a - b) / (np.abs(b) + 1e-3 * np.abs(b).max())))
+        say(f"set {i}: knots {len(tg)} (GPU) / {len(t)} (host); spline differences a {errs[0]:.1e}, κ {errs[1]:.1e}")
+        assert max(errs) <= 1e-5, i
+
+
+def hermite(t, y, dy, x):
+    i = np.clip(np.searchsorted(t, x) - 1, 0, len(t) - 2)
+    h = t[i + 1] - t[i]
+    s = (x - t[i]) / h
+    return ((1 + 2 * s) * (1 - s) ** 2 * y[i] + s * (1 - s) ** 2 * h * dy[i] + s * s * (3 - 2 * s) * y[i + 1]
+            + s * s * (s - 1) * h * dy[i + 1])
