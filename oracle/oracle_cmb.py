@@ -175,3 +175,35 @@ def spectrum_cmb(O, P0_fn, modes, ls, ls_fine=None, normalization="Dl", lensing=
         lf = np.asarray(ls_fine, dtype=np.float64)
         out = np.stack([CubicSpline(L, out[:, i], bc_type="natural")(lf) for i in range(len(modes))], axis=1)
     return out, details
+
+
+def refine_queue(solve1, ks, f, finv, rtol, atol, its, nmaxks=1024):
+    """source_grid_adaptive's refinement as the reference runs it without threads
+    (fourier.jl:343-395): solve the initial points, queue their intervals, then take intervals
+    first in first out, solve each midpoint, compare it with the linear interpolation from the
+    ends (isapprox over the source vector at every τ of `its`) and queue (k2, k3) then (k1, k2).
+    `solve1(k)` returns S [n_τ][n_S] of one wavenumber.  Returns (sorted k, S [n][n_τ][n_S])."""
+    from collections import deque
+    K = [float(k) for k in ks]
+    S = [solve1(k) for k in K]
+    q = deque((i - 1, i) for i in range(1, len(K)))
+    while q:
+        i1, i3 = q.popleft()
+        if len(K) + 1 > nmaxks:
+            raise RuntimeError("more than nmaxks points")
+        k2 = finv((f(K[i1]) + f(K[i3])) / 2)
+        S2 = solve1(k2)
+        i2 = len(K)
+        K.append(float(k2))
+        S.append(S2)
+        Si = (S[i1] + S[i3]) / 2
+        bad = False
+        for it in range(*its.indices(S2.shape[0])):
+            d = np.linalg.norm(S2[it] - Si[it])
+            if not d <= max(atol, rtol * max(np.linalg.norm(S2[it]), np.linalg.norm(Si[it]))):
+                bad = True
+        if bad:
+            q.append((i2, i3))
+            q.append((i1, i2))
+    o = np.argsort(K, kind="stable")
+    return np.array(K)[o], np.stack(S)[o]

@@ -216,6 +216,85 @@ def source_grid_sets(probs, taus_sets, ks, bgs, ptopts=None, timing=None):
     return src.reshape(n_s, n_k, n_t, 3)
 
 
+_SRC_COL = {"ST": 0, "SE": 1, "Sψ": 2}
+
+
+def refine_levels(solve, ks, f, finv, rtol, atol, its, nmaxks=1024, refine=True):
+    """Bisection refinement of a k grid, generation by generation.  `solve(k[])` returns the source
+    arrays [n][n_τ][n_S]; an interval (k1, k3) is split at k2 = f⁻¹((f(k1) + f(k3))/2) and both
+    halves are queued again when, at some τ in `its`, |S(k2) - (S(k1) + S(k3))/2| exceeds
+    max(atol, rtol max(|S(k2)|, |(S(k1) + S(k3))/2|)) (vector norms over the sources).
+    Returns (k in insertion order, S [n][n_τ][n_S], number of solve calls)."""
+    K = list(np.asarray(ks, dtype=np.float64))
+    S = list(solve(np.asarray(K)))
+    pending = [(i - 1, i) for i in range(1, len(K))] if refine else []
+    calls = 1
+    while pending:
+        if len(K) + len(pending) > nmaxks:
+            raise RuntimeError(f"Source function refinement needs more than {nmaxks} k-refinements. "
+                               "Reduce refinement criteria.")
+        mids = np.array([finv((f(K[i1]) + f(K[i3])) / 2) for i1, i3 in pending])
+        Sm = solve(mids)
+        calls += 1
+        nxt = []
+        for (i1, i3), k2, S2 in zip(pending, mids, Sm):
+            i2 = len(K)
+            K.append(float(k2))
+            S.append(S2)
+            Si = (S[i1] + S[i3]) / 2
+            d = np.linalg.norm((S2 - Si)[its], axis=1)
+            tol = np.maximum(atol, rtol * np.maximum(np.linalg.norm(S2[its], axis=1), np.linalg.norm(Si[its], axis=1)))
+            if np.any(~(d <= tol)):
+                nxt += [(i2, i3), (i1, i2)]
+        pending = nxt
+    return np.array(K), np.stack(S), calls
+
+
+def source_grid_adaptive(prob, Ss, taus, ks, bg=None, ptopts=None, refine=True, ktransform=None, sort=True,
+                         rtol=None, atol=0.0, nmaxks=1024, timing=None):
+    """Sources on fixed τs and an adaptively refined k grid (fourier.jl:312-411).
+
+    Every interval (k1, k3) of the grid is bisected in f(k) (ktransform = (f, f⁻¹), identity by
+    default) until the source vector at the midpoint k2 matches the mean of its end values,
+    `isapprox(S(k2), (S(k1) + S(k3))/2; rtol, atol)` with norm taken over the requested sources
+    `Ss` (names among "ST", "SE", "Sψ") at each τ, the last τ excluded when there are several
+    (1/χ terms diverge there).  rtol defaults to √eps when atol is 0 (Julia's isapprox).
+
+    The reference refines interval by interval from a task queue; here every pending interval of
+    a generation is bisected at once and all midpoints are solved in ONE GPU launch, which yields
+    the same set of wavenumbers (each decision depends only on its interval).  More than `nmaxks`
+    points is an error, as in the reference.  taus = None evaluates at τ0 only.
+    Returns (ks [n], S [n_τ][n][len(Ss)]), sorted by k unless sort=False (then in insertion
+    order: initial points, then each generation's midpoints in interval order)."""
+    cols = [_SRC_COL[x] for x in Ss]
+    ks = np.asarray(ks, dtype=np.float64)
+    if len(ks) < 2:
+        raise ValueError("Initial k-grid must have at least 2 values")
+    f, finv = ktransform if ktransform is not None else ((lambda x: x), (lambda y: y))
+    for kk in (ks.min(), ks.max()):
+        if not np.isclose(kk, finv(f(kk))):
+            raise ValueError("ktransform is not a tuple of inverse functions")
+    if rtol is None:
+        rtol = math.sqrt(np.finfo(np.float64).eps) if atol == 0 else 0.0
+    if bg is None:
+        bg = solvebg(prob)
+    tsave = np.array([bg.tau0]) if taus is None else np.asarray(taus, dtype=np.float64)
+    nt = len(tsave)
+    its = slice(0, max(nt - 1, 1))
+
+    def solve(kv):
+        return source_grid(prob, tsave, kv, bg, ptopts, None)[:, :, cols]   # [n][n_τ][n_S]
+
+    K, S, launches = refine_levels(solve, ks, f, finv, rtol, atol, its, nmaxks, refine)
+    S = S.transpose(1, 0, 2)                   # [n_τ][n][n_S]
+    if timing is not None:
+        timing["launches"] = launches
+    if sort:
+        o = np.argsort(K, kind="stable")
+        K, S = K[o], S[:, o]
+    return K, S
+
+
 def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_tau0=math.pi, xs=None, taucut=1e-2,
                  l_limber=10, bgopts=None, ptopts=None, timing=None, details=None, bg=None):
     """Angular power spectra C_l^{AB} for `modes` such as ["TT", "EE", "TE", "ψψ"] at jl.l (or, with

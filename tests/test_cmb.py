@@ -64,6 +64,27 @@ def test_grids_follow_the_reference():
     assert oc.cosgrid(0.0, 1.0, 300)[-1] == 1.0
 
 
+def test_level_refinement_equals_queue_refinement():
+    # the generation-by-generation bisection (all midpoints of a generation in one launch) picks
+    # exactly the wavenumbers of the reference's interval queue, on an analytic "source"
+    from symboltz.cmb import refine_levels
+    taus = np.linspace(0.1, 3.0, 7)
+
+    def S_of(k):
+        k = np.atleast_1d(k)[:, None]
+        t = taus[None, :]
+        return np.stack([np.sin(k * t / 40.0) * np.exp(-k / 900.0), np.cos(k / 70.0) / (1 + t)], axis=2)
+
+    for rtol, f, finv in ((3e-2, np.log, np.exp), (2e-2, lambda x: x, lambda y: y)):
+        ks = np.geomspace(1.0, 3000.0, 9)
+        its = slice(0, len(taus) - 1)
+        K1, S1, calls = refine_levels(S_of, ks, f, finv, rtol, 0.0, its)
+        K2, S2 = oc.refine_queue(lambda k: S_of(k)[0], ks, f, finv, rtol, 0.0, its)
+        o = np.argsort(K1, kind="stable")
+        assert len(K1) > 40 and np.array_equal(K1[o], K2) and np.array_equal(S1[o], S2)
+        assert calls < len(K1) / 4
+
+
 def test_oracle_cmb_spectra_match_class_fixture(fixture5):
     S = fixture5
     d, lsc, ls = fixture_ls()
@@ -140,3 +161,38 @@ def test_gpu_cmb_batch_of_cosmologies_matches_single_runs(fixture5):
     Do, _ = oc.spectrum_cmb(O, lambda k: spectrum_primordial(k, probs[1]), modes, ls)
     for i in range(4):
         assert normwise(Db[1][:, i], Do[:, i]) < 1e-4, i
+
+
+@pytest.mark.gpu
+def test_gpu_source_grid_adaptive_matches_oracle(fixture5):
+    from symboltz.cmb import source_grid_adaptive
+    from symboltz.problem import solvebg
+    from oracle import oracle
+    print("case: adaptive k refinement of S_T, S_E on 30 τ, k in [1, 2000] H0/c", flush=True)
+    S = fixture5
+    prob = CosmologyProblem(S.M, S.pars)
+    bg = solvebg(prob)
+    taus = np.linspace(0.05, bg.tau0, 30)
+    ks0 = np.geomspace(1.0, 2000.0, 8)
+    O = oracle.Oracle(S.M, bg.P, bg.t, bg.u, bg.du)
+
+    def solve1(k):
+        r = O.solve(np.array([k]), np.array([bg.t[0]]), saveat=taus, sources=True, nthreads=1)
+        return r["src"][0][:, :2]
+
+    # absolute tolerance: 1 % of the largest source vector on the initial grid (a relative test
+    # per τ would chase the tiny pre-recombination values)
+    atol = 0.01 * max(np.linalg.norm(solve1(k)[:-1], axis=1).max() for k in ks0)
+    tm = {}
+    K, Sg = source_grid_adaptive(prob, ["ST", "SE"], taus, ks0, bg=bg, ktransform=(np.log, np.exp), atol=atol,
+                                 timing=tm)
+    print(f"{len(K)} wavenumbers from {tm['launches']} launches", flush=True)
+    Ko, So = oc.refine_queue(solve1, ks0, np.log, np.exp, 0.0, atol, slice(0, len(taus) - 1))
+    # the same decisions up to the rare interval whose test sits at the threshold within the
+    # two solvers' agreement
+    common = np.intersect1d(K, Ko)
+    assert len(common) >= 0.97 * max(len(K), len(Ko)), (len(K), len(Ko), len(common))
+    ig = np.searchsorted(K, common)
+    io = np.searchsorted(Ko, common)
+    scale = np.abs(So).max(axis=(0, 1))
+    assert np.max(np.abs(Sg[:, ig, :] - So[io].transpose(1, 0, 2)) / scale) < 1e-4
