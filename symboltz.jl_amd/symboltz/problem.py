@@ -242,6 +242,35 @@ def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=
     return pk, res, bgs
 
 
+def spectrum_matter_jacobian(prob, ks, names, relstep=1e-3, ktini=1e-2, tauinimax=1e-4, bgopts=None, ptopts=None,
+                             timing=None):
+    """∂ log P(k) / ∂ log θ for the parameters `names` (e.g. ["Omega_c0", "Omega_b0"]) at the
+    problem's values: returns (J [n_k][n_θ], P(k) at θ).
+
+    Sensitivities by central differences in log θ with the step of the reference's test
+    (FiniteDiff's central rule, h = relstep · max(|log θ|, 1); test/runtests.jl:363-377), all
+    2 n_θ + 1 cosmologies in ONE persistent launch (`solve_sets`).  The reference differentiates
+    this map with ForwardDiff (dual numbers through the solvers); the GPU path takes the
+    differences instead, which the reference's test holds equal to atol 1e-3."""
+    names = list(names)
+    base = dict(prob.pars)
+    sets = [base]
+    hs = []
+    for n in names:
+        x = math.log(base[n])
+        h = relstep * max(abs(x), 1.0)
+        hs.append(h)
+        sets.append(dict(base, **{n: math.exp(x + h)}))
+        sets.append(dict(base, **{n: math.exp(x - h)}))
+    pk, res, _ = solve_sets(prob.M, sets, ks, bgopts=bgopts, ptopts=ptopts, ktini=ktini, tauinimax=tauinimax,
+                            timing=timing)
+    if not np.all(res["retcode"] == 0):
+        raise RuntimeError("a perturbation solve failed while differencing the power spectrum")
+    lp = np.log(pk)
+    J = np.stack([(lp[1 + 2 * i] - lp[2 + 2 * i]) / (2 * hs[i]) for i in range(len(names))], axis=1)
+    return J, pk[0]
+
+
 def reionization_optical_depth(M, pars, reltol=1e-7, abstol=1e-7):
     """Optical depth of the reionized electrons: κ(τ0) with the tanh reionization terms minus κ(τ0)
     without them (both from full background solves; κ is the reference's _κ, baryons.jl:175-177).

@@ -18,14 +18,13 @@ Parameters (the batched configuration's ranges plus the summed neutrino mass):
 """
 import math
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import models
 from .constants import k_dimensionless
 from .mlp import Mlp
-from .problem import CosmologyProblem, solvebg
+from .problem import CosmologyProblem, solvebg_sets
 
 PARAMS = ["h", "omega_b", "omega_c", "ln_As1e10", "ns", "z_re", "Sum_mnu"]
 LO = np.array([0.60, 0.020, 0.10, 2.9, 0.92, 6.2, 0.06])
@@ -55,7 +54,7 @@ def pars_of(M, x):
     return p
 
 
-def solve_spectra(X, ks_mpc=None, M=None, chunk=256, algo=None, threads=16, log=None):
+def solve_spectra(X, ks_mpc=None, M=None, chunk=1024, algo=None, log=None):
     """log P(k) [n][n_k] for every parameter row of X with the plain GPU solver (P in (c/H0)³ units),
     chunk sets per launch; returns (logP, seconds of GPU kernel time, seconds wall)."""
     from .solver import ALGO_AUTO, GpuModel
@@ -65,28 +64,28 @@ def solve_spectra(X, ks_mpc=None, M=None, chunk=256, algo=None, threads=16, log=
     out = np.zeros((len(X), len(ks_mpc)))
     kern = 0.0
     t_wall = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        for c0 in range(0, len(X), chunk):
-            rows = X[c0:c0 + chunk]
-            pars = [pars_of(M, x) for x in rows]
-            bgs = list(ex.map(lambda p: solvebg(CosmologyProblem(M, p)), pars))
-            iset, kk, tt = [], [], []
-            for s, (p, bg) in enumerate(zip(pars, bgs)):
-                gm.upload_background(s, bg.t, bg.u, bg.du, bg.P)
-                k = k_dimensionless(ks_mpc, p["h"])
-                iset.append(np.full(len(k), s, np.int32))
-                kk.append(k)
-                tt.append(np.clip(np.minimum(1e-2 / k, 1e-4), bg.t[0], bg.t[-1]))
-            plan = gm.plan(np.concatenate(iset), np.concatenate(kk), np.concatenate(tt),
-                           algo=ALGO_AUTO if algo is None else algo)
-            kern += plan.run() * 1e-3
-            r = plan.fetch()
-            if not np.all(r["retcode"] == 0):
-                raise RuntimeError("a perturbation solve failed while generating spectra")
-            out[c0:c0 + len(rows)] = np.log(plan.pk().reshape(len(rows), len(ks_mpc)))
-            plan.close()
-            if log:
-                log(f"spectra {c0 + len(rows)}/{len(X)}")
+    for c0 in range(0, len(X), chunk):
+        rows = X[c0:c0 + chunk]
+        pars = [pars_of(M, x) for x in rows]
+        # backgrounds of the chunk: one GPU launch from 256 sets on, host threads below
+        bgs = solvebg_sets([CosmologyProblem(M, p) for p in pars])
+        iset, kk, tt = [], [], []
+        for s, (p, bg) in enumerate(zip(pars, bgs)):
+            gm.upload_background(s, bg.t, bg.u, bg.du, bg.P)
+            k = k_dimensionless(ks_mpc, p["h"])
+            iset.append(np.full(len(k), s, np.int32))
+            kk.append(k)
+            tt.append(np.clip(np.minimum(1e-2 / k, 1e-4), bg.t[0], bg.t[-1]))
+        plan = gm.plan(np.concatenate(iset), np.concatenate(kk), np.concatenate(tt),
+                       algo=ALGO_AUTO if algo is None else algo)
+        kern += plan.run() * 1e-3
+        r = plan.fetch()
+        if not np.all(r["retcode"] == 0):
+            raise RuntimeError("a perturbation solve failed while generating spectra")
+        out[c0:c0 + len(rows)] = np.log(plan.pk().reshape(len(rows), len(ks_mpc)))
+        plan.close()
+        if log:
+            log(f"spectra {c0 + len(rows)}/{len(X)}")
     return out, kern, time.perf_counter() - t_wall
 
 

@@ -229,3 +229,33 @@ def hermite(t, y, dy, x):
     s = (x - t[i]) / h
     return ((1 + 2 * s) * (1 - s) ** 2 * y[i] + s * (1 - s) ** 2 * h * dy[i] + s * s * (3 - 2 * s) * y[i + 1]
             + s * s * (s - 1) * h * dy[i + 1])
+
+
+def test_spectrum_matter_jacobian():
+    # sensitivities by central differences with every shifted cosmology in one launch; pinned by
+    # the closed forms d log P / d log A_s = 1 and d log P / d log n_s = n_s log(k / k_pivot)
+    # (P0 ∝ A_s (k/k_p)^(n_s - 1), the transfer function depends on neither), and for Ω_c0, Ω_b0
+    # against the oracle's differences at the reference's tolerance (test/runtests.jl:363-377)
+    from symboltz.problem import spectrum_matter_jacobian
+    from symboltz.models import P_KPIVOT
+    say("case: d log P / d log θ for As, ns, Omega_c0, Omega_b0 at 20 k (9 cosmologies, one launch)")
+    M = models.LCDM(lmax=5)
+    pars = dict(models.parameters_Planck18(M))
+    pars.pop("As", None)
+    prob = CosmologyProblem(M, pars)
+    ks = 10 ** np.linspace(0, 3, 20)
+    J, P = spectrum_matter_jacobian(prob, ks, ["ln_As1e10", "ns", "Omega_c0", "Omega_b0"])
+    lnAs = pars["ln_As1e10"]
+    # ln_As1e10 = log(1e10 A_s): d log P / d log(ln_As1e10) = ln_As1e10 exactly
+    assert np.max(np.abs(J[:, 0] - lnAs)) < 5e-6
+    kp = prob.P[P_KPIVOT]
+    assert np.max(np.abs(J[:, 1] - pars["ns"] * np.log(ks / kp))) < 5e-6
+    S = Setup(M, pars)
+    base = dict(pars)
+    for j, n in ((2, "Omega_c0"), (3, "Omega_b0")):
+        x = np.log(base[n])
+        h = 1e-3 * max(abs(x), 1.0)
+        Pp = Setup(M, dict(base, **{n: np.exp(x + h)})).O.spectrum_matter(ks)[0]
+        Pm = Setup(M, dict(base, **{n: np.exp(x - h)})).O.spectrum_matter(ks)[0]
+        Jo = (np.log(Pp) - np.log(Pm)) / (2 * h)
+        assert np.max(np.abs(J[:, j] - Jo)) < 1e-3, n
