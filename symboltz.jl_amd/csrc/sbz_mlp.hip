@@ -131,6 +131,98 @@ extern "C" __global__ void __launch_bounds__(256) sbz_mlp_gemm(SbzGemm g) {
             }
 }
 
+// Large products with both operands contiguous along the contraction index p (A(i, p) = A[i a_i + p],
+// B(p, j) = B[j b_j + p]): the forward pass on the transposed weight copy and the backward-data
+// product on the weights themselves.  A 256-thread workgroup computes a 128 x 128 tile, each
+// wavefront a 64 x 64 quarter as 4 x 4 blocks of v_mfma_f32_16x16x32_bf16; K advances 32 at a time
+// through LDS, filled with 16-byte global loads and 16-byte LDS stores (rows padded by 8 elements).
+// Requires M, N multiples of 128, K a multiple of 32, a_i, b_j multiples of 8 and 16-byte aligned
+// operands (the host checks and otherwise uses sbz_mlp_gemm).  Measured alternatives that were
+// slower on MI355X at 65536 x 512 x 512: K steps of 64 (36 KB of LDS, fewer resident workgroups)
+// and prefetching the next K tile into registers.
+typedef __bf16 sbz_bf16x8 __attribute__((ext_vector_type(8)));
+#define FBM 128
+#define FBN 128
+#define FBK 32
+#define FPAD 8
+
+extern "C" __global__ void __launch_bounds__(256) sbz_mlp_gemm_kk(SbzGemm g) {
+    __shared__ __attribute__((aligned(16))) bf16_t sA[FBM][FBK + FPAD];
+    __shared__ __attribute__((aligned(16))) bf16_t sB[FBN][FBK + FPAD];   // sB[j][p]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = blockIdx.y * FBM, j0 = blockIdx.x * FBN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    sbz_float4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = sbz_float4{0.f, 0.f, 0.f, 0.f};
+    // staging: 128 rows x 32 columns = 512 chunks of 8 elements per operand, 2 per thread
+    const int r0 = tid >> 2, c0 = (tid & 3) * 8;
+    for (int p0 = 0; p0 < g.K; p0 += FBK) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = r0 + 64 * h;
+            const uint4 va = *(const uint4*)(g.A + (long long)(i0 + r) * g.a_i + p0 + c0);
+            const uint4 vb = *(const uint4*)(g.B + (long long)(j0 + r) * g.b_j + p0 + c0);
+            *(uint4*)&sA[r][c0] = va;
+            *(uint4*)&sB[r][c0] = vb;
+        }
+        __syncthreads();
+        // operand layout of the 16x16x32 product: lane l supplies row (l % 16) of A and column
+        // (l % 16) of B at k = 8 (l / 16) .. + 7
+        const int kq = 8 * (lane >> 4);
+        sbz_bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            af[a] = *(const sbz_bf16x8*)&sA[wm + 16 * a + (lane & 15)][kq];
+            bfr[a] = *(const sbz_bf16x8*)&sB[wn + 16 * a + (lane & 15)][kq];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+    // result layout: lane l holds rows 4 (l / 16) .. + 3 of column (l % 16)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + wm + 16 * a + 4 * (lane >> 4) + q;
+                const int j = j0 + wn + 16 * b + (lane & 15);
+                float v = acc[a][b][q];
+                if (g.bias) v += g.bias[j];
+                const long long o = (long long)i * g.N + j;
+                if (g.out_f) g.out_f[o] = v;
+                if (g.out_h) {
+                    float hv = v;
+                    if (g.epi == SBZ_EPI_GELU) hv = sbz_gelu(v);
+                    else if (g.epi == SBZ_EPI_GELU_BWD) hv = v * sbz_gelu_d(g.zin[o]);
+                    g.out_h[o] = sbz_f2bf(hv);
+                }
+            }
+}
+
+// BF16 transposed copy of one weight matrix: dst [N][K] = src [K][N]
+extern "C" __global__ void sbz_mlp_transpose(const bf16_t* __restrict__ src, int K, int N, bf16_t* __restrict__ dst) {
+    __shared__ bf16_t t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 256 threads: 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int p = by + r, j = bx + tx;
+        t[r][tx] = (p < K && j < N) ? src[(long long)p * N + j] : (bf16_t)0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = bx + r, p = by + tx;
+        if (j < N && p < K) dst[(long long)j * K + p] = t[tx][r];
+    }
+}
+
 // batch gather: rows idx[r] of the training inputs (FP32 -> BF16) and targets (FP32)
 extern "C" __global__ void sbz_mlp_gather(const float* __restrict__ X, const float* __restrict__ T,
                                           const int* __restrict__ idx, int B, int nin, int nout,
@@ -231,6 +323,8 @@ struct sbz_mlp {
     std::vector<long long> off_w, off_b;
     float *par = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;   // FP32 [n_par]
     bf16_t* par_h = nullptr;   // BF16 copy of the weights (biases unused)
+    bf16_t* par_t = nullptr;   // BF16 transposed weights ([N][K] per layer) for sbz_mlp_gemm_kk
+    bool par_t_ok = false;     // par_t matches par_h
     std::vector<float*> z;     // per layer: pre-activation [max_batch][w[l+1]] FP32
     std::vector<bf16_t*> x;    // per layer input activation [max_batch][w[l]] BF16 (x[L] = output, unused)
     bf16_t *dh = nullptr;      // gradient scratch BF16 [max_batch][max width] x 2
@@ -247,7 +341,7 @@ struct sbz_mlp {
 };
 
 static void mlp_free(sbz_mlp* h) {
-    for (void* p : {(void*)h->par, (void*)h->m, (void*)h->v, (void*)h->grad, (void*)h->par_h, (void*)h->dh,
+    for (void* p : {(void*)h->par, (void*)h->m, (void*)h->v, (void*)h->grad, (void*)h->par_h, (void*)h->par_t, (void*)h->dh,
                     (void*)h->df, (void*)h->y, (void*)h->tb, (void*)h->idx, (void*)h->loss, (void*)h->data_x,
                     (void*)h->data_t})
         if (p) (void)hipFree(p);
@@ -258,10 +352,36 @@ static void mlp_free(sbz_mlp* h) {
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
+// the 128 x 128 kernel applies to contraction-contiguous operands of tile-multiple shape, and pays
+// only when the grid fills the chip (at least 256 tiles)
+static bool kk_fits(const SbzGemm& g) {
+    auto al16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+    return g.a_p == 1 && g.b_p == 1 && g.M % FBM == 0 && g.N % FBN == 0 && g.K % FBK == 0 && g.a_i % 8 == 0 &&
+           g.b_j % 8 == 0 && al16(g.A) && al16(g.B) && (long long)(g.M / FBM) * (g.N / FBN) >= 256;
+}
+
 static hipError_t gemm(sbz_mlp* h, const SbzGemm& g) {
+    if (kk_fits(g)) {
+        hipLaunchKernelGGL(sbz_mlp_gemm_kk, dim3(g.N / FBN, g.M / FBM), dim3(256), 0, h->stream, g);
+        return hipGetLastError();
+    }
     dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
     hipLaunchKernelGGL(sbz_mlp_gemm, grid, dim3(256), 0, h->stream, g);
     return hipGetLastError();
+}
+
+// refresh the transposed weight copy after the weights changed
+static hipError_t refresh_transposed(sbz_mlp* h) {
+    if (h->par_t_ok) return hipSuccess;
+    for (int l = 0; l < h->L; ++l) {
+        const int K = h->w[l], N = h->w[l + 1];
+        hipLaunchKernelGGL(sbz_mlp_transpose, dim3((N + 31) / 32, (K + 31) / 32), dim3(256), 0, h->stream,
+                           h->par_h + h->off_w[l], K, N, h->par_t + h->off_w[l]);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    h->par_t_ok = true;
+    return hipSuccess;
 }
 
 extern "C" int sbz_mlp_create(int n_layers, const int* widths, int max_batch, unsigned long long seed,
@@ -313,6 +433,7 @@ extern "C" int sbz_mlp_create(int n_layers, const int* widths, int max_batch, un
     for (float** p : {&h->par, &h->m, &h->v, &h->grad})
         if ((e = hipMalloc((void**)p, P4)) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMalloc((void**)&h->par_h, sizeof(bf16_t) * h->n_par)) != hipSuccess) return fail(e, "hipMalloc");
+    if ((e = hipMalloc((void**)&h->par_t, sizeof(bf16_t) * h->n_par)) != hipSuccess) return fail(e, "hipMalloc");
     if ((e = hipMemcpy(h->par, p0.data(), P4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "upload");
     if ((e = hipMemcpy(h->par_h, ph.data(), sizeof(bf16_t) * h->n_par, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "upload");
@@ -372,6 +493,7 @@ extern "C" int sbz_mlp_set_params(sbz_mlp* h, const float* p) {
     for (long long e = 0; e < h->n_par; ++e) ph[e] = sbz_f2bf(p[e]);
     MCHK(hipMemcpy(h->par, p, sizeof(float) * h->n_par, hipMemcpyHostToDevice));
     MCHK(hipMemcpy(h->par_h, ph.data(), sizeof(bf16_t) * h->n_par, hipMemcpyHostToDevice));
+    h->par_t_ok = false;
     return SBZ_OK;
 }
 
@@ -387,6 +509,21 @@ static hipError_t forward(sbz_mlp* h, int B, float* y, bool keep_z) {
         g.B = h->par_h + h->off_w[l];
         g.b_p = h->w[l + 1];
         g.b_j = 1;
+        {
+            // the same product on the transposed weights W^T [N][K], when the large kernel applies
+            SbzGemm t = g;
+            t.B = h->par_t + h->off_w[l];
+            t.b_p = 1;
+            t.b_j = h->w[l];
+            t.M = B;
+            t.N = h->w[l + 1];
+            t.K = h->w[l];
+            if (kk_fits(t)) {
+                hipError_t e = refresh_transposed(h);
+                if (e != hipSuccess) return e;
+                g = t;
+            }
+        }
         g.M = B;
         g.N = h->w[l + 1];
         g.K = h->w[l];
@@ -527,6 +664,7 @@ extern "C" int sbz_mlp_train(sbz_mlp* h, const int* idx, int n_batches, int batc
         hipLaunchKernelGGL(sbz_mlp_adam, dim3(1024), dim3(256), 0, h->stream, h->par, h->m, h->v, h->grad, h->n_par, lr,
                            b1, b2, eps, c1, c2, h->par_h);
         MCHK(hipGetLastError());
+        h->par_t_ok = false;
     }
     MCHK(hipEventRecord(e1, h->stream));
     MCHK(hipEventSynchronize(e1));

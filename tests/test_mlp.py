@@ -60,6 +60,28 @@ def test_gpu_forward_and_gradient_match_oracle():
 
 
 @pytest.mark.gpu
+def test_gpu_large_batch_forward_matches_oracle():
+    # 16384 rows: the hidden layers take the 128 x 128 MFMA kernel on the transposed weights
+    # (sbz_mlp_gemm_kk); the input layer (K = 7) keeps the general kernel
+    from symboltz.mlp import Mlp
+    print("case: MLP forward 16384 x [7, 512, 512, 256]", flush=True)
+    widths = [7, 512, 512, 256]
+    net = Mlp(widths, max_batch=16384, seed=4)
+    p = net.get_params()
+    _, x, _ = _problem(widths, 16384, 2)
+    y, _ = net.forward(x)
+    y_ref = mo.forward(p, widths, x)[0]
+    assert np.max(np.abs(y - y_ref)) <= 2e-2 * np.max(np.abs(y_ref))
+    assert np.linalg.norm(y - y_ref) <= 5e-3 * np.linalg.norm(y_ref)
+    # after an update the transposed copy follows the weights
+    net.set_params(p * 0.5)
+    y2, _ = net.forward(x)
+    y2_ref = mo.forward(p * 0.5, widths, x)[0]
+    assert np.linalg.norm(y2 - y2_ref) <= 5e-3 * np.linalg.norm(y2_ref)
+    net.close()
+
+
+@pytest.mark.gpu
 def test_gpu_training_fits_a_smooth_map():
     from symboltz.mlp import Mlp
     rng = np.random.default_rng(5)
