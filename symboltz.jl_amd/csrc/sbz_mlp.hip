@@ -315,6 +315,18 @@ extern "C" __global__ void sbz_mlp_adam(float* __restrict__ w, float* __restrict
         }                                                                                         \
     } while (0)
 
+// device buffers and events released at scope exit (error paths included)
+struct MlpScoped {
+    std::vector<void*> bufs;
+    std::vector<hipEvent_t> evs;
+    ~MlpScoped() {
+        for (void* p : bufs)
+            if (p) (void)hipFree(p);
+        for (hipEvent_t e : evs)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
 struct sbz_mlp {
     int L = 0;                 // number of layers (weight matrices)
     std::vector<int> w;        // widths w[0..L]
@@ -647,9 +659,12 @@ extern "C" int sbz_mlp_train(sbz_mlp* h, const int* idx, int n_batches, int batc
             sbz_set_error("sbz_mlp_train: index %lld out of range", e);
             return SBZ_ERR_ARG;
         }
-    hipEvent_t e0, e1;
+    MlpScoped sc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     MCHK(hipEventCreate(&e0));
+    sc.evs.push_back(e0);
     MCHK(hipEventCreate(&e1));
+    sc.evs.push_back(e1);
     MCHK(hipEventRecord(e0, h->stream));
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int s = 0; s < n_batches; ++s) {
@@ -671,8 +686,6 @@ extern "C" int sbz_mlp_train(sbz_mlp* h, const int* idx, int n_batches, int batc
     float ms = 0.f;
     MCHK(hipEventElapsedTime(&ms, e0, e1));
     if (kernel_ms) *kernel_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return SBZ_OK;
 }
 
@@ -684,13 +697,18 @@ extern "C" int sbz_mlp_forward(sbz_mlp* h, const float* x, long long n, float* y
         return SBZ_ERR_ARG;
     }
     const int nin = h->w[0], nout = h->w[h->L];
+    MlpScoped sc;
     float *dx = nullptr, *dy = nullptr;
     MCHK(hipMalloc((void**)&dx, sizeof(float) * n * nin));
+    sc.bufs.push_back(dx);
     MCHK(hipMalloc((void**)&dy, sizeof(float) * n * nout));
+    sc.bufs.push_back(dy);
     MCHK(hipMemcpy(dx, x, sizeof(float) * n * nin, hipMemcpyHostToDevice));
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     MCHK(hipEventCreate(&e0));
+    sc.evs.push_back(e0);
     MCHK(hipEventCreate(&e1));
+    sc.evs.push_back(e1);
     MCHK(hipEventRecord(e0, h->stream));
     for (long long r0 = 0; r0 < n; r0 += h->max_batch) {
         const int B = (int)std::min<long long>(h->max_batch, n - r0);
@@ -705,9 +723,5 @@ extern "C" int sbz_mlp_forward(sbz_mlp* h, const float* x, long long n, float* y
     MCHK(hipEventElapsedTime(&ms, e0, e1));
     if (kernel_ms) *kernel_ms = ms;
     MCHK(hipMemcpy(y, dy, sizeof(float) * n * nout, hipMemcpyDeviceToHost));
-    (void)hipFree(dx);
-    (void)hipFree(dy);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return SBZ_OK;
 }

@@ -259,3 +259,34 @@ def test_spectrum_matter_jacobian():
         Pm = Setup(M, dict(base, **{n: np.exp(x - h)})).O.spectrum_matter(ks)[0]
         Jo = (np.log(Pp) - np.log(Pm)) / (2 * h)
         assert np.max(np.abs(J[:, j] - Jo)) < 1e-3, n
+
+
+def test_massive_nu_n100_with_sources_matches_oracle():
+    # the benchmark's model (photon and massless-ν lmax 16, one massive ν with 4 bins and lmax 10,
+    # n = 100): P(k), final states and CMB sources against the oracle
+    from symboltz.solver import SAVE_SOURCES, SAVE_STATES
+    say("case: n = 100 massive-ν model, 12 modes in [1e-4, 0.5]/Mpc, sources on 80 τ")
+    M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
+    pars = dict(models.parameters_Planck18(M), Neff=2.046, N=1, m_eV=0.06)
+    S = Setup(M, pars)
+    from symboltz.constants import k_dimensionless
+    ks = k_dimensionless(10 ** np.linspace(-4, np.log10(0.5), 12), pars["h"])   # below the ill-conditioned ISW range
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    saveat = np.linspace(max(S.t[0], 1e-2), S.t[-1], 80)
+    gm = GpuModel(S.M)
+    gm.upload_background(0, S.t, S.u, S.du, S.P)
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, saveat=saveat, save=SAVE_SOURCES | SAVE_STATES)
+    plan.run()
+    g = plan.fetch(save=True)
+    src = plan.sources()
+    pk = plan.pk()
+    plan.close()
+    o = S.O.solve(ks, t0, saveat=saveat, sources=True)
+    assert np.all(g["retcode"] == 0) and np.all(o["retcode"] == 0)
+    Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
+    assert np.max(np.abs(pk / Po - 1)) < 1e-4
+    assert rel_state_err(g["u"], o["u"]) < 1e-4
+    for i in range(len(ks)):
+        for c in range(3):
+            scale = np.abs(o["src"][i, :, c]).max() + 1e-300
+            assert np.max(np.abs(src[i, :, c] - o["src"][i, :, c])) <= 1e-4 * scale, (i, c)
