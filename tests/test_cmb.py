@@ -196,3 +196,26 @@ def test_gpu_source_grid_adaptive_matches_oracle(fixture5):
     io = np.searchsorted(Ko, common)
     scale = np.abs(So).max(axis=(0, 1))
     assert np.max(np.abs(Sg[:, ig, :] - So[io].transpose(1, 0, 2)) / scale) < 1e-4
+
+
+@pytest.mark.gpu
+def test_gpu_cmb_temperature_polarization_only_matches_oracle(fixture5):
+    # without lensing the reference samples 61 linear Chebyshev nodes on [1e-2, 2e3]
+    # (angular.jl:270-272); TT, TE, EE at the cached l only
+    from symboltz.bessel import SphericalBesselCache
+    from symboltz.cmb import spectrum_cmb
+    print("case: C_l TT/TE/EE without lensing, 61 source modes", flush=True)
+    S = fixture5
+    ls = np.unique(np.round(np.geomspace(2, 1500, 60)).astype(int))
+    prob = CosmologyProblem(S.M, S.pars)
+    from symboltz.problem import solvebg
+    bg = solvebg(prob)
+    jl = SphericalBesselCache(ls)
+    det = {}
+    D = spectrum_cmb(["TT", "TE", "EE"], prob, jl, normalization="Dl", bg=bg, details=det)
+    assert len(det["ks_coarse"]) == 61 and det["ks_coarse"][0] == 2e3
+    from oracle import oracle
+    O = oracle.Oracle(S.M, bg.P, bg.t, bg.u, bg.du)
+    Do, _ = oc.spectrum_cmb(O, lambda k: spectrum_primordial(k, prob), ["TT", "TE", "EE"], ls, lensing=False)
+    for i in range(3):
+        assert normwise(D[:, i], Do[:, i]) < 1e-4, i
