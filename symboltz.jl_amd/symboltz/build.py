@@ -76,6 +76,7 @@ def standard_models():
         LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4),     # photons/ν lmax 16, one massive ν (4 bins, lmax 10)
         w0waCDM(lmax=6),                                  # w0wa dark-energy fluid with massive ν
         LCDM(lmax=6),                                     # the surrogate's model (3 massive ν)
+        LCDM(lmax=32),                                    # the reference's high-lmax test, n = 236 (runtests.jl:808-812)
     ]
 
 

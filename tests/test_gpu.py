@@ -290,3 +290,20 @@ def test_massive_nu_n100_with_sources_matches_oracle():
         for c in range(3):
             scale = np.abs(o["src"][i, :, c]).max() + 1e-300
             assert np.max(np.abs(src[i, :, c] - o["src"][i, :, c])) <= 1e-4 * scale, (i, c)
+
+
+def test_high_lmax_32_matches_oracle():
+    # the reference's "High lmax" test (test/runtests.jl:808-812): ΛCDM(lmax = 32), n = 236, solved
+    # from the background start at k = 0.1 .. 1000; here also against the oracle
+    say("case: ΛCDM lmax 32 (n = 236), 5 modes from the background start")
+    M = models.LCDM(lmax=32)
+    prob = CosmologyProblem(M, models.parameters_Planck18(M))
+    ks = np.array([1e-1, 1e0, 1e1, 1e2, 1e3])
+    sol = solve(prob, ks)
+    assert issuccess(sol)
+    S = Setup(M)
+    o = S.O.solve(ks, np.full(len(ks), S.t[0]))
+    assert np.all(o["retcode"] == 0)
+    assert rel_state_err(sol.pts["u"], o["u"]) < 1e-4
+    Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
+    assert np.max(np.abs(sol.pts["pk"] / Po - 1)) < 1e-4
