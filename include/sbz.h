@@ -136,6 +136,12 @@ int sbz_plan_fetch_sources(sbz_plan* p, double* out_src);
 /* Matter power spectrum P(k, τ0) = P0(k) Δ_m² per instance from the final states
  * (spectrum_matter, src/observables/fourier.jl:79-101). */
 int sbz_plan_pk(sbz_plan* p, double* pk);
+/* P(k, τ) = P0(k) Δ_S(τ)² for species combinations S given as bit masks (1: CDM, 2: baryons,
+ * 4: massive ν; a single species is its own Δ, several are ρ-weighted), at τ0 from the final
+ * states (at_saves = 0) or at every saveat time from the saved states (at_saves = 1, needs
+ * SBZ_SAVE_STATES): spectrum_matter(modes, sol, k, τ) (src/observables/fourier.jl:39-52,90-97).
+ * out [n_masks][n_t][n_inst] with n_t = 1 or n_save.  The massive-ν bit needs a model with them. */
+int sbz_plan_pk_modes(sbz_plan* p, const int* masks, int n_masks, int at_saves, double* out);
 int sbz_plan_destroy(sbz_plan* p);
 /* Resident capacity of a kernel variant on the model's device: wavefronts (SBZ_ALGO_WAVE,
  * SBZ_ALGO_WAVE2) or lanes (SBZ_ALGO_LANE) that run at once.  No reference counterpart: the

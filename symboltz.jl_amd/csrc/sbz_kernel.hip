@@ -1012,6 +1012,38 @@ extern "C" __global__ void sbz_pk_kernel(const double* __restrict__ params, cons
     pk[i] = p0 * dm * dm;
 }
 
+// P(k, τ) = P0(k) Δ_S(τ)² for species combinations S (bit masks: c, b, h) at τ0 from the final
+// states (taus null, n_t = 1) or at every saveat time from the saved states; a single species is
+// its own Δ, several are ρ-weighted (spectrum_matter(modes, sol, k, τ) with
+// total_symbolic_gauge_invariant_overdensities, fourier.jl:39-52,90-97).  out [n_m][n_t][n_inst].
+extern "C" __global__ void sbz_pk_modes_kernel(const double* __restrict__ params, const double* __restrict__ knots,
+                                               const long long* __restrict__ knot_off, const int* __restrict__ knot_n,
+                                               const int* __restrict__ inst_set, const double* __restrict__ inst_k,
+                                               const double* __restrict__ states, int n_inst, int n_t,
+                                               const double* __restrict__ taus, int taus_stride,
+                                               const int* __restrict__ masks, int n_m, double* __restrict__ out) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)n_inst * n_t) return;
+    const int i = (int)(e / n_t), it = (int)(e % n_t);
+    const int set = inst_set[i];
+    const double* P = params + (size_t)set * SBZ_NPAR;
+    const double* kt = knots + knot_off[set];
+    const int nk = knot_n[set];
+    const double k = inst_k[i];
+    const double tau = taus ? taus[(size_t)set * taus_stride + it] : kt[nk - 1];
+    double Cs[SBZ_NC], B[SBZ_NB];
+    sbz_consts(P, Cs);
+    sbz_consts_k(Cs, k);
+    int idx = sbz_find_knot(kt, nk, tau);
+    sbz_bg_plain(P, kt, nk, idx, tau, B);
+    CSV u{states + ((size_t)i * n_t + it) * SBZ_N, 1};
+    const double p0 = 2.0 * sbzc::PI * sbzc::PI / (k * k * k) * P[SBZ_P_AS] * pow(k / P[SBZ_P_KPIVOT], P[SBZ_P_NS] - 1.0);
+    for (int m = 0; m < n_m; ++m) {
+        const double d = sbz_dspecies(B, Cs, k, u, masks[m]);
+        out[((size_t)m * n_t + it) * n_inst + i] = p0 * d * d;
+    }
+}
+
 #ifdef SBZ_HOST_PROBE
 // Host entry points into the emitted model functions, compiled only into the test probe library
 // (SBZ_DEV = __host__ __device__): they let the CPU test-suite compare the emitted RHS, W, LU,

@@ -35,7 +35,7 @@ struct SbzSetTable {
 struct sbz_model {
     int n = 0, nnz_lu = 0, nw = 0;
     hipModule_t module = nullptr;
-    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr;
+    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -172,6 +172,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
     if (hipModuleGetFunction(&m->k_solve, m->module, "sbz_rodas5p_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_wave, m->module, "sbz_wave_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_wave2, m->module, "sbz_wave_kernel_o2") != hipSuccess ||
+        hipModuleGetFunction(&m->k_pk_modes, m->module, "sbz_pk_modes_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk, m->module, "sbz_pk_kernel") != hipSuccess) {
         sbz_set_error("code object lacks the solver kernels");
         (void)hipModuleUnload(m->module);
@@ -429,6 +430,63 @@ extern "C" int sbz_plan_pk(sbz_plan* p, double* pk) {
     HIPCHK(hipMemcpyAsync(pk, p->d_pk, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     return SBZ_OK;
+}
+
+extern "C" int sbz_plan_pk_modes(sbz_plan* p, const int* masks, int n_masks, int at_saves, double* out) {
+    sbz_clear_error();
+    if (!p || !masks || !out || n_masks < 1) {
+        sbz_set_error("sbz_plan_pk_modes: invalid argument");
+        return SBZ_ERR_ARG;
+    }
+    for (int j = 0; j < n_masks; ++j)
+        if (masks[j] < 1 || masks[j] > 7) {
+            sbz_set_error("sbz_plan_pk_modes: species mask %d is not a non-empty subset of c (1), b (2), h (4)",
+                          masks[j]);
+            return SBZ_ERR_ARG;
+        }
+    if (at_saves && (!(p->save_flags & SBZ_SAVE_STATES) || p->n_save < 1 || !p->d_save)) {
+        sbz_set_error("sbz_plan_pk_modes: the plan saved no states at saveat times");
+        return SBZ_ERR_ARG;
+    }
+    if (p->n_inst == 0) return SBZ_OK;
+    sbz_model* m = p->m;
+    const double* params = p->d_params;
+    const double* knots = p->d_knots;
+    const long long* koff = p->d_koff;
+    const int* kn = p->d_knotn;
+    const int* set = p->d_set;
+    const double* kk = p->d_k;
+    const double* states = at_saves ? p->d_save : p->d_out;
+    int n = p->n_inst;
+    int n_t = at_saves ? p->n_save : 1;
+    const double* taus = at_saves ? p->d_saveat : nullptr;
+    int stride = p->saveat_stride;
+    int nm = n_masks;
+    int* d_masks = nullptr;
+    double* d_out = nullptr;
+    const size_t n_out = (size_t)n_masks * n_t * n;
+    HIPCHK(hipMalloc((void**)&d_masks, sizeof(int) * n_masks));
+    if (hipMalloc((void**)&d_out, sizeof(double) * n_out) != hipSuccess) {
+        (void)hipFree(d_masks);
+        sbz_set_error("sbz_plan_pk_modes: hipMalloc of %zu doubles failed", n_out);
+        return SBZ_ERR_HIP;
+    }
+    int st = SBZ_OK;
+    void* args[] = {&params, &knots, &koff, &kn, &set, &kk, &states, &n, &n_t, &taus, &stride, &d_masks, &nm, &d_out};
+    const long long nth = (long long)n * n_t;
+    hipError_t e = hipMemcpyAsync(d_masks, masks, sizeof(int) * n_masks, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess)
+        e = hipModuleLaunchKernel(m->k_pk_modes, (unsigned)((nth + 255) / 256), 1, 1, 256, 1, 1, 0, m->stream, args,
+                                  nullptr);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(double) * n_out, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) {
+        sbz_set_error("sbz_plan_pk_modes: %s", hipGetErrorString(e));
+        st = SBZ_ERR_HIP;
+    }
+    (void)hipFree(d_masks);
+    (void)hipFree(d_out);
+    return st;
 }
 
 extern "C" int sbz_plan_fetch(sbz_plan* p, double* out_u, double* out_save, int* retcode, long long* stats) {

@@ -298,6 +298,22 @@ class System:
         Spsi = -(Psi + Phi) * (B["tau"] - taurec) / (tau0 - taurec) / (tau0 - B["tau"])
         return ST, SE, Spsi
 
+    def species_deltas(self):
+        """(Δ_s, ρ_s) for s = c, b and, with massive ν, h: Δ_s = δ_s + 3ℋ(1+w_s)θ_s/k², the species of
+        total_symbolic_gauge_invariant_overdensities (fourier.jl:39-52)."""
+        M, L, u, B, C, k = self.M, self.L, self.u, self.B, self.C, self.k
+        H = B["Hc"]
+        out = [(u[L["C"]] + 3 * H * u[L["C"] + 1] / k**2, B["rc"]),
+               (u[L["B"]] + 3 * H * u[L["B"] + 1] / k**2, B["rb"])]
+        if M.has_h:
+            Lh = M.lmax_h
+            Idr = sum(C[f"w{i}"] * B[f"E{i}"] * u[L["H"] + i * (Lh + 1)] for i in range(M.nx))
+            uh = sum(C[f"w{i}"] * C[f"x{i}"] * u[L["H"] + i * (Lh + 1) + 1] for i in range(M.nx)) / \
+                (B["Irho"] + B["IP"] / 3)
+            wh = B["ph"] / B["rh"]
+            out.append((Idr / B["Irho"] + 3 * H * (1 + wh) * (uh * k) / k**2, B["rh"]))
+        return out
+
     def delta_m(self):
         """Δ_m = Σ ρ_s Δ_s / Σ ρ_s over c, b, h; Δ_s = δ_s + 3ℋ(1+w_s)θ_s/k² (fourier.jl:39-52)."""
         M, L, u, B, C, k = self.M, self.L, self.u, self.B, self.C, self.k
@@ -651,6 +667,23 @@ class Emitted:
         out += [ln.replace("    double r =", "    const double r =") for ln in lines]
         out.append("    return r;")
         out.append("}")
+        # Δ of a species combination (bit 0: c, 1: b, 2: h); one species is returned as is
+        spc = sysm.species_deltas()
+        items = []
+        for j, (dj, rj) in enumerate(spc):
+            items += [(f"double d{j}", dj), (f"double r{j}", rj)]
+        lines, _ = _emit_block(items, "ts")
+        out.append("template <class UI>")
+        out.append("SBZ_DEV double sbz_dspecies(const double* __restrict__ B, const double* __restrict__ Cs, double k, "
+                   "UI u, int mask) {")
+        out += lines
+        out.append("    double num = 0.0, den = 0.0, one = 0.0;")
+        out.append("    int cnt = 0;")
+        for j in range(len(spc)):
+            out.append(f"    if (mask & {1 << j}) {{ num += r{j} * d{j}; den += r{j}; one = d{j}; ++cnt; }}")
+        out.append("    return cnt == 1 ? one : num / den;")
+        out.append("}")
+        out.append(f"#define SBZ_SPECIES_MASK {(1 << len(spc)) - 1}")
         out += self._emit_wave()
         f_f, f_w, f_lu, f_s, f_d = (self.ops["f"][0] + self.ops["f"][1], self.ops["w"][0] + self.ops["w"][1],
                                     self.ops["lu"][0], self.ops["solve"][0],

@@ -215,6 +215,59 @@ def spectrum_matter(prob, ks, ktini=1e-2, tauinimax=1e-4, **kw):
     return sol.pts["pk"], sol
 
 
+def species_mask(M, mode):
+    """Bit mask of the species of a spectrum_matter mode: "m" (all matter: c, b and massive ν when
+    present) or any combination of "c", "b", "h" (fourier.jl:39-52)."""
+    bits = {"c": 1, "b": 2, "h": 4}
+    if mode == "m":
+        return 7 if M.has_h else 3
+    mask = 0
+    for ch in mode:
+        if ch not in bits:
+            raise ValueError(f"unknown species '{ch}' in matter power spectrum mode '{mode}'")
+        if ch == "h" and not M.has_h:
+            raise ValueError(f"mode '{mode}': the model has no massive neutrinos")
+        mask |= bits[ch]
+    if mask == 0:
+        raise ValueError("empty matter power spectrum mode")
+    return mask
+
+
+def spectrum_matter_modes(modes, prob, ks, taus=None, ktini=1e-2, tauinimax=1e-4, bgopts=None, ptopts=None,
+                          k_unit=None):
+    """spectrum_matter(modes, prob, k[, τ]) (fourier.jl:79-101): P(k, τ) = P0(k) Δ_S(τ)² for each
+    mode S ("m", "c", "b", "h", "cb", "cbh", ...).  With taus, the states are saved at those times in
+    the same launch and the result is [n_modes][n_τ][n_k]; without, it is [n_modes][n_k] at τ0."""
+    ks = _check_ks(ks)
+    if k_unit == "1/Mpc":
+        ks = C.k_dimensionless(ks, prob.P[P_H])
+    elif k_unit is not None:
+        raise ValueError("k_unit must be None (H0/c) or '1/Mpc'")
+    masks = [species_mask(prob.M, m) for m in modes]
+    bgopts = dict(reltol=1e-7, abstol=1e-7) | (bgopts or {})
+    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
+    bg = solvebg(prob, bgopts["reltol"], bgopts["abstol"])
+    gm = prob.gpu
+    gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
+    t0 = np.clip(np.minimum(ktini / ks, tauinimax), bg.t[0], bg.t[-1])
+    saveat = None if taus is None else np.asarray(taus, dtype=np.float64)
+    if saveat is not None and (saveat.min() < bg.t[0] or saveat.max() > bg.t[-1] or np.any(np.diff(saveat) < 0)):
+        raise ValueError("τs must be ascending and within the background timespan")
+    from .solver import SAVE_STATES
+    plan = gm.plan(np.zeros(len(ks), dtype=np.int32), ks, t0, rtol=ptopts["reltol"], atol=ptopts["abstol"],
+                   maxiters=ptopts["maxiters"], saveat=saveat, save=SAVE_STATES if saveat is not None else 0)
+    try:
+        plan.run()
+        res = plan.fetch()
+        for i, r in enumerate(res["retcode"]):
+            if r != 0:
+                warnings.warn(f"Perturbation (mode k = {ks[i]}) solution failed with return code {RETCODES[int(r)]}.")
+        P = plan.pk_modes(masks, at_saves=saveat is not None)
+    finally:
+        plan.close()
+    return P if saveat is not None else P[:, 0, :]
+
+
 def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=1e-4, lanes=0, timing=None):
     """Many parameter sets × the same ks in ONE persistent launch (the batched form of
     parameter_updater + solve, src/solve.jl:272-307 driven in a loop at test/runtests.jl:508-532).

@@ -182,6 +182,18 @@ class Plan:
                    "sbz_plan_fetch")
         return dict(u=u, usave=us, retcode=ret, stats=st)
 
+    def pk_modes(self, masks, at_saves=False):
+        """P(k, τ) for species masks: [n_masks][n_t][n_inst] (n_t = n_save at the saves, else 1)."""
+        masks = np.ascontiguousarray(masks, dtype=np.int32)
+        n_t = self.n_save if at_saves else 1
+        out = np.zeros((len(masks), n_t, self.n_inst))
+        L = self.gm.L
+        L.sbz_plan_pk_modes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
+        _lib.check(L.sbz_plan_pk_modes(self.h, _lib.ptr(masks, ctypes.c_int), len(masks), int(bool(at_saves)),
+                                       _lib.ptr(out)), "sbz_plan_pk_modes")
+        return out
+
     def sources(self):
         out = np.zeros((self.n_inst, self.n_save, 3))
         _lib.check(self.gm.L.sbz_plan_fetch_sources(self.h, _lib.ptr(out)), "sbz_plan_fetch_sources")

@@ -307,3 +307,36 @@ def test_high_lmax_32_matches_oracle():
     assert rel_state_err(sol.pts["u"], o["u"]) < 1e-4
     Po = np.array([S.O.p0(k) * S.O.delta_m(k, S.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks)])
     assert np.max(np.abs(sol.pts["pk"] / Po - 1)) < 1e-4
+
+
+def test_spectrum_matter_modes_and_times(fixture5):
+    # spectrum_matter(modes, prob, k, τ) (test/runtests.jl:609-622): shapes, "m" equal to the total
+    # matter spectrum, "cbh" equal to "m", and the CDM and baryon spectra at τ = 1.5, 3.0 against
+    # Δ_s = δ_s + 3ℋθ_s/k² from the oracle's saved states
+    from symboltz.problem import spectrum_matter_modes
+    say("case: P(k, τ) for modes m, c, b, cb, cbh, h at 2 times, 4 k")
+    S = fixture5
+    prob = CosmologyProblem(S.M, S.pars)
+    ks = np.array([1e-4, 1e-3, 1e-2, 1e-1])
+    taus = np.array([1.5, 3.0])
+    modes = ["m", "c", "b", "cb", "cbh", "h"]
+    P = spectrum_matter_modes(modes, prob, ks, taus, k_unit="1/Mpc")
+    assert P.shape == (6, 2, 4)
+    P0 = spectrum_matter_modes(modes, prob, ks, k_unit="1/Mpc")
+    assert P0.shape == (6, 4)
+    Pm, _ = spectrum_matter(prob, ks, k_unit="1/Mpc")
+    assert np.max(np.abs(P0[0] / Pm - 1)) < 1e-12
+    assert np.array_equal(P0[4], P0[0]) and np.array_equal(P[4], P[0])
+    from symboltz.constants import k_dimensionless
+    kd = k_dimensionless(ks, S.pars["h"])
+    t0 = np.clip(np.minimum(1e-2 / kd, 1e-4), S.t[0], S.t[-1])
+    o = S.O.solve(kd, t0, saveat=taus)
+    L = S.O.layout()
+    for it, tau in enumerate(taus):
+        H = S.O.background(tau)["H"]
+        for j, col in ((1, L["C"]), (2, L["B"])):
+            D = o["usave"][:, it, col] + 3 * H * o["usave"][:, it, col + 1] / kd**2
+            Po = np.array([S.O.p0(k) for k in kd]) * D**2
+            assert np.max(np.abs(P[j, it] / Po - 1)) < 1e-4, (modes[j], tau)
+    with pytest.raises(ValueError):
+        spectrum_matter_modes(["x"], prob, ks)
