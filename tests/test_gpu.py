@@ -340,3 +340,54 @@ def test_spectrum_matter_modes_and_times(fixture5):
             assert np.max(np.abs(P[j, it] / Po - 1)) < 1e-4, (modes[j], tau)
     with pytest.raises(ValueError):
         spectrum_matter_modes(["x"], prob, ks)
+
+
+def test_power_spectrum_converged_to_0p1_percent():
+    # test/runtests.jl:624-630: P(k) at the default tolerances within 0.1 % of a tight solve
+    # (rtol = atol = 1e-10, every mode from the background start) for 100 k in [0.1, 1e4] H0/c
+    say("case: P(k) rtol 1e-5 vs 1e-10, 100 k in [0.1, 1e4]")
+    M = models.LCDM(lmax=5)
+    prob = CosmologyProblem(M, models.parameters_Planck18(M))
+    ks = 10 ** np.linspace(-1, 4, 100)
+    P0, sol0 = spectrum_matter(prob, ks, ktini=0.0, tauinimax=0.0,
+                               ptopts=dict(reltol=1e-10, abstol=1e-10, maxiters=10_000_000))
+    P, _ = spectrum_matter(prob, ks)
+    assert issuccess(sol0)
+    err = np.abs(P / P0 - 1)
+    say(f"max |P/P0 - 1| = {err.max():.2e}")
+    assert np.all(err < 1e-3)
+
+
+def test_stability_over_latin_hypercube():
+    # test/runtests.jl:505-546: every parameter varied by ±50 % around Planck18, 100 Latin-hypercube
+    # samples, k = 1, 10, 100, 1000 — all solves succeed (with and without reionization)
+    from scipy.stats import qmc
+    from symboltz.models import P_REION
+    say("case: 2 x 100 Latin-hypercube cosmologies (every parameter ±50 %), 4 k, one launch each")
+    M = models.LCDM(lmax=5)
+    base = models.parameters_Planck18(M)
+    names = list(base)
+    lo = np.array([0.5 * base[n] for n in names])
+    hi = np.array([1.5 * base[n] for n in names])
+    X = qmc.scale(qmc.LatinHypercube(d=len(names), seed=0).random(100), lo, hi)
+    sets = [dict(zip(names, x)) for x in X]
+    ks = np.array([1e0, 1e1, 1e2, 1e3])
+    pk, res, bgs = solve_sets(M, sets, ks)
+    assert np.all(res["retcode"] == 0) and np.all(np.isfinite(pk)) and np.all(pk > 0)
+    # reionization switched off
+    from symboltz.problem import solvebg_sets
+    probs = [CosmologyProblem(M, p) for p in sets]
+    for p in probs:
+        p.P[P_REION] = 0.0
+    bgs = solvebg_sets(probs)
+    gm = probs[0].gpu
+    for si, bg in enumerate(bgs):
+        gm.upload_background(si, bg.t, bg.u, bg.du, bg.P)
+    n = len(ks)
+    t0 = np.concatenate([np.clip(np.minimum(1e-2 / ks, 1e-4), bg.t[0], bg.t[-1]) for bg in bgs])
+    plan = gm.plan(np.repeat(np.arange(len(bgs), dtype=np.int32), n), np.tile(ks, len(bgs)), t0)
+    plan.run()
+    r = plan.fetch()
+    pk2 = plan.pk()
+    plan.close()
+    assert np.all(r["retcode"] == 0) and np.all(np.isfinite(pk2)) and np.all(pk2 > 0)
