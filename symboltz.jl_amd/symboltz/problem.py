@@ -136,9 +136,98 @@ def solvebg_sets(probs, reltol=1e-7, abstol=1e-7, max_knots=2048, device=None, t
     return [BackgroundSolution(P[i], *knots[i], int(nrej[i])) for i in range(len(probs))]
 
 
+BG_UNKNOWNS = {"a": 0, "XH": 1, "XHe": 2, "dT": 3, "kappa": 4}
+
+
+def _hermite(t, y, dy, x):
+    i = np.clip(np.searchsorted(t, x, side="right") - 1, 0, len(t) - 2)
+    h = t[i + 1] - t[i]
+    s = (x - t[i]) / h
+    return ((2 * s**3 - 3 * s**2 + 1) * y[i] + (s**3 - 2 * s**2 + s) * h * dy[i] + (-2 * s**3 + 3 * s**2) * y[i + 1]
+            + (s**3 - s**2) * h * dy[i + 1])
+
+
 class CosmologySolution:
-    def __init__(self, prob, bg, ks, pts):
+    """Background knots plus the perturbation modes of one `solve` call.
+
+    Calling it mirrors `sol(is, τs[, ks])` (src/solve.jl:720-787):
+    - background unknowns by name ("a", "XH", "XHe", "dT", "kappa") read from the cubic Hermite
+      spline of the knots;
+    - perturbation unknowns by index into the state vector at (τ, k), linear in log k between the
+      two neighbouring solved modes.
+
+    The modes' dense output at arbitrary τ comes from the GPU: the needed modes are integrated again
+    with `saveat = τs`. The integration is deterministic, so they take the same steps as in the
+    original solve and return its cubic Hermite dense output."""
+
+    def __init__(self, prob, bg, ks, pts, t0=None, ptopts=None):
         self.prob, self.bg, self.ks, self.pts = prob, bg, ks, pts
+        self._t0, self._ptopts = t0, ptopts
+
+    def _modes_at(self, modes, taus):
+        gm = self.prob.gpu
+        gm.upload_background(0, self.bg.t, self.bg.u, self.bg.du, self.bg.P)
+        o = self._ptopts
+        order = np.argsort(taus, kind="stable")             # saveat must ascend
+        plan = gm.plan(np.zeros(len(modes), dtype=np.int32), self.ks[modes], self._t0[modes], rtol=o["reltol"],
+                       atol=o["abstol"], maxiters=o["maxiters"], saveat=taus[order])
+        try:
+            plan.run()
+            u = plan.fetch(save=True)["usave"]              # [n_modes][n_τ][n]
+        finally:
+            plan.close()
+        out = np.empty_like(u)
+        out[:, order] = u
+        return out
+
+    def __call__(self, idxs, taus, ks=None, ktransform=np.log):
+        scalar_i = np.isscalar(idxs) or isinstance(idxs, str)
+        scalar_t = np.isscalar(taus)
+        scalar_k = ks is not None and np.isscalar(ks)
+        idxs = [idxs] if scalar_i else list(idxs)
+        taus = np.atleast_1d(np.asarray(taus, dtype=np.float64))
+        if ks is None:
+            if not all(isinstance(i, str) for i in idxs):
+                raise ValueError("perturbation unknowns need wavenumbers: sol(is, τs, ks)")
+            out = np.stack([_hermite(self.bg.t, self.bg.u[:, BG_UNKNOWNS[i]], self.bg.du[:, BG_UNKNOWNS[i]], taus)
+                            for i in idxs])
+        else:
+            if self.ks is None or len(self.ks) == 0:
+                raise ValueError("No perturbations solved for. Pass ks to solve().")
+            if np.any(np.diff(self.ks) < 0):
+                raise ValueError("Solution wavenumbers are not sorted in ascending order")
+            ks = np.atleast_1d(np.asarray(ks, dtype=np.float64))
+            kmin, kmax = self.ks[0], self.ks[-1]
+            if ks.min() < kmin:
+                raise ValueError(f"Requested wavenumber k = {ks.min()} is below the minimum solved wavenumber {kmin}")
+            if ks.max() > kmax:
+                raise ValueError(f"Requested wavenumber k = {ks.max()} is above the maximum solved wavenumber {kmax}")
+            if not all(isinstance(i, (int, np.integer)) for i in idxs):
+                raise ValueError("perturbation unknowns are given by their index in the state vector")
+            i2 = np.searchsorted(self.ks, ks, side="left")
+            i1 = np.maximum(i2 - 1, 0)
+            at = ks == self.ks[np.minimum(i2, len(self.ks) - 1)]
+            i1 = np.where(at, i2, i1)                     # exactly on a solved mode
+            modes = np.unique(np.concatenate([i1, i2]))
+            U = self._modes_at(modes, taus)[:, :, idxs]   # [n_modes][n_τ][n_is]
+            pos = {m: j for j, m in enumerate(modes)}
+            out = np.zeros((len(idxs), len(taus), len(ks)))
+            for ik, k in enumerate(ks):
+                v1, v2 = U[pos[i1[ik]]], U[pos[i2[ik]]]
+                if i1[ik] == i2[ik]:
+                    v = v1
+                else:
+                    w = (ktransform(k) - ktransform(self.ks[i1[ik]])) / (ktransform(self.ks[i2[ik]]) -
+                                                                        ktransform(self.ks[i1[ik]]))
+                    v = v1 + (v2 - v1) * w
+                out[:, :, ik] = v.T
+        if ks is not None and scalar_k:
+            out = out[:, :, 0]
+        if scalar_t:
+            out = out[:, 0] if out.ndim == 2 else out[:, 0, ...]
+        if scalar_i:
+            out = out[0]
+        return out
 
     def __repr__(self):
         s = f"Cosmology solution for model {self.prob.M.name}\n  Background: return code Success; {len(self.bg.t)} points"
@@ -195,7 +284,7 @@ def solve(prob, ks=None, bgopts=None, ptopts=None, ptivini=-math.inf, saveat=Non
             warnings.warn(f"Perturbation (mode k = {ks[i]}) solution failed with return code {RETCODES[int(r)]}.")
         elif verbose:
             print(f"\rSolved perturbations for wavenumber k = {ks[i]}", end="")
-    return CosmologySolution(prob, bg, ks, res)
+    return CosmologySolution(prob, bg, ks, res, t0=t0, ptopts=ptopts)
 
 
 def issuccess(sol):

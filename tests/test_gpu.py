@@ -391,3 +391,41 @@ def test_stability_over_latin_hypercube():
     pk2 = plan.pk()
     plan.close()
     assert np.all(r["retcode"] == 0) and np.all(np.isfinite(pk2)) and np.all(pk2 > 0)
+
+
+def test_solution_interpolation_and_accessing(fixture5):
+    # sol(is, τs, ks) (src/solve.jl:720-787; test/runtests.jl:35-66,89-96): output shapes follow the
+    # arguments, wavenumbers outside the solved range raise, a solved wavenumber returns its own
+    # mode, values between modes are linear in log k, and the background reads the knot spline
+    S = fixture5
+    say("case: solution interpolation, 2 solved modes + 20 modes in [1e-2, 1e1]")
+    M = S.M
+    prob = CosmologyProblem(M, S.pars)
+    ks = np.array([1e2, 1e3])
+    taus = np.array([1.0, 2.0, 3.0])
+    sol = solve(prob, ks)
+    iPhi = 0
+    ii = [iPhi, iPhi, 1, 2]
+    assert np.ndim(sol(iPhi, taus[0], ks[0])) == 0
+    assert np.shape(sol(ii, taus[0], ks[0])) == (4,)
+    assert np.shape(sol(iPhi, taus, ks[0])) == (3,)
+    assert np.shape(sol(ii, taus, ks[0])) == (4, 3)
+    assert np.shape(sol(iPhi, taus[0], ks)) == (2,)
+    assert np.shape(sol(ii, taus[0], ks)) == (4, 2)
+    assert np.shape(sol(iPhi, taus, ks)) == (3, 2)
+    assert np.shape(sol(ii, taus, ks)) == (4, 3, 2)
+    with pytest.raises(ValueError, match="below the minimum solved wavenumber"):
+        sol(ii, taus, ks[0] - 1)
+    with pytest.raises(ValueError, match="above the maximum solved wavenumber"):
+        sol(ii, taus, ks[-1] + 1)
+    assert np.shape(sol("a", taus)) == (3,) and abs(sol("a", S.t[-1]) - 1.0) < 1e-12
+    with pytest.raises(ValueError, match="No perturbations"):
+        solve(prob)(ii, taus, ks)
+    # values: the states at τ0 equal the solve's final states; between modes, log-k linear
+    ks2 = 10 ** np.linspace(-2, 1, 20)
+    sol2 = solve(prob, ks2)
+    u0 = sol2(list(range(6)), S.t[-1], ks2)           # [6][n_k]
+    assert np.max(np.abs(u0.T - sol2.pts["u"][:, :6])) <= 1e-12 * np.abs(sol2.pts["u"][:, :6]).max()
+    km = np.sqrt(ks2[3] * ks2[4])
+    v = sol2(list(range(6)), S.t[-1], km)
+    assert np.allclose(v, 0.5 * (sol2.pts["u"][3, :6] + sol2.pts["u"][4, :6]), rtol=1e-12, atol=0)
