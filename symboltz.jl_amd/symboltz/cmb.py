@@ -295,18 +295,27 @@ def source_grid_adaptive(prob, Ss, taus, ks, bg=None, ptopts=None, refine=True, 
     return K, S
 
 
+_TEMPERATURE_UNITS = {"K": 1.0, "mK": 1e3, "uK": 1e6, "μK": 1e6}
+
+
 def spectrum_cmb(modes, prob, jl, ls=None, normalization="Cl", kinterp=None, dk_tau0=math.pi, xs=None, taucut=1e-2,
-                 l_limber=10, bgopts=None, ptopts=None, timing=None, details=None, bg=None):
+                 l_limber=10, bgopts=None, ptopts=None, timing=None, details=None, bg=None, unit=None):
     """Angular power spectra C_l^{AB} for `modes` such as ["TT", "EE", "TE", "ψψ"] at jl.l (or, with
     `ls`, splined to every l in `ls`, whose extrema must equal those of jl.l), as
-    `spectrum_cmb(modes, prob, jl[, ls]; normalization)` (angular.jl:260-359).  Spectra are of
-    dimensionless temperature fluctuations; normalization "Dl" returns l(l+1) C_l / 2π.
+    `spectrum_cmb(modes, prob, jl[, ls]; normalization, unit)` (angular.jl:260-359).  Spectra are
+    of dimensionless temperature fluctuations, or in unit² for unit "K", "mK" or "uK" (the factor
+    (T0 in that unit)², angular.jl:318-324); normalization "Dl" returns l(l+1) C_l / 2π.
     Returns [n_l][n_modes].  `bg`: a background solution to use instead of solving one."""
+    if unit is not None and unit not in _TEMPERATURE_UNITS:
+        raise ValueError(f"Requested unit {unit} is not a temperature unit")
     dets = [] if details is not None else None
     out = spectrum_cmb_sets(modes, [prob], jl, ls, normalization, kinterp, dk_tau0, xs, taucut, l_limber, bgopts,
                             ptopts, timing, dets, bgs=None if bg is None else [bg])[0]
     if details is not None:
         details.update(dets[0])
+    if unit is not None:
+        from .models import P_T0
+        out = out * (prob.P[P_T0] * _TEMPERATURE_UNITS[unit]) ** 2
     return out
 
 

@@ -219,3 +219,25 @@ def test_gpu_cmb_temperature_polarization_only_matches_oracle(fixture5):
     Do, _ = oc.spectrum_cmb(O, lambda k: spectrum_primordial(k, prob), ["TT", "TE", "EE"], ls, lensing=False)
     for i in range(3):
         assert normwise(D[:, i], Do[:, i]) < 1e-4, i
+
+
+@pytest.mark.gpu
+def test_gpu_cmb_spectra_finite_to_l3000_and_units(fixture5):
+    # test/runtests.jl:561-565: TT and EE finite for l = 20:20:3000; and the μK² unit (angular.jl:318-324)
+    from symboltz.bessel import SphericalBesselCache
+    from symboltz.cmb import spectrum_cmb
+    from symboltz.problem import solvebg
+    print("case: C_l TT, EE at l = 20:20:3000; units", flush=True)
+    S = fixture5
+    prob = CosmologyProblem(S.M, S.pars)
+    bg = solvebg(prob)
+    jl = SphericalBesselCache(np.arange(20, 3001, 20))
+    D = spectrum_cmb(["TT", "EE"], prob, jl, normalization="Dl", bg=bg)
+    assert D.shape == (150, 2) and np.all(np.isfinite(D)) and np.all(D[:, 0] > 0)
+    Du = spectrum_cmb(["TT"], prob, jl, normalization="Dl", bg=bg, unit="uK")
+    assert np.allclose(Du[:, 0], D[:, 0] * (S.pars["T0"] * 1e6) ** 2, rtol=1e-12, atol=0)
+    # the first acoustic peak of D_l^TT sits near l = 220 with a few thousand μK²
+    lpk = jl.l[np.argmax(Du[:15, 0])]
+    assert 180 <= lpk <= 260 and 3000 < Du[:, 0].max() < 8000
+    with pytest.raises(ValueError):
+        spectrum_cmb(["TT"], prob, jl, unit="m")
