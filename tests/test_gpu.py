@@ -429,3 +429,42 @@ def test_solution_interpolation_and_accessing(fixture5):
     km = np.sqrt(ks2[3] * ks2[4])
     v = sol2(list(range(6)), S.t[-1], km)
     assert np.allclose(v, 0.5 * (sol2.pts["u"][3, :6] + sol2.pts["u"][4, :6]), rtol=1e-12, atol=0)
+
+
+def test_config4_model_families_match_oracle():
+    # the mixed workload of four model families packed by state size (n = 38, 56, 100, 236): each
+    # family's sets run in one launch; Latin-hypercube parameters over h, ω_b, ω_c, ln 1e10 As,
+    # ns, and for the families that have them w0, wa and Σm_ν ∈ [0.06, 0.30] eV (3 degenerate
+    # species); P(k) against the oracle.  w(a) = w0 + wa (1 - a) is kept on one side of -1: the
+    # fluid equations divide by 1 + w (c_s²/(1+w), ẇ/(1+w), dark_energy.jl:55-57), so a set whose
+    # w crosses -1 is singular there; such a set must fail per mode without raising (below).
+    from scipy.stats import qmc
+    say("case: 4 model families x 3 LHS sets x 6 k, P(k) against the oracle")
+    families = [models.LCDM(lmax=10, h=False), models.w0waCDM(lmax=6),
+                models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4), models.LCDM(lmax=32)]
+    ks = 10 ** np.linspace(-1, 3, 6)
+    X = qmc.scale(qmc.LatinHypercube(d=8, seed=4).random(3), [0.60, 0.020, 0.10, 2.9, 0.92, -0.95, 0.0, 0.06],
+                  [0.75, 0.024, 0.14, 3.2, 1.00, -0.8, 0.3, 0.30])
+    for M in families:
+        base = models.parameters_Planck18(M)
+        sets = []
+        for h, wb, wc, la, ns, w0, wa, mnu in X:
+            p = dict(base, h=h, Omega_b0=wb / h**2, Omega_c0=wc / h**2, ln_As1e10=la, ns=ns)
+            if "w0" in base:
+                p.update(w0=w0, wa=wa)
+            if "m_eV" in base:
+                p.update(N=3, m_eV=mnu / 3) if base.get("N", 1) == 3 else p.update(m_eV=mnu)
+            sets.append(p)
+        pk, res, bgs = solve_sets(M, sets, ks)
+        assert np.all(res["retcode"] == 0), M.name
+        for si, p in enumerate(sets):
+            P, r = Setup(M, p).O.spectrum_matter(ks)
+            assert np.all(r["retcode"] == 0)
+            assert np.max(np.abs(pk[si] / P - 1)) < 1e-4, (M.name, M.n_unknowns, si)
+    # a w0wa set crossing w = -1 (w0 = -1.1, wa = 0.3: w = -0.8 early, -1.1 today)
+    M = models.w0waCDM(lmax=6)
+    cross = dict(models.parameters_Planck18(M), w0=-1.1, wa=0.3)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        pk, res, _ = solve_sets(M, [cross], ks)
+    assert np.any(res["retcode"] != 0)
