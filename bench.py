@@ -264,8 +264,8 @@ def main():
         print(json.dumps({"copy_mib": args.pmc_copy, "bytes_read_per_copy": 8 * n, "bytes_written_per_copy": 8 * n}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is described in "
-              f"DESIGN.md and not measured here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): instances are independent and would be "
+              f"sharded across GPUs by cost, which is not measured here.", file=sys.stderr)
         raise SystemExit(2)
 
     from symboltz import _lib
