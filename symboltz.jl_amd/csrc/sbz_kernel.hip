@@ -75,7 +75,12 @@ struct SbzSolveArgs {
     unsigned int* queue;       // work counter (zeroed by the host before launch)
     double* ws;                // workspace: per wavefront [SBZ_NW][64]
     int S;                     // number of lanes (multiple of 64)
+    const double* srcbg;       // [n_sets][n_save][SBZ_SRCBG_STRIDE] background at the save times
+                               // (sbz_srcbg_kernel), or null: the wave kernel then evaluates it per mode
 };
+
+// Row of the per-set source background table: B[0..NB), dB/dτ[0..NB), d²κ̇/dτ² at 2 NB.
+#define SBZ_SRCBG_STRIDE 80
 
 // workspace slots (in units of SBZ_N doubles, then the LU)
 #define WS_U0 0
@@ -192,6 +197,30 @@ SBZ_DEV int sbz_find_knot(const double* kt, int nk, double t) {
         if (kt[mid] <= t) lo = mid; else hi = mid;
     }
     return lo;
+}
+
+// Row g = set * n_save + i of the source background table: the Taylor-2 background at save time i
+// of one parameter set.  It depends on the set and τ only, so one evaluation serves every mode of
+// the set.  The knot search in sbz_pt_background ends on the same interval from any start, so the
+// values are those the wave kernel would compute at the save.
+static_assert(2 * SBZ_NB + 1 <= SBZ_SRCBG_STRIDE, "source background row too short");
+SBZ_DEV void sbz_srcbg_row(const double* params, const double* knots, const long long* knot_off, const int* knot_n,
+                           const double* saveat, int saveat_stride, int n_save, long long g, double* out) {
+    const int set = (int)(g / n_save), i = (int)(g % n_save);
+    const double* P = params + (size_t)set * SBZ_NPAR;
+    const double* kt = knots + knot_off[set];
+    const int nk = knot_n[set];
+    const double tau = saveat[(size_t)set * saveat_stride + i];
+    int kk = sbz_find_knot(kt, nk, tau);
+    double B[SBZ_NB], dB[SBZ_NB];
+    const double dd = sbz_bg_tay2(P, kt, nk, kk, tau, B, dB);
+    double* o = out + (size_t)g * SBZ_SRCBG_STRIDE;
+#pragma unroll
+    for (int q = 0; q < SBZ_NB; ++q) {
+        o[q] = B[q];
+        o[SBZ_NB + q] = dB[q];
+    }
+    o[2 * SBZ_NB] = dd;
 }
 
 SBZ_DEV void sbz_consts(const double* P, double* Cs) {
@@ -773,7 +802,15 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                     if (osave) osave[(size_t)isave * SBZ_N + i] = y;
                 }
                 if (osrc && ret == SBZ_RET_SUCCESS) {
-                    if (c.lane == 0) {
+                    if (a.srcbg) {   // the set's table row (sbz_srcbg_kernel), spread over the lanes
+                        const double* sb = a.srcbg + ((size_t)set * a.n_save + isave) * SBZ_SRCBG_STRIDE;
+                        for (int q = c.lane; q <= 2 * SBZ_NB; q += 64) {
+                            const double v = sb[q];
+                            if (q < SBZ_NB) L.B[6][q] = v;
+                            else if (q < 2 * SBZ_NB) L.dB[1][q - SBZ_NB] = v;
+                            else L.ddkd = v;
+                        }
+                    } else if (c.lane == 0) {
                         double Bt[SBZ_NB], dBt[SBZ_NB];
                         int kk = kidx;
                         L.ddkd = sbz_bg_tay2(P, kt, nk, kk, saveat[isave], Bt, dBt);
@@ -989,6 +1026,18 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
 // SIMDs).  SBZ_ALGO_WAVE / SBZ_ALGO_WAVE2 select them.
 extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body(a); }
 extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body(a); }
+
+// Source background table for the wave kernels: one thread per (set, save time), rows as
+// sbz_srcbg_row.  Launched before the solve when the plan saves CMB sources.
+extern "C" __global__ void __launch_bounds__(64) sbz_srcbg_kernel(const double* __restrict__ params,
+                                                                  const double* __restrict__ knots,
+                                                                  const long long* __restrict__ knot_off,
+                                                                  const int* __restrict__ knot_n,
+                                                                  const double* __restrict__ saveat, int saveat_stride,
+                                                                  int n_save, long long n_rows, double* __restrict__ out) {
+    const long long g = (long long)blockIdx.x * 64 + threadIdx.x;
+    if (g < n_rows) sbz_srcbg_row(params, knots, knot_off, knot_n, saveat, saveat_stride, n_save, g, out);
+}
 
 // P(k, τ0) = P0(k) Δ_m(τ0)² for each instance from its final state (fourier.jl:14-52,90-97).
 extern "C" __global__ void sbz_pk_kernel(const double* __restrict__ params, const double* __restrict__ knots,

@@ -36,6 +36,7 @@ struct sbz_model {
     int n = 0, nnz_lu = 0, nw = 0;
     hipModule_t module = nullptr;
     hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
+    hipFunction_t k_srcbg = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -65,7 +66,11 @@ struct SbzSolveArgsHost {
     unsigned int* queue;
     double* ws;
     int S;
+    const double* srcbg;
 };
+
+// Row stride of the source background table (SBZ_SRCBG_STRIDE in sbz_kernel.hip).
+static constexpr int kSrcBgStride = 80;
 
 struct sbz_plan {
     sbz_model* m = nullptr;
@@ -75,6 +80,7 @@ struct sbz_plan {
     // device buffers
     double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
     double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_ws = nullptr, *d_pk = nullptr;
+    double* d_srcbg = nullptr;   // [n_sets][n_save] background rows for the wave kernels' sources
     int save_flags = 0;
     int algo = SBZ_ALGO_WAVE;
     long long *d_koff = nullptr, *d_stats = nullptr;
@@ -173,6 +179,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         hipModuleGetFunction(&m->k_wave, m->module, "sbz_wave_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_wave2, m->module, "sbz_wave_kernel_o2") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk_modes, m->module, "sbz_pk_modes_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_srcbg, m->module, "sbz_srcbg_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk, m->module, "sbz_pk_kernel") != hipSuccess) {
         sbz_set_error("code object lacks the solver kernels");
         (void)hipModuleUnload(m->module);
@@ -224,7 +231,8 @@ extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const doubl
 
 static void plan_free(sbz_plan* p) {
     void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src, p->d_ws,
-                    p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue};
+                    p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue,
+                    p->d_srcbg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -347,6 +355,8 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
         e = hipMalloc((void**)&p->d_save, sizeof(double) * (size_t)n_inst * n_save * m->n);
     if (e == hipSuccess && (p->save_flags & SBZ_SAVE_SOURCES))
         e = hipMalloc((void**)&p->d_src, sizeof(double) * (size_t)n_inst * n_save * 3);
+    if (e == hipSuccess && (p->save_flags & SBZ_SAVE_SOURCES) && p->algo != SBZ_ALGO_LANE)
+        e = hipMalloc((void**)&p->d_srcbg, sizeof(double) * kSrcBgStride * (size_t)n_sets * n_save);
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_ret, sizeof(int) * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, sizeof(long long) * 6 * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_pk, sizeof(double) * std::max(n_inst, 1));
@@ -398,6 +408,21 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     a.queue = p->d_queue;
     a.ws = p->d_ws;
     a.S = p->lanes;
+    a.srcbg = p->d_srcbg;
+    if (p->d_srcbg) {
+        // the background at every save time of every set, shared by all modes of the set
+        const double* params = p->d_params;
+        const double* knots = p->d_knots;
+        const long long* koff = p->d_koff;
+        const int* kn = p->d_knotn;
+        const double* sv = p->d_saveat;
+        int stride = p->saveat_stride, n_save = p->n_save;
+        long long n_rows = (long long)p->n_sets * p->n_save;
+        double* out = p->d_srcbg;
+        void* bargs[] = {&params, &knots, &koff, &kn, &sv, &stride, &n_save, &n_rows, &out};
+        HIPCHK(hipModuleLaunchKernel(m->k_srcbg, (unsigned)((n_rows + 63) / 64), 1, 1, 64, 1, 1, 0, m->stream, bargs,
+                                     nullptr));
+    }
     void* args[] = {&a};
     hipFunction_t kf = (p->algo == SBZ_ALGO_LANE) ? m->k_solve : (p->algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
     unsigned grid = (p->algo == SBZ_ALGO_LANE) ? (unsigned)(p->lanes / p->block) : (unsigned)p->lanes;

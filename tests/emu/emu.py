@@ -48,7 +48,10 @@ class WaveEmu:
                                           _ip, _llp]
         self.n = self.L.emu_n()
 
-    def solve(self, P, t, u, du, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, sources=False):
+    def solve(self, P, t, u, du, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, sources=False,
+              srcbg=False):
+        """srcbg: give the kernel the per-set source background table (as the runtime does for the wave
+        kernels) instead of letting it evaluate the background at every save of every mode."""
         p = lambda a, ty=ctypes.c_double: a.ctypes.data_as(ctypes.POINTER(ty))
         nk = len(t)
         knots = np.concatenate([np.asarray(t, float), np.asarray(u, float).ravel(), np.asarray(du, float).ravel()])
@@ -66,6 +69,7 @@ class WaveEmu:
         out_src = np.zeros((m, ns, 3)) if (ns and sources) else None
         ret = np.zeros(m, dtype=np.int32)
         st = np.zeros((m, 6), dtype=np.int64)
+        self.L.emu_set_srcbg(1 if srcbg else 0)
         rc = self.L.emu_wave_solve(p(P), p(knots), p(koff, ctypes.c_longlong), p(kn, ctypes.c_int),
                                    p(iset, ctypes.c_int), p(ks), p(t0), m, rtol, atol, maxiters,
                                    p(saveat) if ns else None, ns, p(out_u), p(out_save) if ns else None,

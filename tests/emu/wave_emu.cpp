@@ -157,6 +157,11 @@ static void wave_body(void* a) { sbz_wave_kernel(*(SbzSolveArgs*)a); }
 extern "C" const char* emu_error(void) { return emu::error; }
 extern "C" int emu_n(void) { return SBZ_N; }
 
+// With the flag set, emu_wave_solve hands the kernel a source background table built by
+// sbz_srcbg_row (the body of sbz_srcbg_kernel) instead of letting it evaluate the background per mode.
+static int g_srcbg = 0;
+extern "C" void emu_set_srcbg(int on) { g_srcbg = on; }
+
 // Solve n_inst instances with the wave kernel on one emulated workgroup.  The knot table of every
 // set is laid out as the runtime uploads it: t[nk], u[nk][5], du[nk][5] at knot_off[set].
 extern "C" int emu_wave_solve(const double* params, const double* knots, const long long* knot_off,
@@ -189,5 +194,15 @@ extern "C" int emu_wave_solve(const double* params, const double* knots, const l
     a.queue = &queue;
     a.ws = nullptr;
     a.S = 1;
+    std::vector<double> tab;
+    if (g_srcbg && out_src && n_save > 0) {
+        int n_sets = 1;
+        for (int i = 0; i < n_inst; ++i) n_sets = inst_set[i] + 1 > n_sets ? inst_set[i] + 1 : n_sets;
+        const long long n_rows = (long long)n_sets * n_save;
+        tab.assign((size_t)n_rows * SBZ_SRCBG_STRIDE, std::nan(""));
+        for (long long g = 0; g < n_rows; ++g)
+            sbz_srcbg_row(params, knots, knot_off, knot_n, saveat, 0, n_save, g, tab.data());
+        a.srcbg = tab.data();
+    }
     return emu::run(wave_body, &a);
 }

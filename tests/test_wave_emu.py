@@ -33,6 +33,28 @@ def test_wave_kernel_emulated_matches_oracle(fixture5):
     assert np.all(np.abs(g["stats"][:2, 0] - o["stats"][:, 0]) <= 0.05 * o["stats"][:, 0])
 
 
+def test_wave_kernel_emulated_source_background_table(fixture5):
+    """Sources read from the per-set background table (sbz_srcbg_kernel's rows) equal those of the
+    per-mode evaluation; the states are untouched.  The save grid starts on a background knot and
+    ends at τ0, the two places where the knot search could pick another interval."""
+    S = fixture5
+    E = WaveEmu(model_source(S.M).full_src)
+    ks = np.array([0.05, 3.0, 60.0])
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    i0 = int(np.searchsorted(S.t, 1.0))
+    saveat = np.concatenate([[S.t[i0]], np.linspace(S.t[i0], S.t[-1], 40)[1:]])
+    assert saveat[-1] == S.t[-1]
+    g0 = E.solve(S.P, S.t, S.u, S.du, ks, t0, saveat=saveat, sources=True)
+    g1 = E.solve(S.P, S.t, S.u, S.du, ks, t0, saveat=saveat, sources=True, srcbg=True)
+    assert np.all(g0["retcode"] == 0) and np.all(g1["retcode"] == 0)
+    np.testing.assert_array_equal(g1["u"], g0["u"])
+    np.testing.assert_array_equal(g1["usave"], g0["usave"])
+    assert np.all(np.isfinite(g1["src"]))
+    for c in range(3):
+        sc = np.abs(g0["src"][..., c]).max() + 1e-300
+        assert np.max(np.abs(g1["src"][..., c] - g0["src"][..., c])) <= 1e-13 * sc
+
+
 def test_wave_kernel_emulated_massive_nu_lmax16():
     M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
     S = Setup(M, models.parameters_Planck18(M))
