@@ -621,32 +621,33 @@ struct SbzR5 {
     static constexpr double d[9] = {0, SBZ_R5_D1, SBZ_R5_D2, SBZ_R5_D3, SBZ_R5_D4, SBZ_R5_D5, 0, 0, 0};
 };
 
-// stage S input  US = u_n + Σ_{j<S} a_Sj K_j  for this lane's components
+// stage S input  US = u_n + Σ_{j<S} a_Sj K_j  for this lane's components; the same pass over the
+// stage vectors also forms this lane's  ck = Σ_{j<S} (c_Sj / h) K_j  for sbz_stage_rhs
 template <int S>
-SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, double* US) {
+SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, double* US, double idt, double* ck) {
 #pragma unroll
     for (int r = 0; r < SBZ_OWN; ++r) {
         const int i = lane + 64 * r;
         if (i < SBZ_N) {
-            double acc = Un[i];
+            double acc = Un[i], cc = 0.0;
 #pragma unroll
-            for (int j = 1; j < S; ++j) acc += SbzR5::A[S][j - 1] * v[5 + j][i];
+            for (int j = 1; j < S; ++j) {
+                const double kj = v[5 + j][i];
+                acc += SbzR5::A[S][j - 1] * kj;
+                cc += (SbzR5::C[S][j - 1] * idt) * kj;
+            }
             US[i] = acc;
+            ck[r] = cc;
         }
     }
 }
 // stage S right-hand side  K_S = f + h d_S ∂f/∂τ + Σ_{j<S} (c_Sj / h) K_j  (in place on K_S)
 template <int S>
-SBZ_DEV void sbz_stage_rhs(int lane, const double (*v)[SBZ_N], double dt, double idt, const double* DT, double* Ks) {
+SBZ_DEV void sbz_stage_rhs(int lane, double dt, const double* DT, double* Ks, const double* ck) {
 #pragma unroll
     for (int r = 0; r < SBZ_OWN; ++r) {
         const int i = lane + 64 * r;
-        if (i < SBZ_N) {
-            double acc = Ks[i] + (dt * SbzR5::d[S]) * DT[i];
-#pragma unroll
-            for (int j = 1; j < S; ++j) acc += (SbzR5::C[S][j - 1] * idt) * v[5 + j][i];
-            Ks[i] = acc;
-        }
+        if (i < SBZ_N) Ks[i] = (Ks[i] + (dt * SbzR5::d[S]) * DT[i]) + ck[r];
     }
 }
 #define SBZ_STAGE_SWITCH(F, ...)                                                                  \
@@ -910,13 +911,14 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                         for (int i = c.lane; i < SBZ_N; i += 64) Ks[i] = F0[i] + dt * SBZ_R5_D1 * DT[i];
                     } else {
                         double* in = (s <= 8) ? US : U1;
-                        if (s <= 8) SBZ_STAGE_SWITCH(sbz_stage_in, c.lane, L.v, Un, US)
+                        double ck[SBZ_OWN];   // Σ_{j<s} (c_sj / h) K_j of this lane's components
+                        if (s <= 8) SBZ_STAGE_SWITCH(sbz_stage_in, c.lane, L.v, Un, US, idt, ck)
                         __syncthreads();
                         sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
                         st_f++;
                         __syncthreads();
                         if (s == 9) break;
-                        SBZ_STAGE_SWITCH(sbz_stage_rhs, c.lane, L.v, dt, idt, DT, Ks)
+                        SBZ_STAGE_SWITCH(sbz_stage_rhs, c.lane, dt, DT, Ks, ck)
                     }
                     __syncthreads();
                     // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution
@@ -970,9 +972,13 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                 }
             }
             __syncthreads();
+            // the controller's two powers, EEst^β1 (lane 0) and qold^β2 (lane 1), in one evaluation
+            const double pw = pow((c.lane == 1) ? qold : ((EEst < __builtin_inf()) ? EEst : 1.0),
+                                  (c.lane == 1) ? SBZ_CTL_BETA2 : SBZ_CTL_BETA1);
+            const double q11 = sbz_readlane(pw, 0), qob = sbz_readlane(pw, 1);
             if (!(EEst <= 1.0)) {
-                const double q11 = (EEst < __builtin_inf()) ? pow(EEst, SBZ_CTL_BETA1) : 1.0 / SBZ_CTL_QMIN;
-                dt /= fmin(1.0 / SBZ_CTL_QMIN, q11 / SBZ_CTL_GAMMA);
+                const double q11r = (EEst < __builtin_inf()) ? q11 : 1.0 / SBZ_CTL_QMIN;
+                dt /= fmin(1.0 / SBZ_CTL_QMIN, q11r / SBZ_CTL_GAMMA);
                 st_rej++;
                 // the stage combinations read every stage vector with zero weights past the
                 // stage, so a non-finite attempt must not leave non-finite stages behind
@@ -984,10 +990,9 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                 }
                 continue;
             }
-            const double q11 = pow(EEst, SBZ_CTL_BETA1);
             const double q = (EEst == 0.0) ? 1.0 / SBZ_CTL_QMAX
                                            : fmax(1.0 / SBZ_CTL_QMAX,
-                                                  fmin(1.0 / SBZ_CTL_QMIN, q11 / pow(qold, SBZ_CTL_BETA2) / SBZ_CTL_GAMMA));
+                                                  fmin(1.0 / SBZ_CTL_QMIN, q11 / qob / SBZ_CTL_GAMMA));
             qold = fmax(EEst, SBZ_CTL_QOLDINIT);
             double tn = t + dt;
             if (tn >= t1 - 1e-14 * fabs(t1)) tn = t1;
