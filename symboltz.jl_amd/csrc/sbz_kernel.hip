@@ -621,10 +621,39 @@ struct SbzR5 {
     static constexpr double d[9] = {0, SBZ_R5_D1, SBZ_R5_D2, SBZ_R5_D3, SBZ_R5_D4, SBZ_R5_D5, 0, 0, 0};
 };
 
-// stage S input  US = u_n + Σ_{j<S} a_Sj K_j  for this lane's components; the same pass over the
-// stage vectors also forms this lane's  ck = Σ_{j<S} (c_Sj / h) K_j  for sbz_stage_rhs
+// stage S input  US = u_n + Σ_{j<S} a_Sj K_j  for this lane's components
 template <int S>
-SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, double* US, double idt, double* ck) {
+SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, double* US) {
+#pragma unroll
+    for (int r = 0; r < SBZ_OWN; ++r) {
+        const int i = lane + 64 * r;
+        if (i < SBZ_N) {
+            double acc = Un[i];
+#pragma unroll
+            for (int j = 1; j < S; ++j) acc += SbzR5::A[S][j - 1] * v[5 + j][i];
+            US[i] = acc;
+        }
+    }
+}
+// stage S right-hand side  K_S = f + h d_S ∂f/∂τ + Σ_{j<S} (c_Sj / h) K_j  (in place on K_S)
+template <int S>
+SBZ_DEV void sbz_stage_rhs(int lane, const double (*v)[SBZ_N], double dt, double idt, const double* DT, double* Ks) {
+#pragma unroll
+    for (int r = 0; r < SBZ_OWN; ++r) {
+        const int i = lane + 64 * r;
+        if (i < SBZ_N) {
+            double acc = Ks[i] + (dt * SbzR5::d[S]) * DT[i];
+#pragma unroll
+            for (int j = 1; j < S; ++j) acc += (SbzR5::C[S][j - 1] * idt) * v[5 + j][i];
+            Ks[i] = acc;
+        }
+    }
+}
+// The same two combinations with one read of each stage vector: the input pass also forms this
+// lane's  ck = Σ_{j<S} (c_Sj / h) K_j  and keeps it in registers across the RHS application (for
+// the one-wavefront-per-SIMD kernel; at two per SIMD the extra live registers cost more spills)
+template <int S>
+SBZ_DEV void sbz_stage_in_f(int lane, const double (*v)[SBZ_N], const double* Un, double* US, double idt, double* ck) {
 #pragma unroll
     for (int r = 0; r < SBZ_OWN; ++r) {
         const int i = lane + 64 * r;
@@ -641,9 +670,8 @@ SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, 
         }
     }
 }
-// stage S right-hand side  K_S = f + h d_S ∂f/∂τ + Σ_{j<S} (c_Sj / h) K_j  (in place on K_S)
 template <int S>
-SBZ_DEV void sbz_stage_rhs(int lane, double dt, const double* DT, double* Ks, const double* ck) {
+SBZ_DEV void sbz_stage_rhs_f(int lane, double dt, const double* DT, double* Ks, const double* ck) {
 #pragma unroll
     for (int r = 0; r < SBZ_OWN; ++r) {
         const int i = lane + 64 * r;
@@ -672,6 +700,7 @@ SBZ_DEV double sbz_wave_sum(double v) {
     return (sbz_readlane(v, 0) + sbz_readlane(v, 16)) + (sbz_readlane(v, 32) + sbz_readlane(v, 48));
 }
 
+template <bool FUSED>
 SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     __shared__ WaveLDS L;
     WaveCtx c;
@@ -911,14 +940,18 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                         for (int i = c.lane; i < SBZ_N; i += 64) Ks[i] = F0[i] + dt * SBZ_R5_D1 * DT[i];
                     } else {
                         double* in = (s <= 8) ? US : U1;
-                        double ck[SBZ_OWN];   // Σ_{j<s} (c_sj / h) K_j of this lane's components
-                        if (s <= 8) SBZ_STAGE_SWITCH(sbz_stage_in, c.lane, L.v, Un, US, idt, ck)
+                        double ck[SBZ_OWN];   // fused: Σ_{j<s} (c_sj / h) K_j of this lane's components
+                        if (s <= 8) {
+                            if constexpr (FUSED) SBZ_STAGE_SWITCH(sbz_stage_in_f, c.lane, L.v, Un, US, idt, ck)
+                            else SBZ_STAGE_SWITCH(sbz_stage_in, c.lane, L.v, Un, US)
+                        }
                         __syncthreads();
                         sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
                         st_f++;
                         __syncthreads();
                         if (s == 9) break;
-                        SBZ_STAGE_SWITCH(sbz_stage_rhs, c.lane, dt, DT, Ks, ck)
+                        if constexpr (FUSED) SBZ_STAGE_SWITCH(sbz_stage_rhs_f, c.lane, dt, DT, Ks, ck)
+                        else SBZ_STAGE_SWITCH(sbz_stage_rhs, c.lane, L.v, dt, idt, DT, Ks)
                     }
                     __syncthreads();
                     // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution
@@ -972,10 +1005,17 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                 }
             }
             __syncthreads();
-            // the controller's two powers, EEst^β1 (lane 0) and qold^β2 (lane 1), in one evaluation
-            const double pw = pow((c.lane == 1) ? qold : ((EEst < __builtin_inf()) ? EEst : 1.0),
-                                  (c.lane == 1) ? SBZ_CTL_BETA2 : SBZ_CTL_BETA1);
-            const double q11 = sbz_readlane(pw, 0), qob = sbz_readlane(pw, 1);
+            // the controller's two powers EEst^β1 and qold^β2: in the one-wavefront kernel as one
+            // evaluation on lanes 0 and 1; the two-per-SIMD build keeps two calls, which spill less there
+            double q11, qob = 1.0;
+            if constexpr (FUSED) {
+                const double pw = pow((c.lane == 1) ? qold : ((EEst < __builtin_inf()) ? EEst : 1.0),
+                                      (c.lane == 1) ? SBZ_CTL_BETA2 : SBZ_CTL_BETA1);
+                q11 = sbz_readlane(pw, 0);
+                qob = sbz_readlane(pw, 1);
+            } else {
+                q11 = (EEst < __builtin_inf()) ? pow(EEst, SBZ_CTL_BETA1) : 1.0;
+            }
             if (!(EEst <= 1.0)) {
                 const double q11r = (EEst < __builtin_inf()) ? q11 : 1.0 / SBZ_CTL_QMIN;
                 dt /= fmin(1.0 / SBZ_CTL_QMIN, q11r / SBZ_CTL_GAMMA);
@@ -992,7 +1032,8 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
             }
             const double q = (EEst == 0.0) ? 1.0 / SBZ_CTL_QMAX
                                            : fmax(1.0 / SBZ_CTL_QMAX,
-                                                  fmin(1.0 / SBZ_CTL_QMIN, q11 / qob / SBZ_CTL_GAMMA));
+                                                  fmin(1.0 / SBZ_CTL_QMIN,
+                                                       q11 / (FUSED ? qob : pow(qold, SBZ_CTL_BETA2)) / SBZ_CTL_GAMMA));
             qold = fmax(EEst, SBZ_CTL_QOLDINIT);
             double tn = t + dt;
             if (tn >= t1 - 1e-14 * fabs(t1)) tn = t1;
@@ -1029,8 +1070,8 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
 // per lane, fewest spills: lowest latency per mode) and for two (half the registers, a few more
 // spills: the second wavefront hides the first's latency when there are more instances than
 // SIMDs).  SBZ_ALGO_WAVE / SBZ_ALGO_WAVE2 select them.
-extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body(a); }
-extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body(a); }
+extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body<true>(a); }
+extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body<false>(a); }
 
 // Source background table for the wave kernels: one thread per (set, save time), rows as
 // sbz_srcbg_row.  Launched before the solve when the plan saves CMB sources.

@@ -152,7 +152,13 @@ static inline double __longlong_as_double(long long x) {
 
 #include "emu_kernel.inc"
 
-static void wave_body(void* a) { sbz_wave_kernel(*(SbzSolveArgs*)a); }
+// 0: sbz_wave_kernel (fused stage combinations), 1: sbz_wave_kernel_o2 (two-pass combinations)
+static int g_o2 = 0;
+extern "C" void emu_set_o2(int on) { g_o2 = on; }
+static void wave_body(void* a) {
+    if (g_o2) sbz_wave_kernel_o2(*(SbzSolveArgs*)a);
+    else sbz_wave_kernel(*(SbzSolveArgs*)a);
+}
 
 extern "C" const char* emu_error(void) { return emu::error; }
 extern "C" int emu_n(void) { return SBZ_N; }

@@ -55,6 +55,21 @@ def test_wave_kernel_emulated_source_background_table(fixture5):
         assert np.max(np.abs(g1["src"][..., c] - g0["src"][..., c])) <= 1e-13 * sc
 
 
+def test_wave_kernel_emulated_both_occupancy_builds(fixture5):
+    """sbz_wave_kernel forms the stage combinations in one pass over the stage vectors,
+    sbz_wave_kernel_o2 in two; both match the oracle, and each other to rounding."""
+    S = fixture5
+    E = WaveEmu(model_source(S.M).full_src)
+    ks = np.array([0.02, 2.0, 90.0])
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    g1 = E.solve(S.P, S.t, S.u, S.du, ks, t0)
+    g2 = E.solve(S.P, S.t, S.u, S.du, ks, t0, o2=True)
+    o = S.O.solve(ks, t0)
+    assert np.all(g1["retcode"] == 0) and np.all(g2["retcode"] == 0)
+    assert rel(g1["u"], o["u"]) < 1e-4 and rel(g2["u"], o["u"]) < 1e-4
+    assert rel(g1["u"], g2["u"]) < 1e-8
+
+
 def test_wave_kernel_emulated_massive_nu_lmax16():
     M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
     S = Setup(M, models.parameters_Planck18(M))
