@@ -126,6 +126,14 @@ int sbz_plan_create(sbz_model* m, const int* inst_set, const double* inst_k, con
 /* Launch the persistent solver kernel and wait; *kernel_ms receives its duration (HIP events on
  * the model's stream). */
 int sbz_plan_run(sbz_plan* p, float* kernel_ms);
+/* The same run as one HIP graph launch: on the first call the run's kernels (NaN fill of the
+ * save/source outputs, source background table, queue reset, solve, and the P(k) reduction of
+ * sbz_plan_pk) are captured from the model's stream into a graph, which every call then replays;
+ * *graph_ms receives the duration of the whole graph.  sbz_plan_pk after it copies the P(k) the
+ * graph computed.  Replaces the same per-mode loop of solvept (src/solve.jl:543-569) as
+ * sbz_plan_run; the graph form has no reference counterpart (it removes the per-launch host
+ * overhead of repeated runs of one plan). */
+int sbz_plan_run_graph(sbz_plan* p, float* graph_ms);
 /* Copy results to host: final states [n_inst][n], saved states [n_inst][n_save][n] (may be
  * null), per-mode return codes, and stats [n_inst][6] = accepted steps, rejected steps, f evals,
  * J evals, LU factorizations, linear solves (DEStats, src/solve.jl:885-891). */

@@ -50,6 +50,7 @@ def lib():
         L.ora_delta_m.argtypes = [_ip, _dp] + sp + [ctypes.c_double, ctypes.c_double, _dp]
         L.ora_delta_m.restype = ctypes.c_double
         L.ora_sources.argtypes = [_ip, _dp] + sp + [ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.ora_visibility.argtypes = [_dp] + sp + [ctypes.c_double, _dp]
         L.ora_p0.argtypes = [_dp, ctypes.c_double]
         L.ora_p0.restype = ctypes.c_double
         L.ora_solve.argtypes = [_ip, _dp] + sp + [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -129,6 +130,12 @@ class Oracle:
         y = np.ascontiguousarray(y, dtype=np.float64)
         out = np.zeros(3)
         self.L.ora_sources(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), k, tau, _p(y), _p(out))
+        return out
+
+    def visibility(self, tau):
+        """(v, v', v'', e^{-κ}) at τ, as the sources use them."""
+        out = np.zeros(4)
+        self.L.ora_visibility(_p(self.P), *self._sp(), tau, _p(out))
         return out
 
     def p0(self, k):

@@ -492,6 +492,19 @@ U psi_of(const Model& m, const double* P, const Bg<T>& b, double k, const U* u) 
     return u[m.iPhi] - 12.0 * PI * b.a * b.a * Pi / (k * k);
 }
 
+// Visibility v = D(e^{-κ}) = -κ̇ e^{-κ} (baryons.jl:179) and its first two τ-derivatives
+// v' = (-κ̈ + κ̇²) e^{-κ}, v'' = (-κ⃛ + 3 κ̇ κ̈ - κ̇³) e^{-κ}, with κ̇ and its derivatives from
+// second-order Taylor numbers through the background; out = (v, v', v'', e^{-κ}).
+void visibility(const double* P, const Spline& s, double tau, double* out) {
+    Bg<T2> bt;
+    background_at<T2>(P, s, T2(tau, 1.0, 0.0), bt);
+    double kd = bt.kd.v, kd1 = bt.kd.d, kd2 = bt.kd.dd, eK = std::exp(-bt.kappa.v);
+    out[0] = -kd * eK;
+    out[1] = (-kd1 + kd * kd) * eK;
+    out[2] = (-kd2 + 3 * kd * kd1 - kd * kd * kd) * eK;
+    out[3] = eK;
+}
+
 // CMB source functions at (τ, u) (cosmologies.jl:99-105):
 //   S_T = v(δγ/4 + Ψ + Πγ/16) + e^{-κ}(Ψ' + Φ') + (v θ_b)'/k² + 3/(16k²)(v Πγ)'',
 //   S_E = 3 v Πγ / (16 (kχ)²),   S_ψ = -(Ψ+Φ)(τ-τrec)/(τ0-τrec)/(τ0-τ) for τ ≥ τrec.
@@ -503,10 +516,9 @@ void sources(const Model& m, const double* P, const Spline& s, double k, double 
     f_eval(m, P, s, k, tau, f.data(), g.data());
     dfdt(m, P, s, k, tau, u, tmp.data());
     for (int i = 0; i < n; ++i) g[i] += tmp[i];
-    Bg<T2> bt;
-    background_at<T2>(P, s, T2(tau, 1.0, 0.0), bt);
-    double kd = bt.kd.v, kd1 = bt.kd.d, kd2 = bt.kd.dd, eK = std::exp(-bt.kappa.v);
-    double v = -kd * eK, vd = (-kd1 + kd * kd) * eK, vdd = (-kd2 + 3 * kd * kd1 - kd * kd * kd) * eK;
+    double vis[4];
+    visibility(P, s, tau, vis);
+    const double v = vis[0], vd = vis[1], vdd = vis[2], eK = vis[3];
     Bg<double> b;
     background_at<double>(P, s, tau, b);
     Bg<D> bd;
@@ -803,6 +815,13 @@ double ora_delta_m(const int* md, const double* P, const double* kt, const doubl
 }
 
 double ora_p0(const double* P, double k) { return p0(P, k); }
+
+// (v, v', v'', e^{-κ}) at τ as the sources use them.
+void ora_visibility(const double* P, const double* kt, const double* ku, const double* kdu, int nk, double tau,
+                    double* out) {
+    Spline s{kt, ku, kdu, nk};
+    visibility(P, s, tau, out);
+}
 
 void ora_sources(const int* md, const double* P, const double* kt, const double* ku, const double* kdu, int nk,
                  double k, double tau, const double* u, double* S) {

@@ -1082,7 +1082,23 @@ extern "C" __global__ void __launch_bounds__(64) sbz_srcbg_kernel(const double* 
                                                                   const double* __restrict__ saveat, int saveat_stride,
                                                                   int n_save, long long n_rows, double* __restrict__ out) {
     const long long g = (long long)blockIdx.x * 64 + threadIdx.x;
-    if (g < n_rows) sbz_srcbg_row(params, knots, knot_off, knot_n, saveat, saveat_stride, n_save, g, out);
+    if (g >= n_rows) return;
+    // a set index below n_sets that never received a background table has no knots: its rows are
+    // never read (instances refer to valid sets only) and are skipped instead of reading past them
+    if (knot_n[g / n_save] < 2) return;
+    sbz_srcbg_row(params, knots, knot_off, knot_n, saveat, saveat_stride, n_save, g, out);
+}
+
+// Plan outputs start as NaN at every run: a save time that a mode never reached (it failed first)
+// then reads NaN instead of stale memory.  A kernel rather than a memset, so that a run can be
+// captured in a HIP graph.
+extern "C" __global__ void __launch_bounds__(256) sbz_fill_kernel(double* __restrict__ p, long long n, double v) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = v;
+}
+
+// Zeroes the work-queue counter before a solve launch (a kernel node, not a memset node, in graphs).
+extern "C" __global__ void sbz_queue_reset_kernel(unsigned int* __restrict__ q) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0u;
 }
 
 // P(k, τ0) = P0(k) Δ_m(τ0)² for each instance from its final state (fourier.jl:14-52,90-97).

@@ -36,7 +36,7 @@ struct sbz_model {
     int n = 0, nnz_lu = 0, nw = 0;
     hipModule_t module = nullptr;
     hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
-    hipFunction_t k_srcbg = nullptr;
+    hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -88,6 +88,11 @@ struct sbz_plan {
     unsigned int* d_queue = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
+    // sbz_plan_run_graph: the run's launches (output fill, source background, queue reset, solve,
+    // P(k)) captured once into a graph and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool pk_ready = false;   // d_pk holds the P(k) of the last run (graph runs compute it)
 };
 
 static bool read_file(const char* path, std::vector<char>& out) {
@@ -180,6 +185,8 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         hipModuleGetFunction(&m->k_wave2, m->module, "sbz_wave_kernel_o2") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk_modes, m->module, "sbz_pk_modes_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_srcbg, m->module, "sbz_srcbg_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_fill, m->module, "sbz_fill_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_qreset, m->module, "sbz_queue_reset_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk, m->module, "sbz_pk_kernel") != hipSuccess) {
         sbz_set_error("code object lacks the solver kernels");
         (void)hipModuleUnload(m->module);
@@ -233,6 +240,8 @@ static void plan_free(sbz_plan* p) {
     void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src, p->d_ws,
                     p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue,
                     p->d_srcbg};
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -375,15 +384,54 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     return SBZ_OK;
 }
 
-extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
-    sbz_clear_error();
-    if (!p) return SBZ_ERR_ARG;
+static hipError_t launch_fill(sbz_model* m, double* ptr, long long n, double v) {
+    if (!ptr || n <= 0) return hipSuccess;
+    void* args[] = {&ptr, &n, &v};
+    const long long blocks = std::min<long long>((n + 255) / 256, 4096);
+    return hipModuleLaunchKernel(m->k_fill, (unsigned)blocks, 1, 1, 256, 1, 1, 0, m->stream, args, nullptr);
+}
+
+static hipError_t launch_pk(sbz_plan* p) {
     sbz_model* m = p->m;
-    if (p->n_inst == 0) {
-        if (kernel_ms) *kernel_ms = 0.0f;
-        return SBZ_OK;
+    const double* params = p->d_params;
+    const double* knots = p->d_knots;
+    const long long* koff = p->d_koff;
+    const int* kn = p->d_knotn;
+    const int* set = p->d_set;
+    const double* kk = p->d_k;
+    const double* uf = p->d_out;
+    int n = p->n_inst;
+    double* dpk = p->d_pk;
+    void* args[] = {&params, &knots, &koff, &kn, &set, &kk, &uf, &n, &dpk};
+    return hipModuleLaunchKernel(m->k_pk, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, m->stream, args, nullptr);
+}
+
+// The launches of one run on the model's stream: NaN into the save/source outputs, the source
+// background table, the queue reset and the persistent solve kernel (bracketed by ev0/ev1 when
+// `timed`), then the P(k) kernel when `with_pk`.  Used eagerly and under stream capture.
+static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
+    sbz_model* m = p->m;
+    hipError_t e = hipSuccess;
+    const double nan = __builtin_nan("");
+    if (p->d_save) e = launch_fill(m, p->d_save, (long long)p->n_inst * p->n_save * m->n, nan);
+    if (e == hipSuccess && p->d_src) e = launch_fill(m, p->d_src, (long long)p->n_inst * p->n_save * 3, nan);
+    if (e == hipSuccess && p->d_srcbg) {
+        // the background at every save time of every set, shared by all modes of the set
+        const double* params = p->d_params;
+        const double* knots = p->d_knots;
+        const long long* koff = p->d_koff;
+        const int* kn = p->d_knotn;
+        const double* sv = p->d_saveat;
+        int stride = p->saveat_stride, n_save = p->n_save;
+        long long n_rows = (long long)p->n_sets * p->n_save;
+        double* out = p->d_srcbg;
+        void* bargs[] = {&params, &knots, &koff, &kn, &sv, &stride, &n_save, &n_rows, &out};
+        e = hipModuleLaunchKernel(m->k_srcbg, (unsigned)((n_rows + 63) / 64), 1, 1, 64, 1, 1, 0, m->stream, bargs,
+                                  nullptr);
     }
-    HIPCHK(hipMemsetAsync(p->d_queue, 0, sizeof(unsigned int), m->stream));
+    unsigned int* q = p->d_queue;
+    void* qargs[] = {&q};
+    if (e == hipSuccess) e = hipModuleLaunchKernel(m->k_qreset, 1, 1, 1, 64, 1, 1, 0, m->stream, qargs, nullptr);
     SbzSolveArgsHost a;
     a.params = p->d_params;
     a.knots = p->d_knots;
@@ -409,29 +457,66 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     a.ws = p->d_ws;
     a.S = p->lanes;
     a.srcbg = p->d_srcbg;
-    if (p->d_srcbg) {
-        // the background at every save time of every set, shared by all modes of the set
-        const double* params = p->d_params;
-        const double* knots = p->d_knots;
-        const long long* koff = p->d_koff;
-        const int* kn = p->d_knotn;
-        const double* sv = p->d_saveat;
-        int stride = p->saveat_stride, n_save = p->n_save;
-        long long n_rows = (long long)p->n_sets * p->n_save;
-        double* out = p->d_srcbg;
-        void* bargs[] = {&params, &knots, &koff, &kn, &sv, &stride, &n_save, &n_rows, &out};
-        HIPCHK(hipModuleLaunchKernel(m->k_srcbg, (unsigned)((n_rows + 63) / 64), 1, 1, 64, 1, 1, 0, m->stream, bargs,
-                                     nullptr));
-    }
     void* args[] = {&a};
     hipFunction_t kf = (p->algo == SBZ_ALGO_LANE) ? m->k_solve : (p->algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
     unsigned grid = (p->algo == SBZ_ALGO_LANE) ? (unsigned)(p->lanes / p->block) : (unsigned)p->lanes;
-    HIPCHK(hipEventRecord(p->ev0, m->stream));
-    HIPCHK(hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, args, nullptr));
-    HIPCHK(hipEventRecord(p->ev1, m->stream));
+    if (e == hipSuccess && timed) e = hipEventRecord(p->ev0, m->stream);
+    if (e == hipSuccess) e = hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, args, nullptr);
+    if (e == hipSuccess && timed) e = hipEventRecord(p->ev1, m->stream);
+    if (e == hipSuccess && with_pk) e = launch_pk(p);
+    return e;
+}
+
+extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
+    sbz_clear_error();
+    if (!p) return SBZ_ERR_ARG;
+    p->pk_ready = false;
+    if (p->n_inst == 0) {
+        if (kernel_ms) *kernel_ms = 0.0f;
+        return SBZ_OK;
+    }
+    HIPCHK(plan_enqueue(p, true, false));
     HIPCHK(hipEventSynchronize(p->ev1));
     HIPCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));
     if (kernel_ms) *kernel_ms = p->last_ms;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_run_graph(sbz_plan* p, float* graph_ms) {
+    sbz_clear_error();
+    if (!p) return SBZ_ERR_ARG;
+    p->pk_ready = false;
+    if (p->n_inst == 0) {
+        if (graph_ms) *graph_ms = 0.0f;
+        return SBZ_OK;
+    }
+    sbz_model* m = p->m;
+    if (!p->gexec) {
+        HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = plan_enqueue(p, false, true);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(m->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            sbz_set_error("sbz_plan_run_graph: capture failed: %s",
+                          hipGetErrorString(e != hipSuccess ? e : e2));
+            return SBZ_ERR_HIP;
+        }
+        p->graph = g;
+        e = hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            p->gexec = nullptr;
+            sbz_set_error("sbz_plan_run_graph: hipGraphInstantiate: %s", hipGetErrorString(e));
+            return SBZ_ERR_HIP;
+        }
+    }
+    HIPCHK(hipEventRecord(p->ev0, m->stream));
+    HIPCHK(hipGraphLaunch(p->gexec, m->stream));
+    HIPCHK(hipEventRecord(p->ev1, m->stream));
+    HIPCHK(hipEventSynchronize(p->ev1));
+    HIPCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));
+    if (graph_ms) *graph_ms = p->last_ms;
+    p->pk_ready = true;
     return SBZ_OK;
 }
 
@@ -440,19 +525,8 @@ extern "C" int sbz_plan_pk(sbz_plan* p, double* pk) {
     if (!p || !pk) return SBZ_ERR_ARG;
     if (p->n_inst == 0) return SBZ_OK;
     sbz_model* m = p->m;
-    const double* params = p->d_params;
-    const double* knots = p->d_knots;
-    const long long* koff = p->d_koff;
-    const int* kn = p->d_knotn;
-    const int* set = p->d_set;
-    const double* kk = p->d_k;
-    const double* uf = p->d_out;
-    int n = p->n_inst;
-    double* dpk = p->d_pk;
-    void* args[] = {&params, &knots, &koff, &kn, &set, &kk, &uf, &n, &dpk};
-    unsigned grid = (unsigned)((n + 255) / 256);
-    HIPCHK(hipModuleLaunchKernel(m->k_pk, grid, 1, 1, 256, 1, 1, 0, m->stream, args, nullptr));
-    HIPCHK(hipMemcpyAsync(pk, p->d_pk, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    if (!p->pk_ready) HIPCHK(launch_pk(p));
+    HIPCHK(hipMemcpyAsync(pk, p->d_pk, sizeof(double) * p->n_inst, hipMemcpyDeviceToHost, m->stream));
     HIPCHK(hipStreamSynchronize(m->stream));
     return SBZ_OK;
 }

@@ -165,6 +165,12 @@ class CosmologySolution:
         self._t0, self._ptopts = t0, ptopts
 
     def _modes_at(self, modes, taus):
+        """States [n_modes][n_τ][n] of solved modes at the times τs.  Times must lie in the solved
+        timespan [τ_bg0, τ0] (the states before a mode's start time are its initial state);
+        a mode that failed has NaN at the times it never reached, with the reference's warning."""
+        t_lo, t_hi = self.bg.t[0], self.bg.t[-1]
+        if np.any(~(taus >= t_lo)) or np.any(~(taus <= t_hi)):
+            raise ValueError(f"Requested times must lie in the solved timespan [{t_lo}, {t_hi}]")
         gm = self.prob.gpu
         gm.upload_background(0, self.bg.t, self.bg.u, self.bg.du, self.bg.P)
         o = self._ptopts
@@ -173,9 +179,14 @@ class CosmologySolution:
                        atol=o["abstol"], maxiters=o["maxiters"], saveat=taus[order])
         try:
             plan.run()
-            u = plan.fetch(save=True)["usave"]              # [n_modes][n_τ][n]
+            r = plan.fetch(save=True)
         finally:
             plan.close()
+        for j, rc in enumerate(r["retcode"]):
+            if rc != 0:
+                warnings.warn(f"Perturbation (mode k = {self.ks[modes[j]]}) solution failed with return code "
+                              f"{RETCODES[int(rc)]}; its states past the failure are NaN.")
+        u = r["usave"]                                      # [n_modes][n_τ][n]
         out = np.empty_like(u)
         out[:, order] = u
         return out

@@ -84,6 +84,7 @@ def _declare(L):
     L.sbz_plan_create.argtypes = [vp, ip, dp, dp, ctypes.c_int, ctypes.POINTER(_Opts), dp, ctypes.c_int,
                                   ctypes.POINTER(vp)]
     L.sbz_plan_run.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.sbz_plan_run_graph.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.sbz_plan_fetch.argtypes = [vp, dp, dp, ip, llp]
     L.sbz_plan_pk.argtypes = [vp, dp]
     L.sbz_plan_fetch_sources.argtypes = [vp, dp]
@@ -169,6 +170,13 @@ class Plan:
     def run(self):
         ms = ctypes.c_float(0.0)
         _lib.check(self.gm.L.sbz_plan_run(self.h, ctypes.byref(ms)), "sbz_plan_run")
+        return ms.value
+
+    def run_graph(self):
+        """The same run replayed from a captured HIP graph (solve and P(k) in one launch); returns
+        the graph's duration in ms.  `pk()` afterwards returns the graph's P(k)."""
+        ms = ctypes.c_float(0.0)
+        _lib.check(self.gm.L.sbz_plan_run_graph(self.h, ctypes.byref(ms)), "sbz_plan_run_graph")
         return ms.value
 
     def fetch(self, save=False):

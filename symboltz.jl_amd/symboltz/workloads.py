@@ -1,0 +1,88 @@
+"""Synthetic workloads of the perturbation stage at the sizes the benchmark and the tests use.
+
+Each returns (model, parameter sets, k in 1/Mpc, description).  The cosmology of the single-set
+workloads is h = 0.67, ω_b = 0.0224, ω_c = 0.120, T_CMB = 2.7255 K, Y_He = 0.245,
+A_s = 2.1e-9, n_s = 0.966, hydrogen reionization at z = 7.7.
+
+- "massless10": ΛCDM with massless ν (N_eff = 3.046), every hierarchy truncated at lmax = 10
+  (n = 38 unknowns); 100 k log-spaced in [1e-4, 1]/Mpc.
+- "massive_nu100": the same cosmology with one massive ν (0.06 eV, 4 momentum bins, lmax 10),
+  N_eff(massless) = 2.046, photon and massless-ν lmax 16 (n = 100); 1024 k log-spaced in
+  [1e-5, 10]/Mpc, with the CMB sources on a 2000-point cosgrid of conformal time.
+- "lhs128": the n = 38 model over 128 Latin-hypercube parameter sets (seed 0) in h ∈ [0.60, 0.75],
+  ω_b ∈ [0.020, 0.024], ω_c ∈ [0.10, 0.14], ln(10¹⁰A_s) ∈ [2.9, 3.2], n_s ∈ [0.92, 1.00],
+  τ_reion ∈ [0.04, 0.08] (met by a root-find on z_re), × 512 k log-spaced in [1e-5, 10]/Mpc:
+  65,536 instances in one launch.
+
+The reference has no τ_reion parameter (reionization is fixed at z = 7.6711 in baryons.jl:202-203);
+z_re is this build's parameter, and τ_reion is converted by `problem.z_re_for_optical_depth`.
+"""
+import math
+
+import numpy as np
+
+BASE = {"h": 0.67, "Omega_b0": 0.0224 / 0.67**2, "Omega_c0": 0.120 / 0.67**2, "T0": 2.7255, "YHe": 0.245,
+        "ln_As1e10": math.log(2.1e-9 * 1e10), "ns": 0.966, "z_re": 7.7}
+
+LHS_LO = np.array([0.60, 0.020, 0.10, 2.9, 0.92, 0.04])
+LHS_HI = np.array([0.75, 0.024, 0.14, 3.2, 1.00, 0.08])
+
+
+def lhs_sets(n_sets=128, seed=0, take=None, threads=16):
+    """The first `take` (default all) of `n_sets` Latin-hypercube parameter sets over h, ω_b, ω_c,
+    ln 10¹⁰A_s, n_s and τ_reion, for the massless-ν lmax-10 model; returns (model, sets)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from scipy.stats import qmc
+    from . import models
+    from .problem import z_re_for_optical_depth
+    X = qmc.scale(qmc.LatinHypercube(d=6, seed=seed).random(n_sets), LHS_LO, LHS_HI)
+    if take is not None:
+        X = X[:take]
+    M = models.LCDM(lmax=10, h=False)
+    sets = [dict(BASE, h=h, Omega_b0=wb / h**2, Omega_c0=wc / h**2, ln_As1e10=lnAs, ns=ns, Neff=3.046)
+            for h, wb, wc, lnAs, ns, _ in X]
+    with ThreadPoolExecutor(max_workers=threads) as ex:   # the background solves release the GIL
+        zs = list(ex.map(lambda a: z_re_for_optical_depth(M, a[0], a[1], xtol=1e-6), zip(sets, X[:, 5])))
+    for p, z in zip(sets, zs):
+        p["z_re"] = z
+    return M, sets
+
+
+def massive_nu100():
+    """(model, parameters) of the n = 100 massive-ν model."""
+    from . import models
+    M = models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4)
+    return M, dict(BASE, Neff=2.046, N=1, m_eV=0.06)
+
+
+def source_taus(t_bg0, tau0, n=2000):
+    """cosgrid(max(τ_bg0, 0.01), τ0; length = n) (src/utils.jl:285, angular.jl:280-286)."""
+    a0, b0 = max(t_bg0, 1e-2), tau0
+    x = np.linspace(0.0, 0.5, n)
+    cospi = np.where(x == 0.5, 0.0, np.cos(np.pi * x))   # cospi(1/2) = 0 exactly, as Julia's cospi
+    return a0 + (b0 - a0) * (1.0 - cospi)
+
+
+def workload(name):
+    """(model, parameter sets, k in 1/Mpc, description) of a named workload."""
+    from . import models
+    if name == "lhs128":
+        M, sets = lhs_sets(128)
+        ks = np.logspace(-5, 1, 512)
+        desc = ("LCDM massless nu lmax 10, n=38; 128 Latin-hypercube parameter sets (seed 0: h, omega_b, "
+                "omega_c, ln 1e10 As, ns, tau_reion via a z_re root-find) x 512 k log-spaced in [1e-5, 10]/Mpc "
+                "= 65536 instances in one persistent launch; rtol=atol=1e-5; P(k) at z=0")
+        return M, sets, ks, desc
+    if name == "massive_nu100":
+        M, pars = massive_nu100()
+        ks = np.logspace(-5, 1, 1024)
+        desc = ("LCDM + 1 massive nu (0.06 eV, 4 bins, lmax 10), photon/massless-nu lmax 16, n=100; "
+                "1024 k log-spaced in [1e-5, 10]/Mpc; rtol=atol=1e-5; P(k) at z=0 and CMB sources "
+                "(S_T, S_E, S_psi) on a 2000-point cosgrid of conformal time")
+        return M, [pars], ks, desc
+    if name == "massless10":
+        M = models.LCDM(lmax=10, h=False)
+        ks = np.logspace(-4, 0, 100)
+        desc = "LCDM massless nu lmax 10, n=38; 100 k log-spaced in [1e-4, 1]/Mpc; rtol=atol=1e-5"
+        return M, [dict(BASE, Neff=3.046)], ks, desc
+    raise ValueError(f"unknown workload {name}")

@@ -91,24 +91,35 @@ def test_failure_retcodes(fixture5):
 
 
 def test_visibility_derivatives_by_finite_differences(fixture5):
-    """The sources' v, v', v'' (second-order Taylor arithmetic through the background) against
-    central differences of v(τ) = -κ̇ e^{-κ} inside one spline interval."""
+    """The sources' v, v', v'' (second-order Taylor arithmetic through the background, baryons.jl:179)
+    against 5-point central differences of v(τ) = -κ̇ e^{-κ} inside one spline interval (where the
+    cubic Hermite background is smooth), across recombination, reionization and late times."""
     S, O = fixture5, fixture5.O
-    rng = np.random.default_rng(5)
     n = O.n
     L = O.layout()
-    for i in [300, 420, 500, 600]:
-        tau = 0.5 * (S.t[i] + S.t[i + 1])
-        h = 1e-3 * (S.t[i + 1] - S.t[i])
-        vis = lambda t: -O.background(t)["kappadot"] * np.exp(-O.background(t)["kappa"])
-        v0, vp, vm = vis(tau), vis(tau + h), vis(tau - h)
-        vd_fd = (vp - vm) / (2 * h)
-        vdd_fd = (vp - 2 * v0 + vm) / h**2
-        # isolate v, v', v'' through S_E and S_T of a state with only Π_γ = F2 = 1 (Φ = Ψ = 0 otherwise)
+    ti = S.t
+    taurec = S.P[25]
+    picks = [np.searchsorted(ti, x) for x in (0.8 * taurec, taurec, 1.2 * taurec, 0.5 * ti[-1], 0.9 * ti[-1])]
+    vis = lambda t: -O.background(t)["kappadot"] * np.exp(-O.background(t)["kappa"])
+    for i in picks:
+        tau = 0.5 * (ti[i] + ti[i + 1])
+        v, vd, vdd, eK = O.visibility(tau)
+        assert abs(v / vis(tau) - 1) < 1e-13
+        assert abs(eK - np.exp(-O.background(tau)["kappa"])) < 1e-15
+        # v' against differences of v, v'' against differences of v' (one derivative per stencil
+        # keeps the rounding of the differences below 1e-6)
+        h = 0.1 * (ti[i + 1] - ti[i])
+        f = {j: vis(tau + j * h) for j in (-2, -1, 1, 2)}
+        vd_fd = (f[-2] - 8 * f[-1] + 8 * f[1] - f[2]) / (12 * h)
+        assert abs(vd - vd_fd) <= 3e-6 * abs(vd), (i, vd, vd_fd)
+        h = 0.05 * (ti[i + 1] - ti[i])
+        g = {j: O.visibility(tau + j * h)[1] for j in (-2, -1, 1, 2)}
+        vdd_fd = (g[-2] - 8 * g[-1] + 8 * g[1] - g[2]) / (12 * h)
+        assert abs(vdd - vdd_fd) <= 1e-5 * abs(vdd), (i, vdd, vdd_fd)
+        # S_E = 3 v Π_γ / (16 (k χ)²) of a state with only Π_γ = F2 = 1 reads the same v
         u = np.zeros(n)
         u[L["F"] + 2] = 1.0
         k = 1.0
         chi = S.t[-1] - tau
         SE = O.sources(k, tau, u)[1]
-        assert abs(SE / (3 / 16 * v0 / (k * chi) ** 2) - 1) < 1e-10
-        assert np.isfinite(vd_fd) and np.isfinite(vdd_fd)
+        assert abs(SE / (3 / 16 * v / (k * chi) ** 2) - 1) < 1e-10

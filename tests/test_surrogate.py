@@ -30,3 +30,35 @@ def test_surrogate_gain_and_fallback():
     assert np.all(fb)
     assert np.max(np.abs(yo - sg.solve_spectra(Xo, ks_mpc=ks)[0])) < 1e-12
     S.close()
+
+
+@pytest.mark.timeout(900)
+def test_surrogate_full_network_shape():
+    """The surrogate at its full network shape, 7 → 512⁴ → 256 (GELU), an ensemble of 2 trained by
+    the GPU kernels on 4,096 Latin-hypercube spectra of the GPU solver (256 k in [1e-4, 5]/Mpc),
+    judged by the solver on 512 held-out sets: held-out max |P_sur/P_solver - 1| bounded, every
+    held-out request in the box, and ≥ 100× the plain solver's spectra/s at inference batch 65,536."""
+    import sys
+    from symboltz import surrogate as sg
+    say = lambda m: (print(m, flush=True), sys.stdout.flush())   # noqa: E731
+    X, Xh = sg.lhs(4096, seed=1), sg.lhs(512, seed=2)
+    Y, kern, _ = sg.solve_spectra(X)
+    Yh, kern_h, _ = sg.solve_spectra(Xh)
+    say(f"spectra: {len(X)} training sets in {kern:.1f} s of kernel time, {len(Xh)} held-out in {kern_h:.1f} s")
+    S = sg.Surrogate(n_members=2, hidden=(512, 512, 512, 512), n_out=256, max_batch=65536)
+    hist, train_s = S.fit(X, Y, epochs=1500, batch=1024, lr=1e-3)
+    say(f"trained in {train_s:.1f} s; final losses {[h[-1] for h in hist]}")
+    mean, disagree, inbox, _ = S.predict(Xh)
+    err = np.abs(np.exp(mean - Yh) - 1.0).max(axis=1)
+    say(f"held-out max |P_sur/P_solver - 1| = {err.max():.3e} (median of per-set max {np.median(err):.3e}); "
+        f"member disagreement max {disagree.max():.2e}")
+    assert np.all(inbox)
+    assert err.max() < 0.02
+    assert np.median(err) < 0.005
+    Xb = sg.lhs(65536, seed=3)
+    S.predict(Xb[:1024])
+    _, _, _, ms = S.predict(Xb)
+    rate, plain = 65536 / (ms * 1e-3), len(Xh) / kern_h
+    say(f"inference {rate:.3e} spectra/s vs plain solver {plain:.1f} spectra/s")
+    assert rate > 100 * plain
+    S.close()
