@@ -116,10 +116,10 @@ class Oracle:
         return out
 
     def background(self, tau):
-        out = np.zeros(11)
+        out = np.zeros(15)
         self.L.ora_background(_p(self.P), *self._sp(), tau, _p(out))
-        return dict(zip(["a", "H", "kappadot", "cs2", "rho_g", "rho_nu", "rho_c", "rho_b", "rho_h", "P_h", "kappa"],
-                        out.tolist()))
+        return dict(zip(["a", "H", "kappadot", "cs2", "rho_g", "rho_nu", "rho_c", "rho_b", "rho_h", "P_h", "kappa",
+                         "rho_x", "w_x", "Xe", "Tb"], out.tolist()))
 
     def delta_m(self, k, tau, y):
         y = np.ascontiguousarray(y, dtype=np.float64)

@@ -179,7 +179,7 @@ inline void spline_eval(const Spline& s, double tau, double out[5]) {
 
 template <class T>
 struct Bg {
-    T a, H, rg, rn, rc, rb, rh, ph, Irho, IP, rx, wx, wdotx, kd, cs2, kappa;
+    T a, H, rg, rn, rc, rb, rh, ph, Irho, IP, rx, wx, wdotx, kd, cs2, kappa, Xe, Tb;
     T E[8];
 };
 
@@ -252,6 +252,8 @@ void background_at(const double* P, const Spline& s, T tau, Bg<T>& b) {
         Xe = Xe + reion(a, P[P_ZRE1], P[P_DZRE1], P[P_NRE1], 1.0 + fHe);
         Xe = Xe + reion(a, P[P_ZRE2], P[P_DZRE2], P[P_NRE2], fHe);
     }
+    b.Xe = Xe;
+    b.Tb = Tb;
     b.kd = -a / H0 * (Xe * nH) * SIGMA_T * C_LIGHT;
     T DT = -2.0 * Tb * b.H - a / h * 8.0 / 3.0 * SIGMA_T * A_RAD / H100 * (Tg * Tg * Tg * Tg) / (ME_KG * C_LIGHT) *
            Xe / (1.0 + fHe + Xe) * dT;
@@ -797,13 +799,13 @@ void ora_initial_state(const int* md, const double* P, const double* kt, const d
     initial_state(m, P, s, k, tau, u);
 }
 
-// Background quantities at τ: a, ℋ, κ̇, c_s², ρ_γ, ρ_ν, ρ_c, ρ_b, ρ_h, P_h, κ.
+// Background quantities at τ: a, ℋ, κ̇, c_s², ρ_γ, ρ_ν, ρ_c, ρ_b, ρ_h, P_h, κ, ρ_X, w_X, X_e, T_b.
 void ora_background(const double* P, const double* kt, const double* ku, const double* kdu, int nk, double tau,
                     double* out) {
     Spline s{kt, ku, kdu, nk};
     Bg<double> b;
     background_at<double>(P, s, tau, b);
-    double v[11] = {b.a, b.H, b.kd, b.cs2, b.rg, b.rn, b.rc, b.rb, b.rh, b.ph, b.kappa};
+    double v[15] = {b.a, b.H, b.kd, b.cs2, b.rg, b.rn, b.rc, b.rb, b.rh, b.ph, b.kappa, b.rx, b.wx, b.Xe, b.Tb};
     std::memcpy(out, v, sizeof(v));
 }
 

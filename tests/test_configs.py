@@ -82,24 +82,45 @@ def test_lhs_sets_wave2_forced_match_oracle():
     oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), 3), r["u"], pk)
 
 
-def test_lhs_sets_above_capacity_autotuned():
-    # more instances than one wavefront per SIMD can hold: the autotuner times both variants on
-    # the batch and keeps the faster; both must agree with the oracle whichever it picks
-    from symboltz.autotune import capacity, choose
+def test_lhs_sets_above_capacity_both_variants_match_oracle():
+    # twice the one-per-SIMD capacity: both occupancy builds, forced, against the oracle
+    from symboltz.autotune import capacity
     M, sets = workloads.lhs_sets(128, seed=0, take=16)
     gm = GpuModel(M)
     cap1, cap2 = capacity(gm, ALGO_WAVE), capacity(gm, ALGO_WAVE2)
     setups = setups_of(M, sets)
-    ks = np.logspace(-5, 1, max(64, (cap1 + len(sets)) // len(sets) + 1))
-    say(f"case: 16 Latin-hypercube sets x {len(ks)} k, n = 38 (capacity WAVE {cap1}, WAVE2 {cap2})")
+    ks = np.logspace(-5, 1, max(64, 2 * cap1 // len(sets)))
+    say(f"case: 16 Latin-hypercube sets x {len(ks)} k, n = 38 (capacity WAVE {cap1}, WAVE2 {cap2}), both variants")
+    for algo in (ALGO_WAVE, ALGO_WAVE2):
+        plan, iset, kk, tt = plan_for(gm, setups, ks, algo=algo)
+        assert len(kk) >= 2 * cap1
+        ms = plan.run()
+        r = plan.fetch()
+        pk = plan.pk()
+        plan.close()
+        say(f"{'WAVE2' if algo == ALGO_WAVE2 else 'WAVE'}: {ms:.1f} ms")
+        assert np.all(r["retcode"] == 0)
+        oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), max(1, len(kk) // 96)), r["u"], pk)
+
+
+def test_lhs_sets_autotuner_picks_two_per_simd_for_throughput():
+    # at 8x the two-per-SIMD capacity the kernel time is set by throughput, not by the longest
+    # mode, and the second resident wavefront pays; the autotuner must measure that and pick it
+    from symboltz.autotune import capacity
+    M, sets = workloads.lhs_sets(128, seed=0, take=16)
+    gm = GpuModel(M)
+    cap2 = capacity(gm, ALGO_WAVE2)
+    setups = setups_of(M, sets)
+    ks = np.logspace(-5, 1, 8 * cap2 // len(sets))
+    say(f"case: 16 Latin-hypercube sets x {len(ks)} k = {16 * len(ks)} instances, ALGO_AUTO")
     plan, iset, kk, tt = plan_for(gm, setups, ks, algo=ALGO_AUTO)
-    assert len(kk) > cap1
     say(f"autotuner chose {'WAVE2' if plan.algo == ALGO_WAVE2 else 'WAVE'}")
-    assert plan.algo == ALGO_WAVE2   # at twice the one-per-SIMD capacity the second wavefront pays
-    plan.run()
+    assert plan.algo == ALGO_WAVE2
+    ms = plan.run()
     r = plan.fetch()
     pk = plan.pk()
     plan.close()
+    say(f"kernel {ms:.1f} ms, {len(kk) / (ms * 1e-3):.0f} k-mode solves/s")
     assert np.all(r["retcode"] == 0)
     oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), max(1, len(kk) // 96)), r["u"], pk)
 
