@@ -161,6 +161,22 @@ int sbz_solve_batch(sbz_model* m, const int* inst_set, const double* inst_k, con
                     const sbz_opts* opts, const double* saveat, int n_save, double* out_u, double* out_save,
                     int* retcode, long long* stats);
 
+/* sbz_solve_batch without any device allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc).  Same arguments and results as sbz_solve_batch, with these limits:
+ *  - every device buffer of the call (background tables of all uploaded sets, instances, saveat,
+ *    outputs, work counter) must fit SBZ_NOALLOC_SCRATCH_BYTES, a device array the library holds on
+ *    each GPU it runs on until the process ends; larger calls fail with SBZ_ERR_ARG;
+ *  - wavefront kernels only (opts->algo 0, SBZ_ALGO_WAVE or SBZ_ALGO_WAVE2), states only (no
+ *    SBZ_SAVE_SOURCES, no SBZ_SAVEAT_PER_SET);
+ *  - instances are processed in the given order (the plan entry points sort them longest first:
+ *    pass them by descending k for the same load balance);
+ *  - all calls on one GPU share its array: calls that overlap on one GPU race. */
+#define SBZ_NOALLOC_SCRATCH_BYTES (32 * 1024 * 1024)
+int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0,
+                            int n_inst, const sbz_opts* opts, const double* saveat, int n_save, double* out_u,
+                            double* out_save, int* retcode, long long* stats);
+
 /* ---- line-of-sight integration ----------------------------------------------------------- */
 
 /* I_l(k) = Σ_τ w_τ S(τ, k) j_l(k(τ0 - τ)) for every k and tabulated l (los_integrate,

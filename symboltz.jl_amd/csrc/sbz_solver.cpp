@@ -25,50 +25,6 @@
         }                                                                               \
     } while (0)
 
-struct SbzSetTable {
-    std::vector<double> knots;  // t[nk], u[nk][5], du[nk][5]
-    int nk = 0;
-    double params[SBZ_NPAR];
-    bool valid = false;
-};
-
-struct sbz_model {
-    int n = 0, nnz_lu = 0, nw = 0;
-    hipModule_t module = nullptr;
-    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
-    hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
-    hipStream_t stream = nullptr;
-    int device = 0;
-    std::vector<SbzSetTable> sets;
-};
-
-// Mirrors struct SbzSolveArgs of sbz_kernel.hip (passed by value as the kernel argument).
-struct SbzSolveArgsHost {
-    const double* params;
-    const double* knots;
-    const long long* knot_off;
-    const int* knot_n;
-    const int* inst_set;
-    const double* inst_k;
-    const double* inst_t0;
-    const int* order;
-    int n_inst;
-    double rtol, atol;
-    long long maxiters;
-    const double* saveat;
-    int n_save;
-    int saveat_stride;
-    double* out_u;
-    double* out_save;
-    double* out_src;
-    int* retcode;
-    long long* stats;
-    unsigned int* queue;
-    double* ws;
-    int S;
-    const double* srcbg;
-};
-
 // Row stride of the source background table (SBZ_SRCBG_STRIDE in sbz_kernel.hip).
 static constexpr int kSrcBgStride = 80;
 
@@ -255,6 +211,23 @@ static hipError_t upload(T** dst, const T* src, size_t n) {
     return hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
 }
 
+hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units) {
+    void* kargs[] = {args};
+    hipFunction_t kf = (algo == SBZ_ALGO_LANE) ? m->k_solve : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+    const unsigned grid = (algo == SBZ_ALGO_LANE) ? units / 64 : units;
+    return hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, kargs, nullptr);
+}
+
+int sbz_resident_waves(sbz_model* m, int algo) {
+    int per_cu = 0, ncu = 0;
+    hipFunction_t kw = (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kw, 64, 0) != hipSuccess || per_cu < 1)
+        per_cu = (algo == SBZ_ALGO_WAVE2) ? 8 : 4;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess || ncu < 1)
+        ncu = 256;
+    return per_cu * ncu;
+}
+
 extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     sbz_clear_error();
     if (!m || !capacity || (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_LANE)) {
@@ -321,13 +294,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
     if (opts->lanes <= 0 && p->algo != SBZ_ALGO_LANE) {
         // persistent grid: as many wavefronts as can be resident at once, each pulling instances
-        int per_cu = 0, ncu = 0;
-        hipFunction_t kw = (p->algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
-        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kw, 64, 0) != hipSuccess || per_cu < 1)
-            per_cu = (p->algo == SBZ_ALGO_WAVE2) ? 8 : 4;
-        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess || ncu < 1)
-            ncu = 256;
-        lanes = std::min(n_inst, per_cu * ncu);
+        lanes = std::min(n_inst, sbz_resident_waves(m, p->algo));
     }
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
     if (p->algo == SBZ_ALGO_LANE) lanes = (lanes + p->block - 1) / p->block * p->block;
@@ -457,11 +424,8 @@ static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
     a.ws = p->d_ws;
     a.S = p->lanes;
     a.srcbg = p->d_srcbg;
-    void* args[] = {&a};
-    hipFunction_t kf = (p->algo == SBZ_ALGO_LANE) ? m->k_solve : (p->algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
-    unsigned grid = (p->algo == SBZ_ALGO_LANE) ? (unsigned)(p->lanes / p->block) : (unsigned)p->lanes;
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev0, m->stream);
-    if (e == hipSuccess) e = hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, args, nullptr);
+    if (e == hipSuccess) e = sbz_launch_solve(m, p->algo, &a, (unsigned)p->lanes);
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev1, m->stream);
     if (e == hipSuccess && with_pk) e = launch_pk(p);
     return e;

@@ -24,7 +24,7 @@ def build_library(verbose=False):
     out = LIBDIR / "libsbz_hip.so"
     srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_los.hip", "sbz_cmb.hip",
                                     "sbz_bgsolve.hip",
-                                    "sbz_mlp.hip")]
+                                    "sbz_mlp.hip", "sbz_noalloc.hip")]
     cmd = ["hipcc", "-fPIC", "-shared", "-I", str(INCLUDE)] + HIP_FLAGS + srcs + ["-lhiprtc", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
