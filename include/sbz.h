@@ -90,6 +90,8 @@ typedef struct {
 #define SBZ_ALGO_WAVE 1  /* one 64-lane wavefront per mode: chains in lanes, row-parallel border LU (default) */
 #define SBZ_ALGO_LANE 2  /* one lane per mode, state in a strided global workspace */
 #define SBZ_ALGO_WAVE2 3 /* SBZ_ALGO_WAVE compiled for two resident wavefronts per SIMD (throughput) */
+#define SBZ_ALGO_WAVE4 4 /* four modes per wavefront, 16 lanes each (throughput); models whose border and
+                            chains fit 16 lanes (sbz_model_capacity reports 0 otherwise), no saves */
 
 /* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number
  * of concurrently resident modes (0: one per instance).  save: SBZ_SAVE_* flags (0 with saveat
@@ -151,8 +153,9 @@ int sbz_plan_pk(sbz_plan* p, double* pk);
  * out [n_masks][n_t][n_inst] with n_t = 1 or n_save.  The massive-ν bit needs a model with them. */
 int sbz_plan_pk_modes(sbz_plan* p, const int* masks, int n_masks, int at_saves, double* out);
 int sbz_plan_destroy(sbz_plan* p);
-/* Resident capacity of a kernel variant on the model's device: wavefronts (SBZ_ALGO_WAVE,
- * SBZ_ALGO_WAVE2) or lanes (SBZ_ALGO_LANE) that run at once.  No reference counterpart: the
+/* Resident capacity of a kernel variant on the model's device: modes that run at once (one per
+ * wavefront for SBZ_ALGO_WAVE and SBZ_ALGO_WAVE2, four per wavefront for SBZ_ALGO_WAVE4, one per
+ * lane for SBZ_ALGO_LANE); 0 for SBZ_ALGO_WAVE4 when the model has no such kernel.  No reference counterpart: the
  * autotuner (symboltz/autotune.py) uses it to decide whether higher occupancy can pay off. */
 int sbz_model_capacity(sbz_model* m, int algo, int* capacity);
 

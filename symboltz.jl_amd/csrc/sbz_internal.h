@@ -23,6 +23,7 @@ struct sbz_model {
     hipModule_t module = nullptr;
     hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
     hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
+    hipFunction_t k_wave4 = nullptr;   // null when the model's border or chains exceed 16 lanes
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -56,7 +57,7 @@ struct SbzSolveArgsHost {
 };
 
 // Launch the persistent solve kernel of `algo` with `args` on the model's stream: grid `units`
-// wavefronts (wave variants) or lanes/64 blocks (lane variant).
+// wavefronts (wave variants, four modes each for SBZ_ALGO_WAVE4) or lanes/64 blocks (lane variant).
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units);
 // Resident wavefronts of a wave variant on the model's device (the persistent grid size).
 int sbz_resident_waves(sbz_model* m, int algo);

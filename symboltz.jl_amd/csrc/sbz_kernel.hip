@@ -30,6 +30,27 @@ SBZ_DEV double sbz_readlane(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
 }
 
+// DPP lane permutation of a double within rows of 16 lanes (all 64 lanes must be active).
+template <int CTRL>
+SBZ_DEV double sbz_dpp(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(x & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
+}
+
+// Broadcast classes of the emitted border LU and triangular solves (BC::at<J>(v): lane J's v):
+// over the whole wavefront (one mode per wavefront) or within each row of 16 lanes (row_newbcast,
+// four modes per wavefront, one per row).
+struct SbzBcWave {
+    template <int J>
+    static SBZ_DEV double at(double v) { return sbz_readlane(v, J); }
+};
+struct SbzBcRow16 {
+    template <int J>
+    static SBZ_DEV double at(double v) { return sbz_dpp<0x150 + J>(v); }
+};
+
 // @@SBZ_MODEL@@
 
 #ifndef SBZ_N
@@ -594,14 +615,6 @@ SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB,
     }
 }
 
-// DPP lane permutation of a double within rows of 16 lanes (all 64 lanes must be active).
-template <int CTRL>
-SBZ_DEV double sbz_dpp(double v) {
-    const long long x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)(x & 0xffffffffll), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo));
-}
 
 // Rodas5P tableau as compile-time constants, so the stage combinations below fold them into
 // immediates (sbz_rodas5p.h; the __constant__ tables above stay for the lane-per-mode kernel).
@@ -621,12 +634,14 @@ struct SbzR5 {
     static constexpr double d[9] = {0, SBZ_R5_D1, SBZ_R5_D2, SBZ_R5_D3, SBZ_R5_D4, SBZ_R5_D5, 0, 0, 0};
 };
 
+// Stage combinations over the components a lane owns: i = lane + LN r for a mode spread over LN
+// lanes (64: one mode per wavefront; 16: four modes per wavefront, lane = the lane within its row).
 // stage S input  US = u_n + Σ_{j<S} a_Sj K_j  for this lane's components
-template <int S>
+template <int S, int LN>
 SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, double* US) {
 #pragma unroll
-    for (int r = 0; r < SBZ_OWN; ++r) {
-        const int i = lane + 64 * r;
+    for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+        const int i = lane + LN * r;
         if (i < SBZ_N) {
             double acc = Un[i];
 #pragma unroll
@@ -636,11 +651,11 @@ SBZ_DEV void sbz_stage_in(int lane, const double (*v)[SBZ_N], const double* Un, 
     }
 }
 // stage S right-hand side  K_S = f + h d_S ∂f/∂τ + Σ_{j<S} (c_Sj / h) K_j  (in place on K_S)
-template <int S>
+template <int S, int LN>
 SBZ_DEV void sbz_stage_rhs(int lane, const double (*v)[SBZ_N], double dt, double idt, const double* DT, double* Ks) {
 #pragma unroll
-    for (int r = 0; r < SBZ_OWN; ++r) {
-        const int i = lane + 64 * r;
+    for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+        const int i = lane + LN * r;
         if (i < SBZ_N) {
             double acc = Ks[i] + (dt * SbzR5::d[S]) * DT[i];
 #pragma unroll
@@ -651,12 +666,12 @@ SBZ_DEV void sbz_stage_rhs(int lane, const double (*v)[SBZ_N], double dt, double
 }
 // The same two combinations with one read of each stage vector: the input pass also forms this
 // lane's  ck = Σ_{j<S} (c_Sj / h) K_j  and keeps it in registers across the RHS application (for
-// the one-wavefront-per-SIMD kernel; at two per SIMD the extra live registers cost more spills)
-template <int S>
+// the one-wavefront-per-SIMD kernels; at two per SIMD the extra live registers cost more spills)
+template <int S, int LN>
 SBZ_DEV void sbz_stage_in_f(int lane, const double (*v)[SBZ_N], const double* Un, double* US, double idt, double* ck) {
 #pragma unroll
-    for (int r = 0; r < SBZ_OWN; ++r) {
-        const int i = lane + 64 * r;
+    for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+        const int i = lane + LN * r;
         if (i < SBZ_N) {
             double acc = Un[i], cc = 0.0;
 #pragma unroll
@@ -670,24 +685,25 @@ SBZ_DEV void sbz_stage_in_f(int lane, const double (*v)[SBZ_N], const double* Un
         }
     }
 }
-template <int S>
+template <int S, int LN>
 SBZ_DEV void sbz_stage_rhs_f(int lane, double dt, const double* DT, double* Ks, const double* ck) {
 #pragma unroll
-    for (int r = 0; r < SBZ_OWN; ++r) {
-        const int i = lane + 64 * r;
+    for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+        const int i = lane + LN * r;
         if (i < SBZ_N) Ks[i] = (Ks[i] + (dt * SbzR5::d[S]) * DT[i]) + ck[r];
     }
 }
-#define SBZ_STAGE_SWITCH(F, ...)                                                                  \
+#define SBZ_STAGE_SWITCH_LN(F, LN, ...)                                                           \
     switch (s) {                                                                                  \
-    case 2: F<2>(__VA_ARGS__); break;                                                             \
-    case 3: F<3>(__VA_ARGS__); break;                                                             \
-    case 4: F<4>(__VA_ARGS__); break;                                                             \
-    case 5: F<5>(__VA_ARGS__); break;                                                             \
-    case 6: F<6>(__VA_ARGS__); break;                                                             \
-    case 7: F<7>(__VA_ARGS__); break;                                                             \
-    default: F<8>(__VA_ARGS__); break;                                                            \
+    case 2: F<2, LN>(__VA_ARGS__); break;                                                         \
+    case 3: F<3, LN>(__VA_ARGS__); break;                                                         \
+    case 4: F<4, LN>(__VA_ARGS__); break;                                                         \
+    case 5: F<5, LN>(__VA_ARGS__); break;                                                         \
+    case 6: F<6, LN>(__VA_ARGS__); break;                                                         \
+    case 7: F<7, LN>(__VA_ARGS__); break;                                                         \
+    default: F<8, LN>(__VA_ARGS__); break;                                                        \
     }
+#define SBZ_STAGE_SWITCH(F, ...) SBZ_STAGE_SWITCH_LN(F, 64, __VA_ARGS__)
 
 // Wave-wide sum, identical in every lane (called with all 64 lanes active): butterflies within
 // each row of 16 lanes (quad_perm xor 1, xor 2, row_half_mirror, row_mirror), then the four row
@@ -910,10 +926,10 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
             double al, be;
             sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
             double rpo;   // reciprocal pivot of this lane's row
-            bool ok = sbz_wb_lu(c.lane, row, rpo);
+            bool ok = sbz_wb_lu<SbzBcWave>(c.lane, row, rpo);
             sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
             // Woodbury data: z = A^-1 [α β] and C^-1 with C = I - [p q]^T z (2 x 2)
-            sbz_wb_tri2(c.lane, L.M[1], L.M[2], al, be);
+            sbz_wb_tri2<SbzBcWave>(c.lane, L.M[1], L.M[2], al, be);
             wb.z1 = c.isB ? al : 0.0;
             wb.z2 = c.isB ? be : 0.0;
             {
@@ -968,7 +984,7 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
                     {
                         double b = c.isB ? Ks[c.bcomp] : 0.0;
                         if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                        sbz_wb_tri(c.lane, L.M[1], L.M[2], b);   // y = A^-1 b
+                        sbz_wb_tri<SbzBcWave>(c.lane, L.M[1], L.M[2], b);   // y = A^-1 b
                         const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
                         const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                         if (c.isB) {
@@ -1066,12 +1082,389 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     }
 }
 
+#if SBZ_WAVE4_OK
+// ============================================================================================
+// Four modes per wavefront (SBZ_ALGO_WAVE4): the 16 lanes of row g = lane / 16 integrate one mode,
+// for models whose border and chains fit 16 lanes.  Per mode the work is sbz_wave_body's, with
+// the wavefront broadcasts and reductions in their 16-lane row forms (DPP row_newbcast and a
+// row butterfly).  The four rows share one uniform control flow: every iteration each row takes
+// one step attempt; accept/reject, step size and time are per row (uniform within the row); a
+// row whose mode ends writes it and pulls the next instance.  Rows without a mode (the queue is
+// empty) step on harmless values and discard them.  States only: no saves, no sources.
+// ============================================================================================
+struct Wave4Row {
+    double v[14][SBZ_N];           // 0,1 U (ping-pong), 2,3 F (ping-pong), 4 DT, 5 US, 6..13 K1..K8
+    double B[6][SBZ_NB];           // background at the six distinct stage times
+    double dB[SBZ_NB];             // d/dτ of the background at t_n
+    double Cs[SBZ_NC];             // per-mode constants
+    double M[3][SBZ_NBORD * SBZ_WSTRIDE];  // border staging (0); L (1) and U (2) factors
+    double X[SBZ_NBORD];           // border solution broadcast
+    double C[SBZ_NCHAIN];          // chain → border exchange
+    int mode;
+};
+
+// Sum over the 16 lanes of each row, returned in every lane of the row (all 64 lanes active).
+SBZ_DEV double sbz_row_sum(double v) {
+    v += sbz_dpp<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += sbz_dpp<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += sbz_dpp<0x141>(v);   // row_half_mirror
+    v += sbz_dpp<0x140>(v);   // row_mirror
+    return v;
+}
+// p holds in all 16 lanes of row g (all 64 lanes active).
+SBZ_DEV bool sbz_row_all(bool p, int g) {
+    const unsigned long long m = __ballot(p);
+    return ((m >> (16 * g)) & 0xFFFFull) == 0xFFFFull;
+}
+
+template <bool FUSED>
+SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
+    constexpr int LN = 16;
+    constexpr int OWN = (SBZ_N + LN - 1) / LN;
+    __shared__ Wave4Row R4[4];
+    const int g = threadIdx.x >> 4;
+    Wave4Row& L = R4[g];
+    WaveCtx c;
+    c.lane = threadIdx.x & (LN - 1);
+    c.isB = c.lane < SBZ_NBORD;
+    c.bcomp = c.isB ? sbz_bord_comp[c.lane] : 0;
+    c.bchain = c.isB ? sbz_bord_chain[c.lane] : -1;
+    c.isC = c.lane < SBZ_NCHAIN;
+    const int ci = c.isC ? c.lane : 0;
+    c.ckind = sbz_chain_kind[ci];
+    c.cL = sbz_chain_L[ci];
+    c.cbin = sbz_chain_bin[ci];
+    c.catt = sbz_chain_att[ci];
+    c.cattc = sbz_chain_attc[ci];
+    c.cbase = sbz_chain_base[ci];
+    double* const DT = L.v[4];
+    double* const US = L.v[5];
+    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += LN) L.M[0][i] = 0.0;   // structural zeros
+    // per-row state; a row without a mode keeps the tables of a valid set so its steps stay finite
+    bool have = false, exhausted = false;
+    int inst = 0;
+    int set = a.inst_set[a.order ? a.order[0] : 0];
+    const double* P = a.params + (size_t)set * SBZ_NPAR;
+    const double* kt = a.knots + a.knot_off[set];
+    int nk = a.knot_n[set];
+    double k = 1.0, t = kt[0], t1 = kt[nk - 1], dt = 1e-3 * (kt[nk - 1] - kt[0]);
+    int kidx = 0, bidx = 0, ub = 0, fb = 2;
+    long long st_acc = 0, st_rej = 0, st_f = 0, st_j = 0, iters = 0;
+    int ret = SBZ_RET_SUCCESS;
+    double qold = SBZ_CTL_QOLDINIT;
+    double row[SBZ_NBORD];
+    double cpl = 0.0;
+    SbzWb wb;
+    double cinv[SBZ_MCHAIN];
+    double csk = 0.0;
+    for (int i = c.lane; i < SBZ_N; i += LN)
+#pragma unroll
+        for (int j = 0; j < 14; ++j) L.v[j][i] = 0.0;
+    if (c.lane == 0) {
+        double Cs[SBZ_NC];
+        sbz_consts(P, Cs);
+        sbz_consts_k(Cs, k);
+#pragma unroll
+        for (int q = 0; q < SBZ_NC; ++q) L.Cs[q] = Cs[q];
+    }
+    __syncthreads();
+    // the mode of this row ends: write its state, return code and statistics
+    auto finish = [&]() {
+        double* o = a.out_u + (size_t)inst * SBZ_N;
+        for (int i = c.lane; i < SBZ_N; i += LN) o[i] = L.v[ub][i];
+        if (c.lane == 0) {
+            a.retcode[inst] = ret;
+            long long* st = a.stats + (size_t)inst * 6;
+            st[0] = st_acc;
+            st[1] = st_rej;
+            st[2] = st_f;
+            st[3] = st_j;
+            st[4] = st_j;
+            st[5] = 8 * st_j;
+        }
+        have = false;
+    };
+    for (;;) {
+        // ---- rows without a mode pull the next instance and start it
+        const bool need = !have && !exhausted;
+        if (__any(need)) {
+            if (need && c.lane == 0) L.mode = (int)atomicAdd(a.queue, 1u);
+            __syncthreads();
+            const int qi = L.mode;
+            __syncthreads();
+            const bool fresh = need && qi < a.n_inst;
+            if (need && !fresh) exhausted = true;
+            if (__any(fresh)) {
+                if (fresh) {
+                    inst = a.order ? a.order[qi] : qi;
+                    set = a.inst_set[inst];
+                    P = a.params + (size_t)set * SBZ_NPAR;
+                    kt = a.knots + a.knot_off[set];
+                    nk = a.knot_n[set];
+                    k = a.inst_k[inst];
+                    t1 = kt[nk - 1];
+                    t = a.inst_t0[inst];
+                    kidx = sbz_find_knot(kt, nk, t);
+                    bidx = kidx;
+                    ub = 0;
+                    fb = 2;
+                    st_acc = st_rej = st_f = st_j = iters = 0;
+                    ret = SBZ_RET_SUCCESS;
+                    qold = SBZ_CTL_QOLDINIT;
+                    if (c.lane == 0) {
+                        double Cs[SBZ_NC], Bx[SBZ_NB];
+                        sbz_consts(P, Cs);
+                        sbz_consts_k(Cs, k);
+#pragma unroll
+                        for (int q = 0; q < SBZ_NC; ++q) L.Cs[q] = Cs[q];
+                        int kk = kidx;
+                        sbz_bg_plain(P, kt, nk, kk, t, Bx);
+#pragma unroll
+                        for (int q = 0; q < SBZ_NB; ++q) L.B[0][q] = Bx[q];
+                        sbz_ic(L.B[0], Cs, k, L.v[0]);
+                    }
+                }
+                __syncthreads();
+                {
+                    bool fin = (k > 0.0) && (k == k);
+                    for (int i = c.lane; i < SBZ_N; i += LN) {
+                        const double v = L.v[0][i];
+                        if (!(v == v) || fabs(v) > 1e300) fin = false;
+                    }
+                    const bool fin_row = sbz_row_all(fin, g);
+                    if (fresh && !fin_row) ret = SBZ_RET_INITIALFAILURE;
+                }
+                if (fresh)
+                    for (int i = c.lane; i < SBZ_N; i += LN)
+#pragma unroll
+                        for (int j = 6; j <= 13; ++j) L.v[j][i] = 0.0;   // stage vectors start finite
+                // Hairer–Wanner initial step of the starting rows; the other rows evaluate on their
+                // (dead) stage vectors and discard the result
+                const bool init = fresh && ret == SBZ_RET_SUCCESS && t1 > t;
+                double d0 = 0.0, d1 = 0.0;
+                sbz_wave_apply(c, L.B[0], nullptr, L.Cs, k, init ? L.v[0] : L.v[6], init ? L.v[2] : L.v[7]);
+                __syncthreads();
+                if (init)
+                    for (int i = c.lane; i < SBZ_N; i += LN) {
+                        const double sk = a.atol + fabs(L.v[0][i]) * a.rtol;
+                        d0 += (L.v[0][i] / sk) * (L.v[0][i] / sk);
+                        d1 += (L.v[2][i] / sk) * (L.v[2][i] / sk);
+                    }
+                d0 = sqrt(sbz_row_sum(d0) / SBZ_N);
+                d1 = sqrt(sbz_row_sum(d1) / SBZ_N);
+                double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+                dt0 = fmin(dt0, t1 - t);
+                if (init) {
+                    for (int i = c.lane; i < SBZ_N; i += LN) L.v[1][i] = L.v[0][i] + dt0 * L.v[2][i];
+                    if (c.lane == 0) {
+                        double Bx[SBZ_NB];
+                        int kk = kidx;
+                        sbz_bg_plain(P, kt, nk, kk, t + dt0, Bx);
+#pragma unroll
+                        for (int q = 0; q < SBZ_NB; ++q) L.B[1][q] = Bx[q];
+                    }
+                }
+                __syncthreads();
+                sbz_wave_apply(c, L.B[1], nullptr, L.Cs, k, init ? L.v[1] : L.v[6], init ? L.v[3] : L.v[7]);
+                __syncthreads();
+                double d2 = 0.0;
+                if (init)
+                    for (int i = c.lane; i < SBZ_N; i += LN) {
+                        const double sk = a.atol + fabs(L.v[0][i]) * a.rtol;
+                        const double r = (L.v[3][i] - L.v[2][i]) / sk;
+                        d2 += r * r;
+                    }
+                d2 = sqrt(sbz_row_sum(d2) / SBZ_N) / dt0;
+                if (init) {
+                    const double mx = fmax(d1, d2);
+                    const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(0.01 / mx, 1.0 / 6.0);
+                    dt = fmin(fmin(100.0 * dt0, dt1), t1 - t);
+                    st_f += 2;
+                }
+                if (fresh) {
+                    have = true;
+                    if (!init) finish();   // failed initial state, or nothing to integrate
+                }
+            }
+        }
+        if (!__any(have)) break;
+        // ---- per-row checks before the attempt (the rows that stop here write their mode)
+        if (have) {
+            if (++iters > a.maxiters) ret = SBZ_RET_MAXITERS;
+            else if (t + dt >= t1 - 1e-12 * fabs(t1)) dt = t1 - t;
+            else if (dt < 1e-14 * fabs(t)) ret = SBZ_RET_DTLESSTHANMIN;
+            if (ret != SBZ_RET_SUCCESS) finish();
+        }
+        const bool live = have;
+        const double h = live ? dt : 1e-3 * (t1 - kt[0]);   // rows without a mode: a harmless step
+        double* const Un = L.v[ub];
+        double* const U1 = L.v[ub ^ 1];
+        double* const F0 = L.v[fb];
+        double* const F1 = L.v[fb ^ 1];
+        const double ig = 1.0 / (SBZ_R5_GAMMA * h);
+        const double idt = 1.0 / h;
+        // background at the six distinct stage times, lanes 0..5 of every row
+        if (c.lane < 6) {
+            const double cl = (c.lane == 0) ? 0.0 : (c.lane == 1) ? SBZ_R5_C2 : (c.lane == 2) ? SBZ_R5_C3
+                            : (c.lane == 3) ? SBZ_R5_C4 : (c.lane == 4) ? SBZ_R5_C5 : 1.0;
+            double Bq[SBZ_NB], dBq[SBZ_NB];
+            sbz_bg_dual(P, kt, nk, bidx, t + cl * h, Bq, dBq);
+#pragma unroll
+            for (int q = 0; q < SBZ_NB; ++q) L.B[c.lane][q] = Bq[q];
+            if (c.lane == 0) {
+#pragma unroll
+                for (int q = 0; q < SBZ_NB; ++q) L.dB[q] = dBq[q];
+            }
+        }
+        __syncthreads();
+        const double* B0 = L.B[0];
+        if (c.isC) {
+            const SbzChainRow r = sbz_chain_row(c, B0, L.Cs, k);
+            csk = r.sk;
+            double xcs = 0.0;
+#define SBZ_CG_FACTOR(lo, hi, len) \
+            if (SBZ_IN_GROUP(lo, hi)) xcs = sbz_chain_factor<len>(r, ig, cinv);
+            SBZ_CHAIN_GROUPS(SBZ_CG_FACTOR)
+#undef SBZ_CG_FACTOR
+            L.C[c.lane] = xcs;
+        }
+        __syncthreads();
+        const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
+        double al, be;
+        sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
+        double rpo;
+        bool ok = sbz_wb_lu<SbzBcRow16>(c.lane, row, rpo);
+        sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
+        sbz_wb_tri2<SbzBcRow16>(c.lane, L.M[1], L.M[2], al, be);
+        wb.z1 = c.isB ? al : 0.0;
+        wb.z2 = c.isB ? be : 0.0;
+        {
+            const double c11 = 1.0 - sbz_row_sum(wb.p * wb.z1), c12 = -sbz_row_sum(wb.p * wb.z2);
+            const double c21 = -sbz_row_sum(wb.q * wb.z1), c22 = 1.0 - sbz_row_sum(wb.q * wb.z2);
+            const double det = c11 * c22 - c12 * c21;
+            ok = ok && (det != 0.0) && (det == det) && (fabs(det) < 1e300);
+            const double idet = 1.0 / det;
+            wb.c11 = c22 * idet;
+            wb.c12 = -c12 * idet;
+            wb.c21 = -c21 * idet;
+            wb.c22 = c11 * idet;
+        }
+        ok = sbz_row_all(ok || !c.isB, g);
+        sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
+        if (live) st_j++;
+        __syncthreads();
+        double EEst = __builtin_inf();
+        // stages 1..8, then pass 9 evaluates f(t + h, u_{n+1}) for the next step
+        for (int s = 1; s <= 9; ++s) {
+            double* Ks = (s <= 8) ? L.v[5 + s] : F1;
+            if (s == 1) {
+                for (int i = c.lane; i < SBZ_N; i += LN) Ks[i] = F0[i] + h * SBZ_R5_D1 * DT[i];
+            } else {
+                double* in = (s <= 8) ? US : U1;
+                double ck[OWN];
+                if (s <= 8) {
+                    if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_in_f, 16, c.lane, L.v, Un, US, idt, ck)
+                    else SBZ_STAGE_SWITCH_LN(sbz_stage_in, 16, c.lane, L.v, Un, US)
+                }
+                __syncthreads();
+                sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
+                if (live) st_f++;
+                __syncthreads();
+                if (s == 9) break;
+                if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_rhs_f, 16, c.lane, h, DT, Ks, ck)
+                else SBZ_STAGE_SWITCH_LN(sbz_stage_rhs, 16, c.lane, L.v, h, idt, DT, Ks)
+            }
+            __syncthreads();
+            // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution
+            if (c.isC) {
+                double y = 0.0;
+#define SBZ_CG_FWD(lo, hi, len) \
+                if (SBZ_IN_GROUP(lo, hi)) y = sbz_chain_fwd<len>(csk, cinv, Ks + c.cbase);
+                SBZ_CHAIN_GROUPS(SBZ_CG_FWD)
+#undef SBZ_CG_FWD
+                L.C[c.lane] = y;
+            }
+            __syncthreads();
+            {
+                double b = c.isB ? Ks[c.bcomp] : 0.0;
+                if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
+                sbz_wb_tri<SbzBcRow16>(c.lane, L.M[1], L.M[2], b);   // y = A^-1 b
+                const double s1 = sbz_row_sum(wb.p * b), s2 = sbz_row_sum(wb.q * b);
+                const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
+                if (c.isB) {
+                    Ks[c.bcomp] = x;
+                    L.X[c.lane] = x;
+                }
+            }
+            __syncthreads();
+            if (c.isC) {
+                const double xatt = L.X[c.catt];
+#define SBZ_CG_BWD(lo, hi, len) \
+                if (SBZ_IN_GROUP(lo, hi)) sbz_chain_bwd<len>(csk, cinv, xatt, Ks + c.cbase);
+                SBZ_CHAIN_GROUPS(SBZ_CG_BWD)
+#undef SBZ_CG_BWD
+            }
+            __syncthreads();
+            if (s == 8) {
+                // u_{n+1} = stage-8 input + k8; error estimate k8
+                double ss = 0.0;
+                bool fin = true;
+                for (int i = c.lane; i < SBZ_N; i += LN) {
+                    const double un1 = US[i] + Ks[i];
+                    U1[i] = un1;
+                    const double sc = a.atol + fmax(fabs(Un[i]), fabs(un1)) * a.rtol;
+                    const double r = Ks[i] / sc;
+                    ss += r * r;
+                    if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                }
+                ss = sbz_row_sum(ss);
+                const bool fin_row = sbz_row_all(fin, g);   // a ballot: evaluated by every lane
+                EEst = (ok && fin_row) ? sqrt(ss / SBZ_N) : __builtin_inf();
+                if (!__any(live && EEst <= 1.0)) break;   // no row accepts: no f(t + h) needed
+            }
+        }
+        __syncthreads();
+        // the controller's two powers EEst^β1 and qold^β2, on lanes 0 and 1 of every row
+        const double pw = pow((c.lane == 1) ? qold : ((EEst < __builtin_inf()) ? EEst : 1.0),
+                              (c.lane == 1) ? SBZ_CTL_BETA2 : SBZ_CTL_BETA1);
+        const double q11 = sbz_dpp<0x150>(pw), qob = sbz_dpp<0x151>(pw);
+        if (live) {
+            if (!(EEst <= 1.0)) {
+                const double q11r = (EEst < __builtin_inf()) ? q11 : 1.0 / SBZ_CTL_QMIN;
+                dt /= fmin(1.0 / SBZ_CTL_QMIN, q11r / SBZ_CTL_GAMMA);
+                st_rej++;
+            } else {
+                const double q = (EEst == 0.0) ? 1.0 / SBZ_CTL_QMAX
+                                               : fmax(1.0 / SBZ_CTL_QMAX,
+                                                      fmin(1.0 / SBZ_CTL_QMIN, q11 / qob / SBZ_CTL_GAMMA));
+                qold = fmax(EEst, SBZ_CTL_QOLDINIT);
+                double tn = t + dt;
+                if (tn >= t1 - 1e-14 * fabs(t1)) tn = t1;
+                t = tn;
+                ub ^= 1;
+                fb ^= 1;
+                while (kidx < nk - 2 && t >= kt[kidx + 1]) ++kidx;
+                st_acc++;
+                dt = dt / q;
+                if (!(t < t1)) finish();
+            }
+        }
+        __syncthreads();
+    }
+}
+#endif
+
 // The same integrator compiled twice: for one resident wavefront per SIMD (all 512 registers
 // per lane, fewest spills: lowest latency per mode) and for two (half the registers, a few more
 // spills: the second wavefront hides the first's latency when there are more instances than
 // SIMDs).  SBZ_ALGO_WAVE / SBZ_ALGO_WAVE2 select them.
 extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body<true>(a); }
 extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body<false>(a); }
+
+#if SBZ_WAVE4_OK
+// Four modes per wavefront, one resident wavefront per SIMD (all 512 registers per lane).
+extern "C" __global__ void __launch_bounds__(64) sbz_wave4_kernel(SbzSolveArgs a) { sbz_wave4_body<true>(a); }
+#endif
 
 // Source background table for the wave kernels: one thread per (set, save time), rows as
 // sbz_srcbg_row.  Launched before the solve when the plan saves CMB sources.

@@ -149,6 +149,8 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         delete m;
         return SBZ_ERR_COMPILE;
     }
+    if (hipModuleGetFunction(&m->k_wave4, m->module, "sbz_wave4_kernel") != hipSuccess) m->k_wave4 = nullptr;
+    (void)hipGetLastError();
     if (hipStreamCreate(&m->stream) != hipSuccess) {
         sbz_set_error("hipStreamCreate failed");
         (void)hipModuleUnload(m->module);
@@ -213,16 +215,20 @@ static hipError_t upload(T** dst, const T* src, size_t n) {
 
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units) {
     void* kargs[] = {args};
-    hipFunction_t kf = (algo == SBZ_ALGO_LANE) ? m->k_solve : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+    hipFunction_t kf = (algo == SBZ_ALGO_LANE)    ? m->k_solve
+                       : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2
+                       : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4
+                                                  : m->k_wave;
     const unsigned grid = (algo == SBZ_ALGO_LANE) ? units / 64 : units;
     return hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, kargs, nullptr);
 }
 
 int sbz_resident_waves(sbz_model* m, int algo) {
     int per_cu = 0, ncu = 0;
-    hipFunction_t kw = (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+    hipFunction_t kw = (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4 : m->k_wave;
+    if (!kw) return 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kw, 64, 0) != hipSuccess || per_cu < 1)
-        per_cu = (algo == SBZ_ALGO_WAVE2) ? 8 : 4;
+        per_cu = (algo == SBZ_ALGO_WAVE2) ? 8 : (algo == SBZ_ALGO_WAVE4) ? 2 : 4;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess || ncu < 1)
         ncu = 256;
     return per_cu * ncu;
@@ -230,15 +236,23 @@ int sbz_resident_waves(sbz_model* m, int algo) {
 
 extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     sbz_clear_error();
-    if (!m || !capacity || (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_LANE)) {
+    if (!m || !capacity ||
+        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_LANE && algo != SBZ_ALGO_WAVE4)) {
         sbz_set_error("sbz_model_capacity: invalid argument");
         return SBZ_ERR_ARG;
     }
-    hipFunction_t kf = (algo == SBZ_ALGO_LANE) ? m->k_solve : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : m->k_wave;
+    hipFunction_t kf = (algo == SBZ_ALGO_LANE)    ? m->k_solve
+                       : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2
+                       : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4
+                                                  : m->k_wave;
+    if (!kf) {
+        *capacity = 0;
+        return SBZ_OK;
+    }
     int per_cu = 0, ncu = 0;
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 64, 0));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
-    *capacity = per_cu * ncu * (algo == SBZ_ALGO_LANE ? 64 : 1);
+    *capacity = per_cu * ncu * (algo == SBZ_ALGO_LANE ? 64 : algo == SBZ_ALGO_WAVE4 ? 4 : 1);
     return SBZ_OK;
 }
 
@@ -283,8 +297,15 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->save_flags = n_save > 0 ? opts->save : 0;
     if (n_save > 0 && !(p->save_flags & (SBZ_SAVE_STATES | SBZ_SAVE_SOURCES))) p->save_flags |= SBZ_SAVE_STATES;
     p->saveat_stride = per_set ? n_save : 0;
-    if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2) {
+    if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2 &&
+        opts->algo != SBZ_ALGO_WAVE4) {
         sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
+        delete p;
+        return SBZ_ERR_ARG;
+    }
+    if (opts->algo == SBZ_ALGO_WAVE4 && (!m->k_wave4 || p->save_flags != 0)) {
+        sbz_set_error(!m->k_wave4 ? "sbz_plan_create: SBZ_ALGO_WAVE4 needs a model whose border and chains fit 16 lanes"
+                                  : "sbz_plan_create: SBZ_ALGO_WAVE4 saves no states or sources (final states only)");
         delete p;
         return SBZ_ERR_ARG;
     }
@@ -294,7 +315,9 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
     if (opts->lanes <= 0 && p->algo != SBZ_ALGO_LANE) {
         // persistent grid: as many wavefronts as can be resident at once, each pulling instances
-        lanes = std::min(n_inst, sbz_resident_waves(m, p->algo));
+        // (four at a time for SBZ_ALGO_WAVE4)
+        const int need = (p->algo == SBZ_ALGO_WAVE4) ? (n_inst + 3) / 4 : n_inst;
+        lanes = std::min(need, sbz_resident_waves(m, p->algo));
     }
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
     if (p->algo == SBZ_ALGO_LANE) lanes = (lanes + p->block - 1) / p->block * p->block;

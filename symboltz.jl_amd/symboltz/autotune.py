@@ -1,13 +1,20 @@
 """Kernel-variant autotuner for the batched perturbation solve.
 
-For a plan it picks the wavefront kernel's occupancy variant: SBZ_ALGO_WAVE (one resident
-wavefront per SIMD, every register available: the lowest latency per mode) or SBZ_ALGO_WAVE2 (two
-per SIMD: the second hides the first's latency once there are more instances than SIMDs).  When
-the instances fit the one-per-SIMD capacity the choice is WAVE without measuring; otherwise each
-candidate is timed on a cost-stratified sample of the plan's own instances (every m-th of the
-instances sorted by k) large enough (8 × the larger capacity, or all) that its time is set by
-throughput rather than by its single longest mode, and the faster is kept.  Choices are
-cached per model, instance-count bucket and output kind.
+For a plan it picks among the wavefront kernels:
+- SBZ_ALGO_WAVE: one mode per wavefront, one resident wavefront per SIMD, every register
+  available: the lowest latency per mode;
+- SBZ_ALGO_WAVE2: the same with two resident per SIMD: the second hides the first's latency once
+  there are more instances than SIMDs;
+- SBZ_ALGO_WAVE4: four modes per wavefront (16 lanes each), for models whose border and chains
+  fit 16 lanes and plans that save nothing: the per-step work that the lanes of a mode share
+  (border RHS and Jacobian assembly, background) serves four modes.
+
+When the instances fit the one-per-SIMD capacity the choice is WAVE without measuring: the
+launch is then the latency of its longest mode.  Otherwise each candidate is timed on a
+cost-stratified sample of the plan's own instances (every m-th of the instances sorted by k)
+large enough (8 × the largest capacity, or all) that its time is set by throughput rather than
+by its single longest mode, and the fastest is kept.  Choices are cached per model,
+instance-count bucket and output kind.
 """
 import ctypes
 import math
@@ -16,12 +23,14 @@ import numpy as np
 
 from . import _lib
 
-ALGO_WAVE, ALGO_LANE, ALGO_WAVE2 = 1, 2, 3
+ALGO_WAVE, ALGO_LANE, ALGO_WAVE2, ALGO_WAVE4 = 1, 2, 3, 4
+NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4", ALGO_LANE: "LANE"}
 _cache = {}
+last_times = {}
 
 
 def capacity(gm, algo):
-    """Resident wavefronts (lanes for the lane kernel) of a variant on the model's device."""
+    """Resident modes of a variant on the model's device (0 when the model has no such kernel)."""
     L = gm.L
     L.sbz_model_capacity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     c = ctypes.c_int(0)
@@ -36,22 +45,28 @@ def choose(gm, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=N
     key = (gm.em.full_hash, int(math.log2(max(n, 1))), save, n_save > 0)
     if key in _cache:
         return _cache[key]
-    cap1, cap2 = capacity(gm, ALGO_WAVE), capacity(gm, ALGO_WAVE2)
-    if n <= cap1 or cap2 <= cap1:
+    caps = {ALGO_WAVE: capacity(gm, ALGO_WAVE), ALGO_WAVE2: capacity(gm, ALGO_WAVE2)}
+    if n_save == 0:
+        c4 = capacity(gm, ALGO_WAVE4)
+        if c4 > 0:
+            caps[ALGO_WAVE4] = c4
+    if n <= caps[ALGO_WAVE] or max(caps.values()) <= caps[ALGO_WAVE]:
         _cache[key] = ALGO_WAVE
         return ALGO_WAVE
     order = np.argsort(-np.asarray(ks), kind="stable")
-    m = max(1, n // (8 * cap2))
+    m = max(1, n // (8 * max(caps.values())))
     sel = np.sort(order[::m])
     times = {}
-    for algo in (ALGO_WAVE, ALGO_WAVE2):
+    for algo in caps:
         plan = gm.plan(np.asarray(inst_set)[sel], np.asarray(ks)[sel], np.asarray(t0)[sel], rtol=rtol, atol=atol,
                        maxiters=maxiters, saveat=saveat, save=save, algo=algo)
         times[algo] = plan.run()
         plan.close()
     best = min(times, key=times.get)
+    last_times.clear()
+    last_times.update({NAMES[a]: t for a, t in times.items()})
     if verbose:
-        print(f"autotune: {len(sel)} of {n} instances; WAVE {times[ALGO_WAVE]:.1f} ms, WAVE2 "
-              f"{times[ALGO_WAVE2]:.1f} ms -> {'WAVE2' if best == ALGO_WAVE2 else 'WAVE'}")
+        print(f"autotune: {len(sel)} of {n} instances; " + ", ".join(f"{NAMES[a]} {t:.1f} ms" for a, t in times.items())
+              + f" -> {NAMES[best]}")
     _cache[key] = best
     return best

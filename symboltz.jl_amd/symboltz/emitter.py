@@ -760,7 +760,9 @@ class Emitted:
                 groups[-1][1] = ci + 1
             else:
                 groups.append([ci, ci + 1, len(ch[3])])
-        out = [f"#define SBZ_NBORD {nb}", f"#define SBZ_NCHAIN {len(chains)}", f"#define SBZ_MCHAIN {mch}"]
+        out = [f"#define SBZ_NBORD {nb}", f"#define SBZ_NCHAIN {len(chains)}", f"#define SBZ_MCHAIN {mch}",
+               # four modes per wavefront (16 lanes each) when the border and the chains fit 16 lanes
+               f"#define SBZ_WAVE4_OK {1 if (nb <= 16 and len(chains) <= 16) else 0}"]
         out.append("#define SBZ_CHAIN_GROUPS(X) " + " ".join(f"X({a}, {b}, {n_})" for a, b, n_ in groups))
         out.append("__constant__ int sbz_bord_comp[SBZ_NBORD] = {" + ", ".join(map(str, bcomp)) + "};")
         for ch in chains:   # the kernel addresses chain members as base + (l - 3)
@@ -866,13 +868,16 @@ class Emitted:
             levb[j] = 1 + max((levb[c] for c in upper[j]), default=-1)
         nlevb = max(levb) + 1
         self.wave.update(lu_rounds=nlev, bwd_rounds=nlevb)
-        lu = ["SBZ_DEV bool sbz_wb_lu(int lane, double* row, double& rpo) {", "    bool ok = true;", "    rpo = 0.0;"]
+        # the pivot-row and solution broadcasts go through BC::at<j>: readlane over the wavefront
+        # (one mode per wavefront) or a DPP row broadcast within 16 lanes (four modes per wavefront)
+        lu = ["template <class BC>", "SBZ_DEV bool sbz_wb_lu(int lane, double* row, double& rpo) {", "    bool ok = true;",
+              "    rpo = 0.0;"]
         for L_ in range(nlev):
             piv = [j for j in range(nb) if lev[j] == L_]
             lu.append(f"    // round {L_}: pivots {piv} (independent of each other)")
             for j in piv:   # one pivot at a time keeps its broadcast row short-lived
                 lu.append("    {")
-                lu.append(f"        const double p{j} = sbz_readlane(row[{j}], {j});")
+                lu.append(f"        const double p{j} = BC::template at<{j}>(row[{j}]);")
                 lu.append(f"        const double rp{j} = 1.0 / p{j};")
                 lu.append(f"        ok = ok && (p{j} != 0.0) && (p{j} == p{j}) && (fabs(p{j}) < 1e300);")
                 lu.append(f"        rpo = (lane == {j}) ? rp{j} : rpo;")
@@ -881,7 +886,7 @@ class Emitted:
                     lu.append(f"        const bool on = (0x{m:x}ull >> lane) & 1ull;")
                     lu.append(f"        const double l = on ? row[{j}] * rp{j} : 0.0;")
                     for c in upper[j]:
-                        lu.append(f"        row[{c}] -= l * sbz_readlane(row[{c}], {j});")
+                        lu.append(f"        row[{c}] -= l * BC::template at<{j}>(row[{c}]);")
                     lu.append(f"        row[{j}] = on ? l : row[{j}];")
                 lu.append("    }")
         lu.append("    return ok;")
@@ -906,7 +911,8 @@ class Emitted:
         # factor entry times x_j
         def tri(name, nr):
             args = ", ".join(f"double& b{q}" for q in range(nr))
-            so = [f"SBZ_DEV void {name}(int lane, const double* __restrict__ sL, const double* __restrict__ sU, {args}) {{",
+            so = ["template <class BC>",
+                  f"SBZ_DEV void {name}(int lane, const double* __restrict__ sL, const double* __restrict__ sU, {args}) {{",
                   "    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;",
                   "    const double* lr = sL + r * SBZ_WSTRIDE;",
                   "    const double* ur = sU + r * SBZ_WSTRIDE;"]
@@ -922,7 +928,7 @@ class Emitted:
                 so.append("    {")
                 for q in range(nr):
                     for j in piv:
-                        so.append(f"        const double x{q}_{j} = sbz_readlane(b{q}, {j});")
+                        so.append(f"        const double x{q}_{j} = BC::template at<{j}>(b{q});")
                 for q in range(nr):
                     so.append(f"        b{q} -= " + " + ".join(f"l{j} * x{q}_{j}" for j in piv) + ";")
                 so.append("    }")
@@ -937,7 +943,7 @@ class Emitted:
                 so.append("    {")
                 for q in range(nr):
                     for j in piv:
-                        so.append(f"        const double x{q}_{j} = sbz_readlane(b{q}, {j}) * rp{j};")
+                        so.append(f"        const double x{q}_{j} = BC::template at<{j}>(b{q}) * rp{j};")
                 for q in range(nr):
                     so.append(f"        b{q} -= " + " + ".join(f"u{j} * x{q}_{j}" for j in piv) + ";")
                 so.append("    }")

@@ -65,7 +65,8 @@ class _Opts(ctypes.Structure):
 ALGO_WAVE = 1
 ALGO_LANE = 2
 ALGO_WAVE2 = 3
-ALGO_AUTO = -1   # host-side: symboltz.autotune picks WAVE or WAVE2
+ALGO_WAVE4 = 4   # four modes per wavefront (models whose border and chains fit 16 lanes; no saves)
+ALGO_AUTO = -1   # host-side: symboltz.autotune picks among the wavefront variants
 
 
 SAVE_STATES = 1
@@ -120,8 +121,8 @@ class GpuModel:
         self.n_sets = max(self.n_sets, set_index + 1)
 
     def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0, save=0, algo=0):
-        """algo: 0 (library default, one wavefront per SIMD), ALGO_WAVE, ALGO_WAVE2, ALGO_LANE or
-        ALGO_AUTO (timed choice between the two wavefront variants, symboltz.autotune)."""
+        """algo: 0 (library default, one wavefront per SIMD), ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_LANE
+        or ALGO_AUTO (timed choice among the wavefront variants, symboltz.autotune)."""
         if algo == ALGO_AUTO:
             from .autotune import choose
             algo = choose(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, save)

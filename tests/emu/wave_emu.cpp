@@ -72,6 +72,7 @@ static int mov_dpp(int v, int ctrl, int s) {
     if (ctrl < 0x100) src = (cur & ~3) | ((ctrl >> (2 * (cur & 3))) & 3);
     else if (ctrl == 0x140) src = (cur & ~15) | (15 - (cur & 15));
     else if (ctrl == 0x141) src = (cur & ~7) | (7 - (cur & 7));
+    else if (ctrl >= 0x150 && ctrl <= 0x15F) src = (cur & ~15) | (ctrl - 0x150);   // row_newbcast
     else {
         std::fprintf(stderr, "wave_emu: unsupported DPP control 0x%x\n", ctrl);
         std::abort();
@@ -85,6 +86,23 @@ static int all(int p, int s) {
     int r = 1;
     for (int l = 0; l < W; ++l) r &= (int)xi[g][l];
     return r;
+}
+static int any(int p, int s) {
+    const int g = gen[cur] ^= 1;
+    xi[g][cur] = p ? 1 : 0;
+    barrier_at(s);
+    int r = 0;
+    for (int l = 0; l < W; ++l) r |= (int)xi[g][l];
+    return r;
+}
+static unsigned long long ballot(int p, int s) {
+    const int g = gen[cur] ^= 1;
+    xi[g][cur] = p ? 1 : 0;
+    barrier_at(s);
+    unsigned long long m = 0;
+    for (int l = 0; l < W; ++l)
+        if (xi[g][l]) m |= 1ull << l;
+    return m;
 }
 static void lane_main() {
     body(body_arg);
@@ -133,6 +151,8 @@ static int run(void (*fn)(void*), void* arg) {
 #define __builtin_amdgcn_readlane(v, l) emu::readlane_i((v), (l), __LINE__)
 #define __shfl_xor(v, off, w) emu::shfl_xor_d((v), (off), __LINE__)
 #define __all(p) emu::all((p), __LINE__)
+#define __any(p) emu::any((p), __LINE__)
+#define __ballot(p) emu::ballot((p), __LINE__)
 #define __builtin_amdgcn_mov_dpp(v, ctrl, rm, bm, bc) emu::mov_dpp((v), (ctrl), __LINE__)
 static inline unsigned atomicAdd(unsigned* p, unsigned v) {
     const unsigned o = *p;
@@ -152,10 +172,18 @@ static inline double __longlong_as_double(long long x) {
 
 #include "emu_kernel.inc"
 
-// 0: sbz_wave_kernel (fused stage combinations), 1: sbz_wave_kernel_o2 (two-pass combinations)
+// 0: sbz_wave_kernel (fused stage combinations), 1: sbz_wave_kernel_o2 (two-pass combinations),
+// 2: sbz_wave4_kernel (four modes per wavefront; models whose border and chains fit 16 lanes)
 static int g_o2 = 0;
 extern "C" void emu_set_o2(int on) { g_o2 = on; }
+extern "C" int emu_has_wave4(void) { return SBZ_WAVE4_OK; }
 static void wave_body(void* a) {
+#if SBZ_WAVE4_OK
+    if (g_o2 == 2) {
+        sbz_wave4_kernel(*(SbzSolveArgs*)a);
+        return;
+    }
+#endif
     if (g_o2) sbz_wave_kernel_o2(*(SbzSolveArgs*)a);
     else sbz_wave_kernel(*(SbzSolveArgs*)a);
 }

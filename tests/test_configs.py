@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from conftest import Setup
-from symboltz import models, workloads
+from symboltz import _lib, models, workloads
 from symboltz.constants import k_dimensionless
 from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_WAVE2, ALGO_AUTO, SAVE_SOURCES, SAVE_STATES
 
@@ -82,6 +82,35 @@ def test_lhs_sets_wave2_forced_match_oracle():
     oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), 3), r["u"], pk)
 
 
+def test_lhs_sets_wave4_forced_match_oracle():
+    # four modes per wavefront (16-lane rows, DPP row broadcasts) on 8 of the τ_reion sets
+    from symboltz.solver import ALGO_WAVE4
+    from symboltz.autotune import capacity
+    say("case: 8 of the 128 Latin-hypercube τ_reion sets x 64 k in [1e-5, 10]/Mpc, n = 38, WAVE4 forced")
+    M, sets = workloads.lhs_sets(128, seed=0, take=8)
+    setups = setups_of(M, sets)
+    gm = GpuModel(M)
+    assert capacity(gm, ALGO_WAVE4) > 0
+    plan, iset, kk, tt = plan_for(gm, setups, np.logspace(-5, 1, 64), algo=ALGO_WAVE4)
+    ms = plan.run()
+    r = plan.fetch()
+    pk = plan.pk()
+    plan.close()
+    say(f"kernel sbz_wave4_kernel: {len(kk)} instances in {ms:.1f} ms")
+    assert np.all(r["retcode"] == 0)
+    oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), 3), r["u"], pk)
+    # it saves nothing, and needs a model whose border fits 16 lanes: argument errors otherwise
+    with pytest.raises(_lib.SbzError, match="WAVE4"):
+        gm.plan(iset[:4], kk[:4], tt[:4], algo=ALGO_WAVE4, saveat=np.array([1.0]))
+    M5 = models.LCDM(lmax=5)
+    S5 = Setup(M5)
+    gm5 = GpuModel(M5)
+    gm5.upload_background(0, S5.t, S5.u, S5.du, S5.P)
+    assert capacity(gm5, ALGO_WAVE4) == 0
+    with pytest.raises(_lib.SbzError, match="16 lanes"):
+        gm5.plan(np.zeros(2, np.int32), np.array([1.0, 10.0]), np.full(2, 1e-4), algo=ALGO_WAVE4)
+
+
 def test_lhs_sets_above_capacity_both_variants_match_oracle():
     # twice the one-per-SIMD capacity: both occupancy builds, forced, against the oracle
     from symboltz.autotune import capacity
@@ -103,9 +132,10 @@ def test_lhs_sets_above_capacity_both_variants_match_oracle():
         oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), max(1, len(kk) // 96)), r["u"], pk)
 
 
-def test_lhs_sets_autotuner_picks_two_per_simd_for_throughput():
+def test_lhs_sets_autotuner_picks_a_throughput_variant():
     # at 8x the two-per-SIMD capacity the kernel time is set by throughput, not by the longest
-    # mode, and the second resident wavefront pays; the autotuner must measure that and pick it
+    # mode: the autotuner must measure the variants and keep the fastest, a throughput variant
+    # (two wavefronts per SIMD, or four modes per wavefront)
     from symboltz.autotune import capacity
     M, sets = workloads.lhs_sets(128, seed=0, take=16)
     gm = GpuModel(M)
@@ -114,8 +144,9 @@ def test_lhs_sets_autotuner_picks_two_per_simd_for_throughput():
     ks = np.logspace(-5, 1, 8 * cap2 // len(sets))
     say(f"case: 16 Latin-hypercube sets x {len(ks)} k = {16 * len(ks)} instances, ALGO_AUTO")
     plan, iset, kk, tt = plan_for(gm, setups, ks, algo=ALGO_AUTO)
-    say(f"autotuner chose {'WAVE2' if plan.algo == ALGO_WAVE2 else 'WAVE'}")
-    assert plan.algo == ALGO_WAVE2
+    from symboltz.autotune import NAMES, last_times
+    say(f"autotuner chose {NAMES[plan.algo]} (sample times {last_times})")
+    assert plan.algo != ALGO_WAVE and last_times[NAMES[plan.algo]] == min(last_times.values())
     ms = plan.run()
     r = plan.fetch()
     pk = plan.pk()

@@ -93,3 +93,26 @@ def test_wave_kernel_emulated_w0wa():
     o = S.O.solve(ks, t0)
     assert np.all(g["retcode"] == 0) and np.all(o["retcode"] == 0)
     assert rel(g["u"], o["u"]) < 1e-4
+
+
+def test_wave4_kernel_emulated_matches_oracle_and_wave(massless10):
+    """sbz_wave4_kernel (four modes per wavefront, 16 lanes each: DPP row broadcasts, row
+    reductions, per-row step control) on the massless-ν lmax-10 model: 9 instances through the 4
+    rows (rows refill from the queue and run out of work at different times), k = 0 and k = NaN
+    fail with InitialFailure; states against the oracle (1e-4) and the one-mode-per-wavefront
+    kernel (1e-8: the same arithmetic but for the summation order of the error norm)."""
+    S = massless10
+    E = WaveEmu(model_source(S.M).full_src)
+    assert E.L.emu_has_wave4() == 1
+    ks = np.array([3000.0, 0.3, 0.0, 30.0, 1.0, np.nan, 300.0, 0.02, 100.0])
+    kk = np.where(ks > 0, ks, 1.0)
+    t0 = np.clip(np.minimum(1e-2 / kk, 1e-4), S.t[0], S.t[-1])
+    g4 = E.solve(S.P, S.t, S.u, S.du, ks, t0, wave4=True)
+    g1 = E.solve(S.P, S.t, S.u, S.du, ks, t0)
+    good = np.isfinite(ks) & (ks > 0)
+    assert list(g4["retcode"][~good]) == [4, 4]
+    assert np.all(g4["retcode"][good] == 0)
+    o = S.O.solve(ks[good], t0[good])
+    assert rel(g4["u"][good], o["u"]) < 1e-4
+    assert rel(g4["u"][good], g1["u"][good]) < 1e-8
+    assert np.all(np.abs(g4["stats"][good, 0] - g1["stats"][good, 0]) <= 2)
