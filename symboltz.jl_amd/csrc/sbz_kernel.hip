@@ -1097,7 +1097,7 @@ struct Wave4Row {
     double B[6][SBZ_NB];           // background at the six distinct stage times
     double dB[SBZ_NB];             // d/dτ of the background at t_n
     double Cs[SBZ_NC];             // per-mode constants
-    double M[3][SBZ_NBORD * SBZ_WSTRIDE];  // border staging (0); L (1) and U (2) factors
+    double M[SBZ_NBORD * SBZ_WSTRIDE];     // border block staging (the factors stay in registers)
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
     int mode;
@@ -1139,7 +1139,7 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
     c.cbase = sbz_chain_base[ci];
     double* const DT = L.v[4];
     double* const US = L.v[5];
-    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += LN) L.M[0][i] = 0.0;   // structural zeros
+    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += LN) L.M[i] = 0.0;   // structural zeros
     // per-row state; a row without a mode keeps the tables of a valid set so its steps stay finite
     bool have = false, exhausted = false;
     int inst = 0;
@@ -1157,6 +1157,8 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
     SbzWb wb;
     double cinv[SBZ_MCHAIN];
     double csk = 0.0;
+    double lrow[SBZ_NBORD], urow[SBZ_NBORD], rpb[SBZ_NBORD];   // this lane's factored border row
+    double rpo = 0.0;                                           // its reciprocal pivot
     for (int i = c.lane; i < SBZ_N; i += LN)
 #pragma unroll
         for (int j = 0; j < 14; ++j) L.v[j][i] = 0.0;
@@ -1331,11 +1333,10 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
         __syncthreads();
         const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
         double al, be;
-        sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
-        double rpo;
+        sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M, row, cpl, al, be, wb);
         bool ok = sbz_wb_lu<SbzBcRow16>(c.lane, row, rpo);
-        sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
-        sbz_wb_tri2<SbzBcRow16>(c.lane, L.M[1], L.M[2], al, be);
+        sbz_wb_unpack<SbzBcRow16>(c.lane, row, rpo, lrow, urow, rpb);
+        sbz_wb_tri2_r<SbzBcRow16>(lrow, urow, rpb, rpo, al, be);
         wb.z1 = c.isB ? al : 0.0;
         wb.z2 = c.isB ? be : 0.0;
         {
@@ -1388,7 +1389,7 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
             {
                 double b = c.isB ? Ks[c.bcomp] : 0.0;
                 if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                sbz_wb_tri<SbzBcRow16>(c.lane, L.M[1], L.M[2], b);   // y = A^-1 b
+                sbz_wb_tri_r<SbzBcRow16>(lrow, urow, rpb, rpo, b);   // y = A^-1 b
                 const double s1 = sbz_row_sum(wb.p * b), s2 = sbz_row_sum(wb.q * b);
                 const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                 if (c.isB) {

@@ -953,6 +953,53 @@ class Emitted:
             return so
         out += tri("sbz_wb_tri", 1)
         out += tri("sbz_wb_tri2", 2)
+
+        # The same solves with the factors kept in registers (four-modes-per-wavefront kernel, where
+        # LDS bounds the resident wavefronts): each lane unpacks its factored row once per step into
+        # its strictly-lower and strictly-upper parts, plus the broadcast reciprocal pivots
+        bwd_piv = [j for j in range(nb) if ucol[j]]
+        up = ["template <class BC>",
+              "SBZ_DEV void sbz_wb_unpack(int lane, const double* row, double rpo, double* lrow, double* urow, double* rpb) {"]
+        up.append("    const bool own = lane < SBZ_NBORD;   // lanes past the border carry zeros")
+        for j in range(nb):
+            up.append(f"    lrow[{j}] = (own && {j} < lane) ? row[{j}] : 0.0; urow[{j}] = (own && {j} > lane) ? row[{j}] : 0.0;")
+        for j in bwd_piv:
+            up.append(f"    rpb[{j}] = BC::template at<{j}>(rpo);")
+        up.append("}")
+        out += up
+
+        def tri_r(name, nr):
+            args = ", ".join(f"double& b{q}" for q in range(nr))
+            so = ["template <class BC>",
+                  f"SBZ_DEV void {name}(const double* lrow, const double* urow, const double* rpb, double rpo, {args}) {{"]
+            for L_ in range(nlev):
+                piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
+                if not piv:
+                    continue
+                so.append("    {")
+                for q in range(nr):
+                    for j in piv:
+                        so.append(f"        const double x{q}_{j} = BC::template at<{j}>(b{q});")
+                for q in range(nr):
+                    so.append(f"        b{q} -= " + " + ".join(f"lrow[{j}] * x{q}_{j}" for j in piv) + ";")
+                so.append("    }")
+            for L_ in range(nlevb):
+                piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]
+                if not piv:
+                    continue
+                so.append("    {")
+                for q in range(nr):
+                    for j in piv:
+                        so.append(f"        const double x{q}_{j} = BC::template at<{j}>(b{q}) * rpb[{j}];")
+                for q in range(nr):
+                    so.append(f"        b{q} -= " + " + ".join(f"urow[{j}] * x{q}_{j}" for j in piv) + ";")
+                so.append("    }")
+            for q in range(nr):
+                so.append(f"    b{q} *= rpo;")
+            so.append("}")
+            return so
+        out += tri_r("sbz_wb_tri_r", 1)
+        out += tri_r("sbz_wb_tri2_r", 2)
         return out
 
 
