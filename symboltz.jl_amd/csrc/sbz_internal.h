@@ -54,7 +54,13 @@ struct SbzSolveArgsHost {
     double* ws;
     int S;
     const double* srcbg;
+    const int* xcd_off;
+    int n_xcd;
 };
+
+// Work queues of a launch: one per XCD (instances grouped by set mod SBZ_N_XCD) when a plan has
+// several parameter sets, else one.
+constexpr int SBZ_N_XCD = 8;
 
 // Launch the persistent solve kernel of `algo` with `args` on the model's stream: grid `units`
 // wavefronts (wave variants, four modes each for SBZ_ALGO_WAVE4) or lanes/64 blocks (lane variant).

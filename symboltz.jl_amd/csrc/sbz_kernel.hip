@@ -98,7 +98,28 @@ struct SbzSolveArgs {
     int S;                     // number of lanes (multiple of 64)
     const double* srcbg;       // [n_sets][n_save][SBZ_SRCBG_STRIDE] background at the save times
                                // (sbz_srcbg_kernel), or null: the wave kernel then evaluates it per mode
+    const int* xcd_off;        // [n_xcd + 1] segments of `order`, one work queue per XCD; null: one queue
+    int n_xcd;                 // number of queues (queue[0..n_xcd) counters)
 };
+
+// Next position in `order` for a wavefront on XCD x (blockIdx mod n_xcd: workgroups are dispatched
+// round-robin over the XCDs), or -1 when all work is taken.  With per-XCD queues each XCD first
+// drains its own segment (the instances of the parameter sets assigned to it, so their knot tables
+// stay in that XCD's L2), then takes work from the other segments.
+SBZ_DEV int sbz_pull(const SbzSolveArgs& a, int x) {
+    if (!a.xcd_off) {
+        const unsigned int qi = atomicAdd(a.queue, 1u);
+        return qi < (unsigned int)a.n_inst ? (int)qi : -1;
+    }
+    for (int d = 0; d < a.n_xcd; ++d) {
+        const int q = (x + d) % a.n_xcd;
+        const int lo = a.xcd_off[q], cnt = a.xcd_off[q + 1] - lo;
+        if (cnt <= 0) continue;
+        const unsigned int qi = atomicAdd(a.queue + q, 1u);
+        if (qi < (unsigned int)cnt) return lo + (int)qi;
+    }
+    return -1;
+}
 
 // Row of the per-set source background table: B[0..NB), dB/dτ[0..NB), d²κ̇/dτ² at 2 NB.
 #define SBZ_SRCBG_STRIDE 80
@@ -739,11 +760,11 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     __syncthreads();
 
     for (;;) {
-        if (c.lane == 0) L.mode = (int)atomicAdd(a.queue, 1u);
+        if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
         __syncthreads();
         const int qi = L.mode;
         __syncthreads();
-        if (qi >= a.n_inst) break;
+        if (qi < 0) break;
         const int inst = a.order ? a.order[qi] : qi;
         const int set = a.inst_set[inst];
         const double* saveat = a.saveat ? a.saveat + (size_t)set * a.saveat_stride : nullptr;
@@ -1190,11 +1211,11 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
         // ---- rows without a mode pull the next instance and start it
         const bool need = !have && !exhausted;
         if (__any(need)) {
-            if (need && c.lane == 0) L.mode = (int)atomicAdd(a.queue, 1u);
+            if (need && c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
             __syncthreads();
             const int qi = L.mode;
             __syncthreads();
-            const bool fresh = need && qi < a.n_inst;
+            const bool fresh = need && qi >= 0;
             if (need && !fresh) exhausted = true;
             if (__any(fresh)) {
                 if (fresh) {
@@ -1491,8 +1512,8 @@ extern "C" __global__ void __launch_bounds__(256) sbz_fill_kernel(double* __rest
 }
 
 // Zeroes the work-queue counter before a solve launch (a kernel node, not a memset node, in graphs).
-extern "C" __global__ void sbz_queue_reset_kernel(unsigned int* __restrict__ q) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0u;
+extern "C" __global__ void sbz_queue_reset_kernel(unsigned int* __restrict__ q, int n) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < n) q[threadIdx.x] = 0u;
 }
 
 // P(k, τ0) = P0(k) Δ_m(τ0)² for each instance from its final state (fourier.jl:14-52,90-97).

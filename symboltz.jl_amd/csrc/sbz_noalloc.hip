@@ -158,7 +158,8 @@ extern "C" int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const 
             NCHK(hipModuleLaunchKernel(m->k_fill, (unsigned)std::min<long long>((cnt + 255) / 256, 4096), 1, 1, 256, 1,
                                        1, 0, s0, fargs, nullptr));
         unsigned int* q = reinterpret_cast<unsigned int*>(base + o_queue);
-        void* qargs[] = {&q};
+        int nq = 1;
+        void* qargs[] = {&q, &nq};
         NCHK(hipModuleLaunchKernel(m->k_qreset, 1, 1, 1, 64, 1, 1, 0, s0, qargs, nullptr));
     }
     SbzSolveArgsHost a;
@@ -187,6 +188,8 @@ extern "C" int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const 
     const int waves = std::max(1, std::min(n_inst, opts->lanes > 0 ? opts->lanes : sbz_resident_waves(m, algo)));
     a.S = waves;
     a.srcbg = nullptr;
+    a.xcd_off = nullptr;   // one queue
+    a.n_xcd = 1;
     NCHK(sbz_launch_solve(m, algo, &a, (unsigned)waves));
     if (out_u) NCHK(hipMemcpyAsync(out_u, a.out_u, sizeof(double) * (size_t)n_inst * n, hipMemcpyDeviceToHost, s0));
     if (out_save && n_save > 0)

@@ -41,7 +41,9 @@ struct sbz_plan {
     int algo = SBZ_ALGO_WAVE;
     long long *d_koff = nullptr, *d_stats = nullptr;
     int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
-    unsigned int* d_queue = nullptr;
+    int* d_xcd_off = nullptr;   // per-XCD segments of d_order (null: one queue)
+    int n_xcd = 1;
+    unsigned int* d_queue = nullptr;   // one counter per queue
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
     // sbz_plan_run_graph: the run's launches (output fill, source background, queue reset, solve,
@@ -197,6 +199,7 @@ extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const doubl
 static void plan_free(sbz_plan* p) {
     void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src, p->d_ws,
                     p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue,
+                    p->d_xcd_off,
                     p->d_srcbg};
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -332,10 +335,29 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
         kn[s] = m->sets[s].nk;
         knots.insert(knots.end(), m->sets[s].knots.begin(), m->sets[s].knots.end());
     }
-    // longest-first order: estimated cost grows with k (step count ∝ k at high k)
+    // longest-first order: estimated cost grows with k (step count ∝ k at high k).  With several
+    // parameter sets and a wavefront kernel, the order is cut into one segment per XCD (the sets
+    // with set mod 8 = x), each longest first, so every XCD's L2 holds only its sets' knot tables.
     std::vector<int> order(n_inst);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst_k[a] > inst_k[b]; });
+    int used_sets = 0;
+    {
+        std::vector<char> seen(n_sets, 0);
+        for (int i = 0; i < n_inst; ++i)
+            if (!seen[inst_set[i]]) { seen[inst_set[i]] = 1; ++used_sets; }
+    }
+    p->n_xcd = (p->algo != SBZ_ALGO_LANE && used_sets >= SBZ_N_XCD) ? SBZ_N_XCD : 1;
+    std::vector<int> xcd_off(SBZ_N_XCD + 1, 0);
+    if (p->n_xcd > 1) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const int xa = inst_set[a] % SBZ_N_XCD, xb = inst_set[b] % SBZ_N_XCD;
+            return xa != xb ? xa < xb : inst_k[a] > inst_k[b];
+        });
+        for (int i = 0; i < n_inst; ++i) ++xcd_off[inst_set[i] % SBZ_N_XCD + 1];
+        for (int x = 0; x < SBZ_N_XCD; ++x) xcd_off[x + 1] += xcd_off[x];
+    } else {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst_k[a] > inst_k[b]; });
+    }
     hipError_t e = hipSuccess;
 #define UP(dst, src, n) \
     if (e == hipSuccess) e = upload(dst, src, n)
@@ -347,6 +369,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     UP(&p->d_k, inst_k, (size_t)n_inst);
     UP(&p->d_t0, inst_t0, (size_t)n_inst);
     UP(&p->d_order, order.data(), (size_t)n_inst);
+    if (p->n_xcd > 1) UP(&p->d_xcd_off, xcd_off.data(), xcd_off.size());
     UP(&p->d_saveat, saveat, (size_t)n_save * n_rows);
 #undef UP
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_out, sizeof(double) * std::max<size_t>((size_t)n_inst * m->n, 1));
@@ -359,7 +382,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_ret, sizeof(int) * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, sizeof(long long) * 6 * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_pk, sizeof(double) * std::max(n_inst, 1));
-    if (e == hipSuccess) e = hipMalloc((void**)&p->d_queue, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_queue, sizeof(unsigned int) * SBZ_N_XCD);
     if (e == hipSuccess && p->algo == SBZ_ALGO_LANE)
         e = hipMalloc((void**)&p->d_ws, sizeof(double) * (size_t)m->nw * lanes);
     if (e == hipSuccess) e = hipEventCreate(&p->ev0);
@@ -420,7 +443,8 @@ static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
                                   nullptr);
     }
     unsigned int* q = p->d_queue;
-    void* qargs[] = {&q};
+    int nq = p->n_xcd;
+    void* qargs[] = {&q, &nq};
     if (e == hipSuccess) e = hipModuleLaunchKernel(m->k_qreset, 1, 1, 1, 64, 1, 1, 0, m->stream, qargs, nullptr);
     SbzSolveArgsHost a;
     a.params = p->d_params;
@@ -447,6 +471,8 @@ static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
     a.ws = p->d_ws;
     a.S = p->lanes;
     a.srcbg = p->d_srcbg;
+    a.xcd_off = p->d_xcd_off;
+    a.n_xcd = p->n_xcd;
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev0, m->stream);
     if (e == hipSuccess) e = sbz_launch_solve(m, p->algo, &a, (unsigned)p->lanes);
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev1, m->stream);

@@ -228,6 +228,8 @@ extern "C" int emu_wave_solve(const double* params, const double* knots, const l
     a.queue = &queue;
     a.ws = nullptr;
     a.S = 1;
+    a.xcd_off = nullptr;
+    a.n_xcd = 1;
     std::vector<double> tab;
     if (g_srcbg && out_src && n_save > 0) {
         int n_sets = 1;
