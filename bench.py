@@ -254,7 +254,9 @@ def main():
                          "wavefront; lane: one lane per k-mode")
     ap.add_argument("--graph", action="store_true", help="replay each step from a captured HIP graph")
     ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="oracle baseline: every N-th instance (0: all for one set, every 16th for many sets)")
+                    help="oracle baseline: every N-th instance (0: all for one set, every 4th for many sets)")
+    ap.add_argument("--cpu-dense", action="store_true",
+                    help="oracle baseline with its dense path (J by unit vectors, partial-pivot LU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-side", action="store_true", help="skip the n = 100 side measurement")
     ap.add_argument("--no-ref", action="store_true", help="skip the rtol 1e-8 accuracy run")
@@ -363,7 +365,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
-        step = args.cpu_sample if args.cpu_sample > 0 else (16 if len(sets) > 1 else 1)
+        step = args.cpu_sample if args.cpu_sample > 0 else (4 if len(sets) > 1 else 1)
+        sparse = not args.cpu_dense
         sel = np.arange(0, n_modes, step)
         cores, cpu_model = host_cpu()
         tc = 0.0
@@ -376,7 +379,7 @@ def main():
             b = bgs[si]
             O = oracle.Oracle(M, b.P, b.t, b.u, b.du)
             t_c = time.perf_counter()
-            o = O.solve(ks[m], t0[m], nthreads=cores, saveat=taus, sources=taus is not None)
+            o = O.solve(ks[m], t0[m], nthreads=cores, saveat=taus, sources=taus is not None, sparse=sparse)
             tc += time.perf_counter() - t_c
             po[np.searchsorted(sel, m)] = [O.p0(k) * O.delta_m(k, b.t[-1], o["u"][i]) ** 2 for i, k in enumerate(ks[m])]
             if taus is not None:
@@ -391,11 +394,14 @@ def main():
             lo = ks[sel] <= kc
             sp = (f"; sources max |ΔS|/max|S| = {e[lo].max():.2e} over the {int(lo.sum())} modes with k <= {kc:g} H0/c "
                   f"(above it the late-time ISW term is tolerance-limited in both paths)")
+        how = ("sparse path: J by column coloring of the linear RHS (one background evaluation per step), "
+               "static-pattern LU in a minimum-degree order" if sparse else
+               "dense path: J by unit vectors, partial-pivot LU")
         cpu = {"value": len(sel) / tc, "unit": "k-mode solves/s", "cores": cores, "kind": "port",
-               "cpu": cpu_model, "nproc": os.cpu_count(),
+               "solver": "sparse" if sparse else "dense", "cpu": cpu_model, "nproc": os.cpu_count(),
                "sample": f"{'all' if step == 1 else f'every {step}th'} instance of the workload ({len(sel)} modes, "
                          f"{'with' if taus is not None else 'without'} the sources), {tc:.1f} s on {cores} host "
-                         f"threads ({cpu_model}); oracle: dense J by unit vectors and dense partial-pivot LU; "
+                         f"threads ({cpu_model}); oracle {how}, Rodas5P with the GPU path's controller; "
                          f"max |P_gpu/P_cpu - 1| = {parity:.2e}{sp}"}
         log(f"cpu baseline: {cpu['value']:.1f} solves/s, parity {parity:.2e}{sp}")
 

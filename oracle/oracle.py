@@ -57,6 +57,7 @@ def lib():
                                                   ctypes.c_long, _dp, ctypes.c_int, _dp, _dp, _ip, _i64p,
                                                   ctypes.c_int]
         L.ora_solve_sources.argtypes = L.ora_solve.argtypes + [_dp]
+        L.ora_solve_sparse.argtypes = L.ora_solve.argtypes + [_dp]
         _L = L
     return _L
 
@@ -141,7 +142,10 @@ class Oracle:
     def p0(self, k):
         return self.L.ora_p0(_p(self.P), k)
 
-    def solve(self, ks, tstart, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, nthreads=None, sources=False):
+    def solve(self, ks, tstart, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, nthreads=None, sources=False,
+              sparse=False):
+        """sparse: Jacobian by column coloring and a static-pattern sparse LU (the CPU baseline's
+        solver) instead of the dense J by unit vectors and the dense partial-pivot LU."""
         ks = np.ascontiguousarray(ks, dtype=np.float64)
         tstart = np.ascontiguousarray(np.broadcast_to(tstart, ks.shape), dtype=np.float64)
         nm = len(ks)
@@ -156,7 +160,8 @@ class Oracle:
         if nthreads is None:
             nthreads = os.cpu_count() or 1
         src = np.zeros((nm, ns, 3)) if (sources and ns) else None
-        self.L.ora_solve_sources(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), _p(ks), _p(tstart), nm, rtol,
+        fn = self.L.ora_solve_sparse if sparse else self.L.ora_solve_sources
+        fn(_p(self.md, ctypes.c_int), _p(self.P), *self._sp(), _p(ks), _p(tstart), nm, rtol,
                                  atol, maxiters, _p(saveat), ns, _p(usave), _p(ufinal), _p(ret, ctypes.c_int),
                                  _p(stats, ctypes.c_int64), nthreads, _p(src) if src is not None else None)
         return dict(u=ufinal, usave=usave if ns else None, retcode=ret, stats=stats, src=src)

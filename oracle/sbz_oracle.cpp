@@ -35,6 +35,7 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <set>
 #include <vector>
 
 namespace ora {
@@ -607,6 +608,7 @@ const double RD[9] = {0, 0.21193756319429014, -0.42387512638858027, -0.338462712
 struct Opts {
     double rtol, atol;
     long maxiters;
+    bool sparse = false;   // J by column coloring + static-pattern sparse LU (the CPU baseline path)
 };
 
 struct Stats {
@@ -623,6 +625,144 @@ double rms_scaled(int n, const double* x, const double* sk) {
 
 // Integrate one k-mode from t0 to t1.  saveat: sorted times in [t0, t1] at which the state is
 // stored by cubic Hermite interpolation over the accepted step that contains them.
+// ---------------------------------------------------------------- sparse path
+// The structure of the reference's default solver: an analytic sparse Jacobian (MTK `jac = true,
+// sparse = true`, src/solve.jl:126-127) factored by KLU (src/solve.jl:327-343).  Here: the pattern
+// of J is found once per mode; J is then assembled per step from one background evaluation and
+// one RHS application per column color (f is linear in u, so f(Σ_{j∈c} e_j) holds column j of J
+// in every row that only column j of color c reaches); W = I/(γh) - J is factored on its static
+// fill pattern in a minimum-degree elimination order, without pivoting, like KLU's numeric
+// refactorization on a fixed symbolic analysis.
+struct Sparse {
+    int n = 0, ncolor = 0;
+    std::vector<std::vector<int>> col_rows;    // rows of the J pattern in each column
+    std::vector<std::vector<int>> color_cols;  // columns of each color
+    std::vector<int> order;                    // elimination order (pivots)
+    std::vector<std::vector<int>> lower, upper;  // per elimination step: rows below, columns right
+};
+
+Sparse sparse_structure(const Model& m, const double* P, const Spline& s, double k, double t0, double t1) {
+    const int n = m.n;
+    Sparse sp;
+    sp.n = n;
+    std::vector<char> pat((size_t)n * n, 0);
+    std::vector<double> J((size_t)n * n);
+    const double ts[3] = {t0, std::sqrt(std::max(t0, 1e-300) * t1), t1};
+    for (double tau : ts) {
+        jacobian(m, P, s, k, tau, J.data());
+        for (size_t q = 0; q < J.size(); ++q)
+            if (J[q] != 0.0) pat[q] = 1;
+    }
+    sp.col_rows.assign(n, {});
+    std::vector<std::vector<int>> row_cols(n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (pat[(size_t)i * n + j]) {
+                sp.col_rows[j].push_back(i);
+                row_cols[i].push_back(j);
+            }
+    // greedy coloring: columns that share a row get different colors
+    std::vector<int> color(n, -1);
+    for (int j = 0; j < n; ++j) {
+        std::vector<char> used(n + 1, 0);
+        for (int i : sp.col_rows[j])
+            for (int c2 : row_cols[i])
+                if (color[c2] >= 0) used[color[c2]] = 1;
+        int c = 0;
+        while (used[c]) ++c;
+        color[j] = c;
+        sp.ncolor = std::max(sp.ncolor, c + 1);
+    }
+    sp.color_cols.assign(sp.ncolor, {});
+    for (int j = 0; j < n; ++j) sp.color_cols[color[j]].push_back(j);
+    // minimum-degree order on the symmetrized pattern, then the fill of elimination in that order
+    std::vector<std::set<int>> adj(n);
+    for (int i = 0; i < n; ++i)
+        for (int j : row_cols[i])
+            if (i != j) { adj[i].insert(j); adj[j].insert(i); }
+    std::vector<char> alive(n, 1);
+    for (int r = 0; r < n; ++r) {
+        int best = -1;
+        for (int v = 0; v < n; ++v)
+            if (alive[v] && (best < 0 || adj[v].size() < adj[best].size())) best = v;
+        sp.order.push_back(best);
+        alive[best] = 0;
+        for (int a : adj[best]) {
+            adj[a].erase(best);
+            for (int b : adj[best])
+                if (b != a) adj[a].insert(b);
+        }
+        adj[best].clear();
+    }
+    std::vector<int> pos(n);
+    for (int r = 0; r < n; ++r) pos[sp.order[r]] = r;
+    std::vector<std::set<int>> rows(n), cols(n);   // filled pattern by row and by column (W = diag + J)
+    for (int i = 0; i < n; ++i) {
+        rows[i].insert(i);
+        cols[i].insert(i);
+        for (int j : row_cols[i]) { rows[i].insert(j); cols[j].insert(i); }
+    }
+    for (int r = 0; r < n; ++r) {
+        const int p = sp.order[r];
+        std::vector<int> lo, up;
+        for (int i : cols[p]) if (pos[i] > r) lo.push_back(i);
+        for (int j : rows[p]) if (pos[j] > r) up.push_back(j);
+        for (int i : lo)
+            for (int j : up)
+                if (!rows[i].count(j)) { rows[i].insert(j); cols[j].insert(i); }
+        sp.lower.push_back(lo);
+        sp.upper.push_back(up);
+    }
+    return sp;
+}
+
+// J on the pattern at τ: one background evaluation, one RHS application per color.
+void jacobian_colored(const Model& m, const double* P, const Spline& s, const Sparse& sp, double k, double tau,
+                      double* J, double* e, double* col) {
+    const int n = m.n;
+    Bg<double> b;
+    background_at<double>(P, s, tau, b);
+    for (int c = 0; c < sp.ncolor; ++c) {
+        for (int j : sp.color_cols[c]) e[j] = 1.0;
+        rhs<double, double>(m, P, b, k, tau, e, col);
+        for (int j : sp.color_cols[c]) {
+            e[j] = 0.0;
+            for (int i : sp.col_rows[j]) J[(size_t)i * n + j] = col[i];
+        }
+    }
+}
+
+// W = I/(γh) - J factored in place on its fill pattern; diagonal entries hold the reciprocal pivots.
+bool sparse_lu(const Sparse& sp, double* W) {
+    const int n = sp.n;
+    for (int r = 0; r < n; ++r) {
+        const int p = sp.order[r];
+        const double piv = W[(size_t)p * n + p];
+        if (!(piv != 0.0) || !std::isfinite(piv)) return false;
+        const double rp = 1.0 / piv;
+        W[(size_t)p * n + p] = rp;
+        for (int i : sp.lower[r]) {
+            const double l = W[(size_t)i * n + p] * rp;
+            W[(size_t)i * n + p] = l;
+            for (int j : sp.upper[r]) W[(size_t)i * n + j] -= l * W[(size_t)p * n + j];
+        }
+    }
+    return true;
+}
+void sparse_solve(const Sparse& sp, const double* W, double* b) {
+    const int n = sp.n;
+    for (int r = 0; r < n; ++r) {
+        const int p = sp.order[r];
+        for (int i : sp.lower[r]) b[i] -= W[(size_t)i * n + p] * b[p];
+    }
+    for (int r = n - 1; r >= 0; --r) {
+        const int p = sp.order[r];
+        double acc = b[p];
+        for (int j : sp.upper[r]) acc -= W[(size_t)p * n + j] * b[j];
+        b[p] = acc * W[(size_t)p * n + p];
+    }
+}
+
 int solve_mode(const Model& m, const double* P, const Spline& s, double k, double t0, double t1, const Opts& o,
                const double* saveat, int nsave, double* usave, double* ufinal, Stats& st) {
     int n = m.n;
@@ -647,6 +787,14 @@ int solve_mode(const Model& m, const double* P, const Spline& s, double k, doubl
         std::copy(u.begin(), u.end(), ufinal);
         return RET_SUCCESS;
     }
+    Sparse sp;
+    std::vector<double> se, scol;
+    if (o.sparse) {
+        sp = sparse_structure(m, P, s, k, t0, t1);
+        se.assign(n, 0.0);
+        scol.assign(n, 0.0);
+        std::fill(J.begin(), J.end(), 0.0);
+    }
     // initial step (Hairer–Wanner, as OrdinaryDiffEqCore)
     f_eval(m, P, s, k, t, u.data(), f0.data());
     st.nf++;
@@ -670,14 +818,26 @@ int solve_mode(const Model& m, const double* P, const Spline& s, double k, doubl
         if (t + dt >= t1 - 1e-12 * std::fabs(t1)) dt = t1 - t;  // final step lands on t1
         else if (dt < 1e-14 * std::fabs(t)) { std::copy(u.begin(), u.end(), ufinal); return RET_DTMIN; }
         // Jacobian, time derivative and W = I/(γh) - J at (t, u)
-        jacobian(m, P, s, k, t, J.data());
+        if (o.sparse) jacobian_colored(m, P, s, sp, k, t, J.data(), se.data(), scol.data());
+        else jacobian(m, P, s, k, t, J.data());
         st.njac++;
         dfdt(m, P, s, k, t, u.data(), dT.data());
         if (!have_f0) { f_eval(m, P, s, k, t, u.data(), f0.data()); st.nf++; have_f0 = true; }
         double ig = 1.0 / (GAM * dt);
-        for (int i = 0; i < n * n; ++i) W[i] = -J[i];
-        for (int i = 0; i < n; ++i) W[i * n + i] += ig;
-        bool ok = lu_factor(n, W.data(), piv.data());
+        bool ok;
+        if (o.sparse) {   // W on the fill pattern (fill entries start at zero: J holds zeros there)
+            for (int r = 0; r < n; ++r) {
+                const int p = sp.order[r];
+                W[(size_t)p * n + p] = ig - J[(size_t)p * n + p];
+                for (int i : sp.lower[r]) W[(size_t)i * n + p] = -J[(size_t)i * n + p];
+                for (int j : sp.upper[r]) W[(size_t)p * n + j] = -J[(size_t)p * n + j];
+            }
+            ok = sparse_lu(sp, W.data());
+        } else {
+            for (int i = 0; i < n * n; ++i) W[i] = -J[i];
+            for (int i = 0; i < n; ++i) W[i * n + i] += ig;
+            ok = lu_factor(n, W.data(), piv.data());
+        }
         st.nlu++;
         if (ok) {
             for (int si = 1; si <= 8; ++si) {
@@ -698,7 +858,8 @@ int solve_mode(const Model& m, const double* P, const Spline& s, double k, doubl
                         rhsv[i] = acc;
                     }
                 }
-                lu_solve(n, W.data(), piv.data(), rhsv.data());
+                if (o.sparse) sparse_solve(sp, W.data(), rhsv.data());
+                else lu_solve(n, W.data(), piv.data(), rhsv.data());
                 st.nsolve++;
                 std::copy(rhsv.begin(), rhsv.end(), ks);
             }
@@ -849,13 +1010,37 @@ void ora_solve(const int* md, const double* P, const double* kt, const double* k
                       ufinal, retcode, stats, nthreads, nullptr);
 }
 
+static void solve_batch(const int* md, const double* P, const double* kt, const double* ku, const double* kdu,
+                        int nk, const double* ks, const double* tstart, int n_modes, double rtol, double atol,
+                        long maxiters, const double* saveat, int nsave, double* usave, double* ufinal, int* retcode,
+                        int64_t* stats, int nthreads, double* src, bool sparse);
+
 void ora_solve_sources(const int* md, const double* P, const double* kt, const double* ku, const double* kdu,
                        int nk, const double* ks, const double* tstart, int n_modes, double rtol, double atol,
                        long maxiters, const double* saveat, int nsave, double* usave, double* ufinal, int* retcode,
                        int64_t* stats, int nthreads, double* src) {
+    solve_batch(md, P, kt, ku, kdu, nk, ks, tstart, n_modes, rtol, atol, maxiters, saveat, nsave, usave, ufinal,
+                retcode, stats, nthreads, src, false);
+}
+
+// The same batch with the sparse path (colored J, static-pattern LU): the CPU baseline's solver.
+void ora_solve_sparse(const int* md, const double* P, const double* kt, const double* ku, const double* kdu, int nk,
+                      const double* ks, const double* tstart, int n_modes, double rtol, double atol, long maxiters,
+                      const double* saveat, int nsave, double* usave, double* ufinal, int* retcode, int64_t* stats,
+                      int nthreads, double* src) {
+    solve_batch(md, P, kt, ku, kdu, nk, ks, tstart, n_modes, rtol, atol, maxiters, saveat, nsave, usave, ufinal,
+                retcode, stats, nthreads, src, true);
+}
+}  // extern "C"
+
+static void solve_batch(const int* md, const double* P, const double* kt, const double* ku, const double* kdu,
+                        int nk, const double* ks, const double* tstart, int n_modes, double rtol, double atol,
+                        long maxiters, const double* saveat, int nsave, double* usave, double* ufinal, int* retcode,
+                        int64_t* stats, int nthreads, double* src, bool sparse) {
     Model m = make_model(md);
     Spline s{kt, ku, kdu, nk};
     Opts o{rtol, atol, maxiters};
+    o.sparse = sparse;
     double t1 = kt[nk - 1];
     std::atomic<int> next(0);
     auto worker = [&]() {
@@ -881,5 +1066,3 @@ void ora_solve_sources(const int* md, const double* P, const double* kt, const d
         for (auto& x : th) x.join();
     }
 }
-
-}  // extern "C"

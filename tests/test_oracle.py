@@ -12,7 +12,7 @@
 import numpy as np
 import pytest
 
-from conftest import GOLDEN
+from conftest import GOLDEN, Setup
 
 
 def test_class_pk_fixture(fixture5):
@@ -123,3 +123,25 @@ def test_visibility_derivatives_by_finite_differences(fixture5):
         chi = S.t[-1] - tau
         SE = O.sources(k, tau, u)[1]
         assert abs(SE / (3 / 16 * v / (k * chi) ** 2) - 1) < 1e-10
+
+
+def test_sparse_path_matches_dense(massless10):
+    """The oracle's sparse path (J by column coloring of the linear RHS, static-pattern LU in a
+    minimum-degree order without pivoting: the CPU baseline's solver) against its dense path
+    (J by unit vectors, partial-pivot LU): same states to rounding, same step counts (±1), and
+    the same CMB sources for k ≤ 3000 H0/c."""
+    from symboltz import workloads
+    for S in (massless10, Setup(*workloads.massive_nu100())):
+        ks = 10 ** np.linspace(-1, 4, 12)
+        t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+        saveat = np.linspace(0.05, S.t[-1], 7)
+        d = S.O.solve(ks, t0, saveat=saveat, sources=True)
+        s = S.O.solve(ks, t0, saveat=saveat, sources=True, sparse=True)
+        assert np.all(d["retcode"] == 0) and np.all(s["retcode"] == 0)
+        sc = np.abs(d["u"]).max(axis=1, keepdims=True)
+        assert np.max(np.abs(s["u"] - d["u"]) / sc) < 1e-6
+        assert np.max(np.abs(s["stats"][:, 0] - d["stats"][:, 0])) <= 1
+        lo = ks <= 3000.0   # above it the late-time ISW term amplifies rounding-level state differences
+        for c in range(3):
+            scs = np.abs(d["src"][lo, :, c]).max() + 1e-300
+            assert np.max(np.abs(s["src"][lo, :, c] - d["src"][lo, :, c])) <= 1e-5 * scs
