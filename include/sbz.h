@@ -159,6 +159,24 @@ int sbz_plan_destroy(sbz_plan* p);
  * autotuner (symboltz/autotune.py) uses it to decide whether higher occupancy can pay off. */
 int sbz_model_capacity(sbz_model* m, int algo, int* capacity);
 
+/* Tangent plans: forward-mode sensitivities of the perturbation solve.  Replaces ForwardDiff
+ * through solve/spectrum_matter (the reference's AD tests, test/runtests.jl:363-377, differentiate
+ * log P(k) with respect to log θ).  Each instance integrates u and s = ∂u/∂log θ together: the
+ * augmented linear system u' = J u, s' = J s + J_θ u on the primal's step sequence, one W
+ * factorization per step serving both blocks.  inst_setp / inst_setm name the uploaded parameter
+ * sets of log θ + h and log θ - h (backgrounds the caller solved at a tight tolerance) and
+ * inst_tscale = 1/(2h): the θ-dependence of the background enters as the central difference of
+ * those two sets at equal τ.  States only (no saveat); fails with SBZ_ERR_ARG for models whose
+ * tangent kernel does not fit a workgroup's LDS.  sbz_plan_run runs it. */
+int sbz_plan_create_tangent(sbz_model* m, const int* inst_set, const int* inst_setp, const int* inst_setm,
+                            const double* inst_tscale, const double* inst_k, const double* inst_t0, int n_inst,
+                            const sbz_opts* opts, sbz_plan** out);
+/* Tangents s = ∂u/∂log θ at τ0 [n_inst][n] and f(τ0, u(τ0)) [n_inst][n] (either may be NULL). */
+int sbz_plan_fetch_tangent(sbz_plan* p, double* out_s, double* out_fend);
+/* P(k, τ0) [n_inst] and d log P / d log θ [n_inst] of a tangent plan, including the shift of τ0
+ * with θ (spectrum_matter differentiated, src/observables/fourier.jl:79-101). */
+int sbz_plan_pk_tangent(sbz_plan* p, double* pk, double* dlogpk);
+
 /* One-shot: create a plan, run, fetch, destroy. */
 int sbz_solve_batch(sbz_model* m, const int* inst_set, const double* inst_k, const double* inst_t0, int n_inst,
                     const sbz_opts* opts, const double* saveat, int n_save, double* out_u, double* out_save,

@@ -24,6 +24,7 @@ struct sbz_model {
     hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
     hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
     hipFunction_t k_wave4 = nullptr;   // null when the model's border or chains exceed 16 lanes
+    hipFunction_t k_tan = nullptr, k_pk_tan = nullptr;   // null when the tangent kernel does not fit LDS
     hipStream_t stream = nullptr;
     int device = 0;
     std::vector<SbzSetTable> sets;
@@ -56,7 +57,15 @@ struct SbzSolveArgsHost {
     const double* srcbg;
     const int* xcd_off;
     int n_xcd;
+    const int* inst_setp;
+    const int* inst_setm;
+    const double* inst_tscale;
+    double* out_s;
+    double* out_fend;
 };
+
+// Kernel variant of tangent plans (not a caller-selectable algo: sbz_plan_create_tangent sets it).
+constexpr int SBZ_ALGO_TANGENT_INTERNAL = 100;
 
 // Work queues of a launch: one per XCD (instances grouped by set mod SBZ_N_XCD) when a plan has
 // several parameter sets, else one.

@@ -100,6 +100,13 @@ struct SbzSolveArgs {
                                // (sbz_srcbg_kernel), or null: the wave kernel then evaluates it per mode
     const int* xcd_off;        // [n_xcd + 1] segments of `order`, one work queue per XCD; null: one queue
     int n_xcd;                 // number of queues (queue[0..n_xcd) counters)
+    // tangent solves (sbz_wave_tan_kernel): per instance the parameter sets of log θ + h and
+    // log θ - h and 1/(2h); outputs s = ∂u/∂log θ at τ0 and f(τ0, u(τ0))
+    const int* inst_setp;
+    const int* inst_setm;
+    const double* inst_tscale;
+    double* out_s;             // [n_inst][SBZ_N]
+    double* out_fend;          // [n_inst][SBZ_N]
 };
 
 // Next position in `order` for a wavefront on XCD x (blockIdx mod n_xcd: workgroups are dispatched
@@ -1476,12 +1483,402 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
 }
 #endif
 
+// ============================================================================================
+// Tangent solve (SBZ_ALGO_TANGENT): u and s = ∂u/∂log θ integrated together, one mode per
+// wavefront.  The augmented system  u' = J u,  s' = J s + J_θ u  is linear with the block
+// lower-triangular Jacobian [[J, 0], [J_θ, J]], so every Rosenbrock stage needs W x = b twice
+// with the one factorization of W = I/(γh) - J:  W x_u = b_u,  W x_s = b_s + J_θ x_u.  J_θ v is
+// (f(v; B₊) - f(v; B₋)) / (2h), with B± the background at the same τ from the parameter sets of
+// log θ ± h (solved tightly by the host), and ∂(J_θ u)/∂τ is the same difference of ∂f/∂τ.  The
+// step sequence is the primal's: the error norm and the controller read u only, so u equals the
+// plain solve's and s is the derivative of the discrete solution along that step sequence.
+// States only: no saves or sources.
+// ============================================================================================
+#if SBZ_TAN_OK
+struct WaveTanLDS {
+    double v[15][SBZ_N];           // primal: 0,1 U; 2,3 F; 4 DT; 5 US; 6..13 K1..K8; 14 scratch
+    double w[15][SBZ_N];           // tangent, same slots
+    double B[3][7][SBZ_NB];        // background of the sets θ, θ+, θ- at the stage times (slot 6: spare)
+    double dB[3][SBZ_NB];          // their τ-derivatives at t_n
+    double Cs[SBZ_NC];
+    double M[3][SBZ_NBORD * SBZ_WSTRIDE];
+    double X[SBZ_NBORD];
+    double C[SBZ_NCHAIN];
+    int mode;
+};
+
+SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
+    __shared__ WaveTanLDS L;
+    WaveCtx c;
+    c.lane = threadIdx.x;
+    c.isB = c.lane < SBZ_NBORD;
+    c.bcomp = c.isB ? sbz_bord_comp[c.lane] : 0;
+    c.bchain = c.isB ? sbz_bord_chain[c.lane] : -1;
+    c.isC = c.lane < SBZ_NCHAIN;
+    const int ci = c.isC ? c.lane : 0;
+    c.ckind = sbz_chain_kind[ci];
+    c.cL = sbz_chain_L[ci];
+    c.cbin = sbz_chain_bin[ci];
+    c.catt = sbz_chain_att[ci];
+    c.cattc = sbz_chain_attc[ci];
+    c.cbase = sbz_chain_base[ci];
+    double* const DT = L.v[4];
+    double* const US = L.v[5];
+    double* const TMP = L.v[14];
+    double* const DTs = L.w[4];
+    double* const SS = L.w[5];
+    double* const TMPs = L.w[14];
+    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += 64) L.M[0][i] = 0.0;   // structural zeros
+    __syncthreads();
+    for (;;) {
+        if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
+        __syncthreads();
+        const int qi = L.mode;
+        __syncthreads();
+        if (qi < 0) break;
+        const int inst = a.order ? a.order[qi] : qi;
+        const int set0 = a.inst_set[inst], setp = a.inst_setp[inst], setm = a.inst_setm[inst];
+        const double tsc = a.inst_tscale[inst];   // 1 / (2h)
+        const double* P = a.params + (size_t)set0 * SBZ_NPAR;
+        const double* kt = a.knots + a.knot_off[set0];
+        const int nk = a.knot_n[set0];
+        // the table this lane evaluates at the stage times: lanes 0..5 θ, 6..11 θ + h, 12..17 θ - h
+        const int bs = (c.lane < 18) ? c.lane / 6 : 0;
+        const int bset = (bs == 0) ? set0 : (bs == 1) ? setp : setm;
+        const double* Pb = a.params + (size_t)bset * SBZ_NPAR;
+        const double* ktb = a.knots + a.knot_off[bset];
+        const int nkb = a.knot_n[bset];
+        const double k = a.inst_k[inst];
+        const double t1 = kt[nk - 1];
+        double t = a.inst_t0[inst];
+        int kidx = sbz_find_knot(kt, nk, t);
+        int bidx = sbz_find_knot(ktb, nkb, t);
+        int ub = 0, fb = 2;
+        long long st_acc = 0, st_rej = 0, st_f = 0, st_j = 0;
+        int ret = SBZ_RET_SUCCESS;
+        // the three backgrounds at the start time (slot 6), the constants
+        if (c.lane < 3) {
+            const int sq = (c.lane == 0) ? set0 : (c.lane == 1) ? setp : setm;
+            const double* Pq = a.params + (size_t)sq * SBZ_NPAR;
+            const double* ktq = a.knots + a.knot_off[sq];
+            const int nkq = a.knot_n[sq];
+            double Bx[SBZ_NB];
+            int kk = sbz_find_knot(ktq, nkq, t);
+            sbz_bg_plain(Pq, ktq, nkq, kk, t, Bx);
+#pragma unroll
+            for (int q = 0; q < SBZ_NB; ++q) L.B[c.lane][6][q] = Bx[q];
+            if (c.lane == 0) {
+                double Cs[SBZ_NC];
+                sbz_consts(P, Cs);
+                sbz_consts_k(Cs, k);
+#pragma unroll
+                for (int q = 0; q < SBZ_NC; ++q) L.Cs[q] = Cs[q];
+            }
+        }
+        __syncthreads();
+        // initial state u0 and its tangent s0 = (u0(θ + h) - u0(θ - h)) / (2h)
+        if (c.lane == 0) {
+            sbz_ic(L.B[0][6], L.Cs, k, L.v[0]);
+            sbz_ic(L.B[1][6], L.Cs, k, L.w[6]);
+            sbz_ic(L.B[2][6], L.Cs, k, L.w[7]);
+        }
+        __syncthreads();
+        {
+            bool fin = (k > 0.0) && (k == k);
+            for (int i = c.lane; i < SBZ_N; i += 64) {
+                L.w[0][i] = (L.w[6][i] - L.w[7][i]) * tsc;
+                const double v = L.v[0][i], ws = L.w[0][i];
+                if (!(v == v) || fabs(v) > 1e300 || !(ws == ws) || fabs(ws) > 1e300) fin = false;
+            }
+            if (!__all(fin)) ret = SBZ_RET_INITIALFAILURE;
+        }
+        for (int i = c.lane; i < SBZ_N; i += 64)
+#pragma unroll
+            for (int j = 6; j <= 13; ++j) {
+                L.v[j][i] = 0.0;
+                L.w[j][i] = 0.0;
+            }
+        __syncthreads();
+        // g(S, U) = f(S; B) + (f(U; B+) - f(U; B-)) / (2h), the tangent block's right-hand side
+        auto tangent_rhs = [&](const double* B0q, const double* Bpq, const double* Bmq, const double* Sin,
+                               const double* Uin, double* out) {
+            sbz_wave_apply(c, B0q, nullptr, L.Cs, k, Sin, out);
+            sbz_wave_apply(c, Bpq, nullptr, L.Cs, k, Uin, TMP);
+            sbz_wave_apply(c, Bmq, nullptr, L.Cs, k, Uin, TMPs);
+            __syncthreads();
+            for (int i = c.lane; i < SBZ_N; i += 64) out[i] += (TMP[i] - TMPs[i]) * tsc;
+            __syncthreads();
+        };
+        // f and g at the start: F0 = v[2], G0 = w[2]
+        sbz_wave_apply(c, L.B[0][6], nullptr, L.Cs, k, L.v[0], L.v[2]);
+        tangent_rhs(L.B[0][6], L.B[1][6], L.B[2][6], L.w[0], L.v[0], L.w[2]);
+        double dt = 0.0, qold = SBZ_CTL_QOLDINIT;
+        if (ret == SBZ_RET_SUCCESS && t1 > t) {
+            // Hairer–Wanner initial step from the primal, as the plain solve
+            double d0 = 0.0, d1 = 0.0;
+            for (int i = c.lane; i < SBZ_N; i += 64) {
+                const double skv = a.atol + fabs(L.v[0][i]) * a.rtol;
+                d0 += (L.v[0][i] / skv) * (L.v[0][i] / skv);
+                d1 += (L.v[2][i] / skv) * (L.v[2][i] / skv);
+            }
+            d0 = sqrt(sbz_wave_sum(d0) / SBZ_N);
+            d1 = sqrt(sbz_wave_sum(d1) / SBZ_N);
+            double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+            dt0 = fmin(dt0, t1 - t);
+            for (int i = c.lane; i < SBZ_N; i += 64) L.v[1][i] = L.v[0][i] + dt0 * L.v[2][i];
+            if (c.lane == 0) {
+                double Bx[SBZ_NB];
+                int kk = kidx;
+                sbz_bg_plain(P, kt, nk, kk, t + dt0, Bx);
+#pragma unroll
+                for (int q = 0; q < SBZ_NB; ++q) L.B[0][6][q] = Bx[q];
+            }
+            __syncthreads();
+            sbz_wave_apply(c, L.B[0][6], nullptr, L.Cs, k, L.v[1], L.v[3]);
+            __syncthreads();
+            double d2 = 0.0;
+            for (int i = c.lane; i < SBZ_N; i += 64) {
+                const double skv = a.atol + fabs(L.v[0][i]) * a.rtol;
+                const double r = (L.v[3][i] - L.v[2][i]) / skv;
+                d2 += r * r;
+            }
+            d2 = sqrt(sbz_wave_sum(d2) / SBZ_N) / dt0;
+            const double mx = fmax(d1, d2);
+            const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(0.01 / mx, 1.0 / 6.0);
+            dt = fmin(fmin(100.0 * dt0, dt1), t1 - t);
+            st_f += 2;
+        }
+        double row[SBZ_NBORD];
+        double cpl = 0.0;
+        SbzWb wb;
+        double cinv[SBZ_MCHAIN];
+        double csk = 0.0;
+        long long iters = 0;
+        // W x = b in place on the LDS vector b: chain forward sweeps, border solve, back-substitution
+        auto solve_W = [&](double* bv) {
+            if (c.isC) {
+                double y = 0.0;
+#define SBZ_CG_FWD(lo, hi, len) \
+                if (SBZ_IN_GROUP(lo, hi)) y = sbz_chain_fwd<len>(csk, cinv, bv + c.cbase);
+                SBZ_CHAIN_GROUPS(SBZ_CG_FWD)
+#undef SBZ_CG_FWD
+                L.C[c.lane] = y;
+            }
+            __syncthreads();
+            {
+                double b = c.isB ? bv[c.bcomp] : 0.0;
+                if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
+                sbz_wb_tri<SbzBcWave>(c.lane, L.M[1], L.M[2], b);
+                const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
+                const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
+                if (c.isB) {
+                    bv[c.bcomp] = x;
+                    L.X[c.lane] = x;
+                }
+            }
+            __syncthreads();
+            if (c.isC) {
+                const double xatt = L.X[c.catt];
+#define SBZ_CG_BWD(lo, hi, len) \
+                if (SBZ_IN_GROUP(lo, hi)) sbz_chain_bwd<len>(csk, cinv, xatt, bv + c.cbase);
+                SBZ_CHAIN_GROUPS(SBZ_CG_BWD)
+#undef SBZ_CG_BWD
+            }
+            __syncthreads();
+        };
+        for (;;) {
+            if (!(ret == SBZ_RET_SUCCESS && t < t1)) break;
+            double* const Un = L.v[ub];
+            double* const U1 = L.v[ub ^ 1];
+            double* const F0 = L.v[fb];
+            double* const F1 = L.v[fb ^ 1];
+            double* const Sn = L.w[ub];
+            double* const S1 = L.w[ub ^ 1];
+            double* const G0 = L.w[fb];
+            double* const G1 = L.w[fb ^ 1];
+            if (++iters > a.maxiters) { ret = SBZ_RET_MAXITERS; break; }
+            if (t + dt >= t1 - 1e-12 * fabs(t1)) dt = t1 - t;
+            else if (dt < 1e-14 * fabs(t)) { ret = SBZ_RET_DTLESSTHANMIN; break; }
+            const double ig = 1.0 / (SBZ_R5_GAMMA * dt);
+            const double idt = 1.0 / dt;
+            // the three backgrounds at the six stage times, lanes 0..17 (dual: d/dτ at t_n)
+            if (c.lane < 18) {
+                const int q = c.lane - 6 * bs;
+                const double cl = (q == 0) ? 0.0 : (q == 1) ? SBZ_R5_C2 : (q == 2) ? SBZ_R5_C3
+                                : (q == 3) ? SBZ_R5_C4 : (q == 4) ? SBZ_R5_C5 : 1.0;
+                double Bq[SBZ_NB], dBq[SBZ_NB];
+                sbz_bg_dual(Pb, ktb, nkb, bidx, t + cl * dt, Bq, dBq);
+#pragma unroll
+                for (int r = 0; r < SBZ_NB; ++r) L.B[bs][q][r] = Bq[r];
+                if (q == 0) {
+#pragma unroll
+                    for (int r = 0; r < SBZ_NB; ++r) L.dB[bs][r] = dBq[r];
+                }
+            }
+            __syncthreads();
+            const double* B0 = L.B[0][0];
+            if (c.isC) {
+                const SbzChainRow r = sbz_chain_row(c, B0, L.Cs, k);
+                csk = r.sk;
+                double xcs = 0.0;
+#define SBZ_CG_FACTOR(lo, hi, len) \
+                if (SBZ_IN_GROUP(lo, hi)) xcs = sbz_chain_factor<len>(r, ig, cinv);
+                SBZ_CHAIN_GROUPS(SBZ_CG_FACTOR)
+#undef SBZ_CG_FACTOR
+                L.C[c.lane] = xcs;
+            }
+            __syncthreads();
+            const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
+            double al, be;
+            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
+            double rpo;
+            bool ok = sbz_wb_lu<SbzBcWave>(c.lane, row, rpo);
+            sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
+            sbz_wb_tri2<SbzBcWave>(c.lane, L.M[1], L.M[2], al, be);
+            wb.z1 = c.isB ? al : 0.0;
+            wb.z2 = c.isB ? be : 0.0;
+            {
+                const double c11 = 1.0 - sbz_wave_sum(wb.p * wb.z1), c12 = -sbz_wave_sum(wb.p * wb.z2);
+                const double c21 = -sbz_wave_sum(wb.q * wb.z1), c22 = 1.0 - sbz_wave_sum(wb.q * wb.z2);
+                const double det = c11 * c22 - c12 * c21;
+                ok = ok && (det != 0.0) && (det == det) && (fabs(det) < 1e300);
+                const double idet = 1.0 / det;
+                wb.c11 = c22 * idet;
+                wb.c12 = -c12 * idet;
+                wb.c21 = -c21 * idet;
+                wb.c22 = c11 * idet;
+            }
+            ok = __all(ok || !c.isB);
+            // ∂f/∂τ of the primal, and of the tangent block: ∂f/∂τ(s) + (∂f/∂τ(u; θ+) - ∂f/∂τ(u; θ-)) / (2h)
+            sbz_wave_apply(c, B0, L.dB[0], L.Cs, k, Un, DT);
+            sbz_wave_apply(c, B0, L.dB[0], L.Cs, k, Sn, DTs);
+            sbz_wave_apply(c, L.B[1][0], L.dB[1], L.Cs, k, Un, TMP);
+            sbz_wave_apply(c, L.B[2][0], L.dB[2], L.Cs, k, Un, TMPs);
+            __syncthreads();
+            for (int i = c.lane; i < SBZ_N; i += 64) DTs[i] += (TMP[i] - TMPs[i]) * tsc;
+            st_j++;
+            __syncthreads();
+            double EEst = __builtin_inf();
+            if (ok) {
+                for (int s = 1; s <= 9; ++s) {
+                    double* Ks = (s <= 8) ? L.v[5 + s] : F1;
+                    double* KSs = (s <= 8) ? L.w[5 + s] : G1;
+                    if (s == 1) {
+                        for (int i = c.lane; i < SBZ_N; i += 64) {
+                            Ks[i] = F0[i] + dt * SBZ_R5_D1 * DT[i];
+                            KSs[i] = G0[i] + dt * SBZ_R5_D1 * DTs[i];
+                        }
+                    } else {
+                        double* inU = (s <= 8) ? US : U1;
+                        double* inS = (s <= 8) ? SS : S1;
+                        if (s <= 8) {
+                            SBZ_STAGE_SWITCH_LN(sbz_stage_in, 64, c.lane, L.v, Un, US)
+                            SBZ_STAGE_SWITCH_LN(sbz_stage_in, 64, c.lane, L.w, Sn, SS)
+                        }
+                        __syncthreads();
+                        const int bq = sbz_stage_bg[s];
+                        sbz_wave_apply(c, L.B[0][bq], nullptr, L.Cs, k, inU, Ks);
+                        tangent_rhs(L.B[0][bq], L.B[1][bq], L.B[2][bq], inS, inU, KSs);
+                        st_f++;
+                        if (s == 9) break;
+                        SBZ_STAGE_SWITCH_LN(sbz_stage_rhs, 64, c.lane, L.v, dt, idt, DT, Ks)
+                        SBZ_STAGE_SWITCH_LN(sbz_stage_rhs, 64, c.lane, L.w, dt, idt, DTs, KSs)
+                    }
+                    __syncthreads();
+                    solve_W(Ks);   // W x_u = b_u
+                    // W x_s = b_s + J_θ x_u, J_θ at t_n
+                    sbz_wave_apply(c, L.B[1][0], nullptr, L.Cs, k, Ks, TMP);
+                    sbz_wave_apply(c, L.B[2][0], nullptr, L.Cs, k, Ks, TMPs);
+                    __syncthreads();
+                    for (int i = c.lane; i < SBZ_N; i += 64) KSs[i] += (TMP[i] - TMPs[i]) * tsc;
+                    __syncthreads();
+                    solve_W(KSs);
+                    if (s == 8) {
+                        // u_{n+1}, s_{n+1}; the error estimate of the primal only
+                        double ss = 0.0;
+                        bool fin = true;
+                        for (int i = c.lane; i < SBZ_N; i += 64) {
+                            const double un1 = US[i] + Ks[i], sn1 = SS[i] + KSs[i];
+                            U1[i] = un1;
+                            S1[i] = sn1;
+                            const double scl = a.atol + fmax(fabs(Un[i]), fabs(un1)) * a.rtol;
+                            const double r = Ks[i] / scl;
+                            ss += r * r;
+                            if (!(un1 == un1) || fabs(un1) > 1e300 || !(sn1 == sn1) || fabs(sn1) > 1e300) fin = false;
+                        }
+                        ss = sbz_wave_sum(ss);
+                        EEst = __all(fin) ? sqrt(ss / SBZ_N) : __builtin_inf();
+                        if (!(EEst <= 1.0)) break;
+                    }
+                }
+            }
+            __syncthreads();
+            const double q11 = (EEst < __builtin_inf()) ? pow(EEst, SBZ_CTL_BETA1) : 1.0;
+            if (!(EEst <= 1.0)) {
+                const double q11r = (EEst < __builtin_inf()) ? q11 : 1.0 / SBZ_CTL_QMIN;
+                dt /= fmin(1.0 / SBZ_CTL_QMIN, q11r / SBZ_CTL_GAMMA);
+                st_rej++;
+                if (!(EEst < __builtin_inf())) {
+                    for (int i = c.lane; i < SBZ_N; i += 64)
+#pragma unroll
+                        for (int j = 6; j <= 13; ++j) {
+                            L.v[j][i] = 0.0;
+                            L.w[j][i] = 0.0;
+                        }
+                    __syncthreads();
+                }
+                continue;
+            }
+            const double q = (EEst == 0.0) ? 1.0 / SBZ_CTL_QMAX
+                                           : fmax(1.0 / SBZ_CTL_QMAX,
+                                                  fmin(1.0 / SBZ_CTL_QMIN, q11 / pow(qold, SBZ_CTL_BETA2) / SBZ_CTL_GAMMA));
+            qold = fmax(EEst, SBZ_CTL_QOLDINIT);
+            double tn = t + dt;
+            if (tn >= t1 - 1e-14 * fabs(t1)) tn = t1;
+            t = tn;
+            ub ^= 1;
+            fb ^= 1;
+            while (kidx < nk - 2 && t >= kt[kidx + 1]) ++kidx;
+            st_acc++;
+            dt = dt / q;
+        }
+        __syncthreads();
+        {
+            double* ou = a.out_u + (size_t)inst * SBZ_N;
+            double* os = a.out_s + (size_t)inst * SBZ_N;
+            double* of = a.out_fend + (size_t)inst * SBZ_N;
+            for (int i = c.lane; i < SBZ_N; i += 64) {
+                ou[i] = L.v[ub][i];
+                os[i] = L.w[ub][i];
+                of[i] = L.v[fb][i];   // f(t, u) at the end: the last pass 9, or the start's f
+            }
+        }
+        if (c.lane == 0) {
+            a.retcode[inst] = ret;
+            long long* st = a.stats + (size_t)inst * 6;
+            st[0] = st_acc;
+            st[1] = st_rej;
+            st[2] = st_f;
+            st[3] = st_j;
+            st[4] = st_j;
+            st[5] = 16 * st_j;
+        }
+        __syncthreads();
+    }
+}
+#endif
+
 // The same integrator compiled twice: for one resident wavefront per SIMD (all 512 registers
 // per lane, fewest spills: lowest latency per mode) and for two (half the registers, a few more
 // spills: the second wavefront hides the first's latency when there are more instances than
 // SIMDs).  SBZ_ALGO_WAVE / SBZ_ALGO_WAVE2 select them.
 extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body<true>(a); }
 extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body<false>(a); }
+
+#if SBZ_TAN_OK
+// Tangent solve, one mode and its parameter derivative per wavefront.
+extern "C" __global__ void __launch_bounds__(64) sbz_wave_tan_kernel(SbzSolveArgs a) { sbz_wave_tan_body(a); }
+#endif
 
 #if SBZ_WAVE4_OK
 // Four modes per wavefront, one resident wavefront per SIMD (all 512 registers per lane).
@@ -1536,6 +1933,48 @@ extern "C" __global__ void sbz_pk_kernel(const double* __restrict__ params, cons
     double dm = sbz_dm(B, Cs, k, u);
     double p0 = 2.0 * sbzc::PI * sbzc::PI / (k * k * k) * P[SBZ_P_AS] * pow(k / P[SBZ_P_KPIVOT], P[SBZ_P_NS] - 1.0);
     pk[i] = p0 * dm * dm;
+}
+
+// P(k, τ0) and d log P / d log θ of tangent solves (sbz_wave_tan_kernel's u, s, f(τ0)).  τ0 itself
+// depends on θ (a(τ0) = 1), so the total derivative of the final state is s + f(τ0) dτ0/dlog θ,
+// with dτ0/dlog θ from the τ0 of the sets θ ± h; and Δ_m (linear in u) reads the background at τ0
+// of each set:  dΔ_m/dlog θ = (Δ_m(u + h ṡ; B₊(τ0₊)) - Δ_m(u - h ṡ; B₋(τ0₋))) / (2h), ṡ the total
+// tangent.  d log P = 2 dΔ_m / Δ_m (P0 does not depend on θ for the parameters this serves).
+extern "C" __global__ void sbz_pk_tan_kernel(const double* __restrict__ params, const double* __restrict__ knots,
+                                             const long long* __restrict__ knot_off, const int* __restrict__ knot_n,
+                                             const int* __restrict__ inst_set, const int* __restrict__ inst_setp,
+                                             const int* __restrict__ inst_setm, const double* __restrict__ inst_tscale,
+                                             const double* __restrict__ inst_k, const double* __restrict__ u_final,
+                                             const double* __restrict__ s_final, const double* __restrict__ f_end,
+                                             int n_inst, double* __restrict__ pk, double* __restrict__ dlogpk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    const int sets[3] = {inst_set[i], inst_setp[i], inst_setm[i]};
+    const double k = inst_k[i], tsc = inst_tscale[i], h = 0.5 / tsc;
+    const double* P = params + (size_t)sets[0] * SBZ_NPAR;
+    double Cs[SBZ_NC];
+    sbz_consts(P, Cs);
+    CSV u{u_final + (size_t)i * SBZ_N, 1}, sv{s_final + (size_t)i * SBZ_N, 1}, fv{f_end + (size_t)i * SBZ_N, 1};
+    double dm[3][3], tau0[3];   // [set][u, s, f]
+    for (int q = 0; q < 3; ++q) {
+        const double* Pq = params + (size_t)sets[q] * SBZ_NPAR;
+        const double* kt = knots + knot_off[sets[q]];
+        const int nk = knot_n[sets[q]];
+        tau0[q] = kt[nk - 1];
+        double B[SBZ_NB];
+        int idx = nk - 2;
+        sbz_bg_plain(Pq, kt, nk, idx, kt[nk - 1], B);
+        dm[q][0] = sbz_dm(B, Cs, k, u);
+        dm[q][1] = sbz_dm(B, Cs, k, sv);
+        dm[q][2] = sbz_dm(B, Cs, k, fv);
+    }
+    const double dtau0 = (tau0[1] - tau0[2]) * tsc;
+    const double Dp = dm[1][0] + h * (dm[1][1] + dtau0 * dm[1][2]);
+    const double Dm = dm[2][0] - h * (dm[2][1] + dtau0 * dm[2][2]);
+    const double D0 = dm[0][0];
+    const double p0 = 2.0 * sbzc::PI * sbzc::PI / (k * k * k) * P[SBZ_P_AS] * pow(k / P[SBZ_P_KPIVOT], P[SBZ_P_NS] - 1.0);
+    pk[i] = p0 * D0 * D0;
+    dlogpk[i] = 2.0 * (Dp - Dm) * tsc / D0;
 }
 
 // P(k, τ) = P0(k) Δ_S(τ)² for species combinations S (bit masks: c, b, h) at τ0 from the final

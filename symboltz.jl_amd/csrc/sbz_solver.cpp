@@ -42,6 +42,9 @@ struct sbz_plan {
     long long *d_koff = nullptr, *d_stats = nullptr;
     int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
     int* d_xcd_off = nullptr;   // per-XCD segments of d_order (null: one queue)
+    // tangent plans: the sets of log θ ± h, 1/(2h), tangents and f(τ0) per instance, d log P
+    int *d_setp = nullptr, *d_setm = nullptr;
+    double *d_tscale = nullptr, *d_s = nullptr, *d_fend = nullptr, *d_dlogpk = nullptr;
     int n_xcd = 1;
     unsigned int* d_queue = nullptr;   // one counter per queue
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -152,6 +155,9 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         return SBZ_ERR_COMPILE;
     }
     if (hipModuleGetFunction(&m->k_wave4, m->module, "sbz_wave4_kernel") != hipSuccess) m->k_wave4 = nullptr;
+    if (hipModuleGetFunction(&m->k_tan, m->module, "sbz_wave_tan_kernel") != hipSuccess ||
+        hipModuleGetFunction(&m->k_pk_tan, m->module, "sbz_pk_tan_kernel") != hipSuccess)
+        m->k_tan = m->k_pk_tan = nullptr;
     (void)hipGetLastError();
     if (hipStreamCreate(&m->stream) != hipSuccess) {
         sbz_set_error("hipStreamCreate failed");
@@ -199,7 +205,7 @@ extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const doubl
 static void plan_free(sbz_plan* p) {
     void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src, p->d_ws,
                     p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue,
-                    p->d_xcd_off,
+                    p->d_xcd_off, p->d_setp, p->d_setm, p->d_tscale, p->d_s, p->d_fend, p->d_dlogpk,
                     p->d_srcbg};
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -218,17 +224,21 @@ static hipError_t upload(T** dst, const T* src, size_t n) {
 
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units) {
     void* kargs[] = {args};
-    hipFunction_t kf = (algo == SBZ_ALGO_LANE)    ? m->k_solve
-                       : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2
-                       : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4
-                                                  : m->k_wave;
+    hipFunction_t kf = (algo == SBZ_ALGO_LANE)              ? m->k_solve
+                       : (algo == SBZ_ALGO_WAVE2)           ? m->k_wave2
+                       : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
+                       : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
+                                                            : m->k_wave;
     const unsigned grid = (algo == SBZ_ALGO_LANE) ? units / 64 : units;
     return hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, kargs, nullptr);
 }
 
 int sbz_resident_waves(sbz_model* m, int algo) {
     int per_cu = 0, ncu = 0;
-    hipFunction_t kw = (algo == SBZ_ALGO_WAVE2) ? m->k_wave2 : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4 : m->k_wave;
+    hipFunction_t kw = (algo == SBZ_ALGO_WAVE2)             ? m->k_wave2
+                       : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
+                       : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
+                                                            : m->k_wave;
     if (!kw) return 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kw, 64, 0) != hipSuccess || per_cu < 1)
         per_cu = (algo == SBZ_ALGO_WAVE2) ? 8 : (algo == SBZ_ALGO_WAVE4) ? 2 : 4;
@@ -473,6 +483,11 @@ static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
     a.srcbg = p->d_srcbg;
     a.xcd_off = p->d_xcd_off;
     a.n_xcd = p->n_xcd;
+    a.inst_setp = p->d_setp;
+    a.inst_setm = p->d_setm;
+    a.inst_tscale = p->d_tscale;
+    a.out_s = p->d_s;
+    a.out_fend = p->d_fend;
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev0, m->stream);
     if (e == hipSuccess) e = sbz_launch_solve(m, p->algo, &a, (unsigned)p->lanes);
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev1, m->stream);
@@ -634,6 +649,102 @@ extern "C" int sbz_plan_destroy(sbz_plan* p) {
     if (!p) return SBZ_OK;
     plan_free(p);
     delete p;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_create_tangent(sbz_model* m, const int* inst_set, const int* inst_setp, const int* inst_setm,
+                                       const double* inst_tscale, const double* inst_k, const double* inst_t0,
+                                       int n_inst, const sbz_opts* opts, sbz_plan** out) {
+    sbz_clear_error();
+    if (!m || !inst_setp || !inst_setm || !inst_tscale || !opts || !out || n_inst < 0) {
+        sbz_set_error("sbz_plan_create_tangent: invalid argument");
+        return SBZ_ERR_ARG;
+    }
+    if (!m->k_tan) {
+        sbz_set_error("sbz_plan_create_tangent: the model's tangent kernel does not fit a workgroup's LDS");
+        return SBZ_ERR_ARG;
+    }
+    const int n_sets = (int)m->sets.size();
+    for (int i = 0; i < n_inst; ++i) {
+        const int sp = inst_setp[i], sm = inst_setm[i];
+        if (sp < 0 || sp >= n_sets || !m->sets[sp].valid || sm < 0 || sm >= n_sets || !m->sets[sm].valid) {
+            sbz_set_error("sbz_plan_create_tangent: instance %d refers to a perturbed parameter set without a "
+                          "background table", i);
+            return SBZ_ERR_ARG;
+        }
+        if (!(inst_tscale[i] > 0.0) || !std::isfinite(inst_tscale[i])) {
+            sbz_set_error("sbz_plan_create_tangent: instance %d has a non-positive 1/(2h)", i);
+            return SBZ_ERR_ARG;
+        }
+    }
+    sbz_opts o = *opts;
+    o.algo = SBZ_ALGO_WAVE;
+    o.save = 0;
+    sbz_plan* p = nullptr;
+    int st = sbz_plan_create(m, inst_set, inst_k, inst_t0, n_inst, &o, nullptr, 0, &p);
+    if (st != SBZ_OK) return st;
+    p->algo = SBZ_ALGO_TANGENT_INTERNAL;
+    if (opts->lanes <= 0) p->lanes = std::max(1, std::min(n_inst, sbz_resident_waves(m, p->algo)));
+    hipError_t e = hipSuccess;
+    const size_t nn = (size_t)std::max(n_inst, 1);
+    e = upload(&p->d_setp, inst_setp, (size_t)n_inst);
+    if (e == hipSuccess) e = upload(&p->d_setm, inst_setm, (size_t)n_inst);
+    if (e == hipSuccess) e = upload(&p->d_tscale, inst_tscale, (size_t)n_inst);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_s, sizeof(double) * nn * m->n);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_fend, sizeof(double) * nn * m->n);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_dlogpk, sizeof(double) * nn);
+    if (e != hipSuccess) {
+        sbz_set_error("sbz_plan_create_tangent: %s", hipGetErrorString(e));
+        sbz_plan_destroy(p);
+        return SBZ_ERR_HIP;
+    }
+    *out = p;
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_fetch_tangent(sbz_plan* p, double* out_s, double* out_fend) {
+    sbz_clear_error();
+    if (!p || !p->d_s) {
+        sbz_set_error("sbz_plan_fetch_tangent: not a tangent plan");
+        return SBZ_ERR_ARG;
+    }
+    const size_t b = sizeof(double) * (size_t)p->n_inst * p->m->n;
+    if (p->n_inst == 0) return SBZ_OK;
+    if (out_s) HIPCHK(hipMemcpyAsync(out_s, p->d_s, b, hipMemcpyDeviceToHost, p->m->stream));
+    if (out_fend) HIPCHK(hipMemcpyAsync(out_fend, p->d_fend, b, hipMemcpyDeviceToHost, p->m->stream));
+    HIPCHK(hipStreamSynchronize(p->m->stream));
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_pk_tangent(sbz_plan* p, double* pk, double* dlogpk) {
+    sbz_clear_error();
+    if (!p || !p->d_s || !pk || !dlogpk) {
+        sbz_set_error("sbz_plan_pk_tangent: not a tangent plan, or a null output");
+        return SBZ_ERR_ARG;
+    }
+    if (p->n_inst == 0) return SBZ_OK;
+    sbz_model* m = p->m;
+    const double* params = p->d_params;
+    const double* knots = p->d_knots;
+    const long long* koff = p->d_koff;
+    const int* kn = p->d_knotn;
+    const int* set = p->d_set;
+    const int* setp = p->d_setp;
+    const int* setm = p->d_setm;
+    const double* tsc = p->d_tscale;
+    const double* kk = p->d_k;
+    const double* uf = p->d_out;
+    const double* sf = p->d_s;
+    const double* ff = p->d_fend;
+    int n = p->n_inst;
+    double* dpk = p->d_pk;
+    double* ddl = p->d_dlogpk;
+    void* args[] = {&params, &knots, &koff, &kn, &set, &setp, &setm, &tsc, &kk, &uf, &sf, &ff, &n, &dpk, &ddl};
+    HIPCHK(hipModuleLaunchKernel(m->k_pk_tan, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, m->stream, args,
+                                 nullptr));
+    HIPCHK(hipMemcpyAsync(pk, p->d_pk, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipMemcpyAsync(dlogpk, p->d_dlogpk, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
     return SBZ_OK;
 }
 

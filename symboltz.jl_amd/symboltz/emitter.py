@@ -763,6 +763,10 @@ class Emitted:
         out = [f"#define SBZ_NBORD {nb}", f"#define SBZ_NCHAIN {len(chains)}", f"#define SBZ_MCHAIN {mch}",
                # four modes per wavefront (16 lanes each) when the border and the chains fit 16 lanes
                f"#define SBZ_WAVE4_OK {1 if (nb <= 16 and len(chains) <= 16) else 0}"]
+        # the tangent kernel when its LDS (primal and tangent vectors, three backgrounds, border
+        # staging and factors) fits one workgroup's 64 KB
+        tan_lds = 8 * (30 * n + 24 * NB + len(C_NAMES) + 3 * nb * (nb + 6) + nb + len(chains) + 1)
+        out.append(f"#define SBZ_TAN_OK {1 if tan_lds <= 65536 else 0}")
         out.append("#define SBZ_CHAIN_GROUPS(X) " + " ".join(f"X({a}, {b}, {n_})" for a, b, n_ in groups))
         out.append("__constant__ int sbz_bord_comp[SBZ_NBORD] = {" + ", ".join(map(str, bcomp)) + "};")
         for ch in chains:   # the kernel addresses chain members as base + (l - 3)

@@ -424,6 +424,49 @@ def spectrum_matter_jacobian(prob, ks, names, relstep=1e-3, ktini=1e-2, tauinima
     return J, pk[0]
 
 
+def spectrum_matter_tangent(prob, ks, names, hlog=1e-3, ktini=1e-2, tauinimax=1e-4, ptopts=None, bg_tol=1e-10,
+                            timing=None):
+    """∂ log P(k) / ∂ log θ by forward-mode sensitivities on the GPU: returns (J [n_k][n_θ], P(k)).
+
+    The reference differentiates `log ∘ spectrum_matter ∘ parameter_updater` with ForwardDiff
+    (test/runtests.jl:363-377).  Here every (k, θ) instance integrates the perturbations u together
+    with s = ∂u/∂log θ in one launch (sbz_plan_create_tangent): the augmented linear system on the
+    primal's adaptive step sequence.  The background enters through its θ-dependence at fixed τ,
+    the central difference of two background solves at log θ ± hlog with tolerance bg_tol (cheap:
+    5 unknowns per set), and the moving end time τ0(θ) is included in the P(k) derivative."""
+    names = list(names)
+    ks = _check_ks(ks)
+    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
+    base = dict(prob.pars)
+    bg0 = solvebg(prob)
+    shifted = []
+    for n in names:
+        x = math.log(base[n])
+        shifted += [dict(base, **{n: math.exp(x + hlog)}), dict(base, **{n: math.exp(x - hlog)})]
+    bgs = [solvebg(CosmologyProblem(prob.M, p, prob.ivspan), bg_tol, bg_tol) for p in shifted]
+    gm = prob.gpu
+    gm.upload_background(0, bg0.t, bg0.u, bg0.du, bg0.P)
+    for s_, bg in enumerate(bgs):
+        gm.upload_background(1 + s_, bg.t, bg.u, bg.du, bg.P)
+    nk, nt = len(ks), len(names)
+    t0 = np.clip(np.minimum(ktini / ks, tauinimax), bg0.t[0], bg0.t[-1])
+    j = np.repeat(np.arange(nt), nk)
+    plan = gm.plan_tangent(np.zeros(nk * nt, np.int32), 1 + 2 * j, 2 + 2 * j, np.full(nk * nt, 0.5 / hlog),
+                           np.tile(ks, nt), np.tile(t0, nt), rtol=ptopts["reltol"], atol=ptopts["abstol"],
+                           maxiters=ptopts["maxiters"])
+    try:
+        ms = plan.run()
+        if timing is not None:
+            timing["kernel_ms"] = ms
+        r = plan.fetch()
+        pk, d = plan.pk_tangent()
+    finally:
+        plan.close()
+    if not np.all(r["retcode"] == 0):
+        raise RuntimeError("a perturbation solve failed in the tangent launch")
+    return d.reshape(nt, nk).T, pk[:nk]
+
+
 def reionization_optical_depth(M, pars, reltol=1e-7, abstol=1e-7):
     """Optical depth of the reionized electrons: κ(τ0) with the tanh reionization terms minus κ(τ0)
     without them (both from full background solves; κ is the reference's _κ, baryons.jl:175-177).

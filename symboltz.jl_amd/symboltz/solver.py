@@ -130,6 +130,11 @@ class GpuModel:
         p.algo = algo
         return p
 
+    def plan_tangent(self, inst_set, setp, setm, tscale, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000):
+        """A tangent plan (sbz_plan_create_tangent): u and s = ∂u/∂log θ per instance, with the
+        parameter sets of log θ ± h (setp, setm) and 1/(2h) (tscale) per instance."""
+        return TangentPlan(self, inst_set, setp, setm, tscale, ks, t0, rtol, atol, maxiters)
+
     def close(self):
         if getattr(self, "h", None):
             self.L.sbz_model_destroy(self.h)
@@ -223,3 +228,47 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class TangentPlan(Plan):
+    """Instances integrated with their forward sensitivity s = ∂u/∂log θ (see include/sbz.h)."""
+
+    def __init__(self, gm, inst_set, setp, setm, tscale, ks, t0, rtol, atol, maxiters):
+        self.gm = gm
+        L = gm.L
+        ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        L.sbz_plan_create_tangent.argtypes = [ctypes.c_void_p, ip, ip, ip, dp, dp, dp, ctypes.c_int,
+                                              ctypes.POINTER(_Opts), ctypes.POINTER(ctypes.c_void_p)]
+        L.sbz_plan_fetch_tangent.argtypes = [ctypes.c_void_p, dp, dp]
+        L.sbz_plan_pk_tangent.argtypes = [ctypes.c_void_p, dp, dp]
+        self.set = np.ascontiguousarray(inst_set, dtype=np.int32)
+        self.setp = np.ascontiguousarray(setp, dtype=np.int32)
+        self.setm = np.ascontiguousarray(setm, dtype=np.int32)
+        self.tscale = np.ascontiguousarray(tscale, dtype=np.float64)
+        self.k = np.ascontiguousarray(ks, dtype=np.float64)
+        self.t0 = np.ascontiguousarray(t0, dtype=np.float64)
+        self.n_inst = len(self.k)
+        self.n_save = 0
+        self.save = 0
+        self.saveat = np.zeros(0)
+        self.algo = None
+        opts = _Opts(rtol, atol, int(maxiters), 0, 0, 0)
+        h = ctypes.c_void_p()
+        _lib.check(L.sbz_plan_create_tangent(gm.h, _lib.ptr(self.set, ctypes.c_int), _lib.ptr(self.setp, ctypes.c_int),
+                                             _lib.ptr(self.setm, ctypes.c_int), _lib.ptr(self.tscale), _lib.ptr(self.k),
+                                             _lib.ptr(self.t0), self.n_inst, ctypes.byref(opts), ctypes.byref(h)),
+                   "sbz_plan_create_tangent")
+        self.h = h
+
+    def fetch_tangent(self):
+        """(s [n_inst][n], f(τ0, u(τ0)) [n_inst][n])."""
+        n, m = self.n_inst, self.gm.n
+        s, f = np.zeros((n, m)), np.zeros((n, m))
+        _lib.check(self.gm.L.sbz_plan_fetch_tangent(self.h, _lib.ptr(s), _lib.ptr(f)), "sbz_plan_fetch_tangent")
+        return s, f
+
+    def pk_tangent(self):
+        """(P(k, τ0) [n_inst], d log P / d log θ [n_inst])."""
+        pk, d = np.zeros(self.n_inst), np.zeros(self.n_inst)
+        _lib.check(self.gm.L.sbz_plan_pk_tangent(self.h, _lib.ptr(pk), _lib.ptr(d)), "sbz_plan_pk_tangent")
+        return pk, d

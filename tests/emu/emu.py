@@ -81,3 +81,35 @@ class WaveEmu:
         if rc != 0:
             raise RuntimeError("wave kernel emulation: " + self.L.emu_error().decode())
         return dict(u=out_u, usave=out_save, src=out_src, retcode=ret, stats=st)
+
+    def solve_tangent(self, sets, ks, t0, tscale, rtol=1e-5, atol=1e-5, maxiters=100000):
+        """sbz_wave_tan_kernel on the CPU: sets = [(P, t, u, du)] for θ, θ + h, θ - h; returns u, s and
+        f(τ0, u) per instance."""
+        p = lambda a, ty=ctypes.c_double: a.ctypes.data_as(ctypes.POINTER(ty))
+        params = np.ascontiguousarray(np.stack([np.asarray(S[0], float) for S in sets]))
+        parts, koff, kn = [], [], []
+        off = 0
+        for P, t, u, du in sets:
+            kn.append(len(t))
+            koff.append(off)
+            part = np.concatenate([np.asarray(t, float), np.asarray(u, float).ravel(), np.asarray(du, float).ravel()])
+            parts.append(part)
+            off += len(part)
+        knots = np.ascontiguousarray(np.concatenate(parts))
+        koff = np.array(koff, dtype=np.int64)
+        kn = np.array(kn, dtype=np.int32)
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        t0 = np.ascontiguousarray(np.broadcast_to(t0, ks.shape), dtype=np.float64)
+        m = len(ks)
+        out_u, out_s, out_f = np.zeros((m, self.n)), np.zeros((m, self.n)), np.zeros((m, self.n))
+        ret = np.zeros(m, dtype=np.int32)
+        st = np.zeros((m, 6), dtype=np.int64)
+        L = self.L
+        L.emu_tan_solve.argtypes = [_dp, _dp, _llp, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_longlong, ctypes.c_double, _dp, _dp, _dp, _ip, _llp]
+        rc = L.emu_tan_solve(p(params), p(knots), p(koff, ctypes.c_longlong), p(kn, ctypes.c_int), p(ks), p(t0), m, rtol,
+                             atol, maxiters, tscale, p(out_u), p(out_s), p(out_f), p(ret, ctypes.c_int),
+                             p(st, ctypes.c_longlong))
+        if rc != 0:
+            raise RuntimeError("tangent kernel emulation: " + L.emu_error().decode())
+        return dict(u=out_u, s=out_s, fend=out_f, retcode=ret, stats=st)

@@ -242,3 +242,45 @@ extern "C" int emu_wave_solve(const double* params, const double* knots, const l
     }
     return emu::run(wave_body, &a);
 }
+
+#if SBZ_TAN_OK
+// Tangent solve (sbz_wave_tan_kernel) of n_inst instances of set 0 with the perturbed sets 1 (log θ + h)
+// and 2 (log θ - h): params [3][SBZ_NPAR], knots of the three sets at knot_off[0..2].
+static void tan_body(void* a) { sbz_wave_tan_kernel(*(SbzSolveArgs*)a); }
+extern "C" int emu_tan_solve(const double* params, const double* knots, const long long* knot_off, const int* knot_n,
+                             const double* inst_k, const double* inst_t0, int n_inst, double rtol, double atol,
+                             long long maxiters, double tscale, double* out_u, double* out_s, double* out_fend,
+                             int* retcode, long long* stats) {
+    unsigned queue = 0;
+    std::vector<int> set0(n_inst, 0), setp(n_inst, 1), setm(n_inst, 2);
+    std::vector<double> tsc(n_inst, tscale);
+    SbzSolveArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.params = params;
+    a.knots = knots;
+    a.knot_off = knot_off;
+    a.knot_n = knot_n;
+    a.inst_set = set0.data();
+    a.inst_setp = setp.data();
+    a.inst_setm = setm.data();
+    a.inst_tscale = tsc.data();
+    a.inst_k = inst_k;
+    a.inst_t0 = inst_t0;
+    a.n_inst = n_inst;
+    a.rtol = rtol;
+    a.atol = atol;
+    a.maxiters = maxiters;
+    a.out_u = out_u;
+    a.out_s = out_s;
+    a.out_fend = out_fend;
+    a.retcode = retcode;
+    a.stats = stats;
+    a.queue = &queue;
+    a.S = 1;
+    a.n_xcd = 1;
+    return emu::run(tan_body, &a);
+}
+extern "C" int emu_has_tan(void) { return 1; }
+#else
+extern "C" int emu_has_tan(void) { return 0; }
+#endif

@@ -116,3 +116,43 @@ def test_wave4_kernel_emulated_matches_oracle_and_wave(massless10):
     assert rel(g4["u"][good], o["u"]) < 1e-4
     assert rel(g4["u"][good], g1["u"][good]) < 1e-8
     assert np.all(np.abs(g4["stats"][good, 0] - g1["stats"][good, 0]) <= 2)
+
+
+def test_tangent_kernel_emulated_matches_oracle_differences(fixture5):
+    """sbz_wave_tan_kernel (u and s = ∂u/∂log θ on one step sequence, the background's θ-dependence
+    from two tight background solves at log θ ± h): u matches the plain kernel, and the total
+    tangent of the final state, s + f(τ0) dτ0/dlog θ, matches the central difference of oracle
+    solves at log θ ± h (rtol 1e-9) for θ = Ω_c0 and Ω_b0, the reference's AD test parameters
+    (test/runtests.jl:363-377)."""
+    from oracle import oracle
+    from symboltz import _lib
+    S = fixture5
+    M = S.M
+    E = WaveEmu(model_source(M).full_src)
+    assert E.L.emu_has_tan() == 1
+    ks = np.array([1.0, 30.0, 300.0])
+    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
+    h = 1e-3
+    g1 = E.solve(S.P, S.t, S.u, S.du, ks, t0)
+    for name in ("Omega_c0", "Omega_b0"):
+        x = np.log(S.pars[name])
+        tight = []
+        for sgn in (1, -1):
+            P = models.make_params(M, dict(S.pars, **{name: np.exp(x + sgn * h)}))
+            tight.append(_lib.background_solve(P, 1e-10, 1e-10, 100000)[:4])
+        g = E.solve_tangent([(S.P, S.t, S.u, S.du)] + tight, ks, t0, 0.5 / h)
+        assert np.all(g["retcode"] == 0)
+        assert rel(g["u"], g1["u"]) < 1e-8
+        assert np.all(g["stats"][:, 0] == g1["stats"][:, 0])
+        dtau0 = (tight[0][1][-1] - tight[1][1][-1]) / (2 * h)
+        s_tot = g["s"] + g["fend"] * dtau0
+        fd = []
+        for P, t, u, du in tight:
+            O = oracle.Oracle(M, P, t, u, du)
+            o = O.solve(ks, t0, rtol=1e-9, atol=1e-9, maxiters=10**6)
+            assert np.all(o["retcode"] == 0)
+            fd.append(o["u"])
+        fd = (fd[0] - fd[1]) / (2 * h)
+        err = np.max(np.abs(s_tot - fd) / np.abs(fd).max(axis=1, keepdims=True), axis=1)
+        print(name, err)
+        assert np.all(err < 1e-3), (name, err)
