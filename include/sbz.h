@@ -70,12 +70,11 @@ typedef struct sbz_plan sbz_plan;
 
 /* Model descriptor.  n: number of perturbation unknowns (layout: Φ, photon F_0..F_l, G_0..G_l,
  * massless ν F_0..F_l, massive ν ψ_i,0..l for each momentum bin, δ_c, θ_c, δ_b, θ_b[, δ_X, θ_X]);
- * nnz_lu: slots of the static LU pattern; nw: workspace doubles per lane; cache_path: code-object
- * file loaded if present and written after a hiprtc compile otherwise. */
+ * nnz_lu: slots of the static sparse LU pattern (the algorithmic flop count's); cache_path:
+ * code-object file loaded if present and written after a hiprtc compile otherwise. */
 typedef struct {
     int n;
     int nnz_lu;
-    int nw;
     const char* cache_path;
 } sbz_model_desc;
 
@@ -88,14 +87,13 @@ typedef struct {
 
 /* Kernel variants. */
 #define SBZ_ALGO_WAVE 1  /* one 64-lane wavefront per mode: chains in lanes, row-parallel border LU (default) */
-#define SBZ_ALGO_LANE 2  /* one lane per mode, state in a strided global workspace */
 #define SBZ_ALGO_WAVE2 3 /* SBZ_ALGO_WAVE compiled for two resident wavefronts per SIMD (throughput) */
 #define SBZ_ALGO_WAVE4 4 /* four modes per wavefront, 16 lanes each (throughput); models whose border and
                             chains fit 16 lanes (sbz_model_capacity reports 0 otherwise), no saves */
 
-/* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number
- * of concurrently resident modes (0: one per instance).  save: SBZ_SAVE_* flags (0 with saveat
- * given: states).  algo: SBZ_ALGO_* (0: default). */
+/* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number of
+ * resident wavefronts of the persistent grid (0: as many as fit the device).  save: SBZ_SAVE_* flags
+ * (0 with saveat given: states).  algo: SBZ_ALGO_* (0: default). */
 typedef struct {
     double rtol;
     double atol;
@@ -154,8 +152,8 @@ int sbz_plan_pk(sbz_plan* p, double* pk);
 int sbz_plan_pk_modes(sbz_plan* p, const int* masks, int n_masks, int at_saves, double* out);
 int sbz_plan_destroy(sbz_plan* p);
 /* Resident capacity of a kernel variant on the model's device: modes that run at once (one per
- * wavefront for SBZ_ALGO_WAVE and SBZ_ALGO_WAVE2, four per wavefront for SBZ_ALGO_WAVE4, one per
- * lane for SBZ_ALGO_LANE); 0 for SBZ_ALGO_WAVE4 when the model has no such kernel.  No reference counterpart: the
+ * wavefront for SBZ_ALGO_WAVE and SBZ_ALGO_WAVE2, four per wavefront for SBZ_ALGO_WAVE4), and 0 for
+ * SBZ_ALGO_WAVE4 when the model has no such kernel.  No reference counterpart: the
  * autotuner (symboltz/autotune.py) uses it to decide whether higher occupancy can pay off. */
 int sbz_model_capacity(sbz_model* m, int algo, int* capacity);
 
@@ -199,15 +197,6 @@ int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const double* ins
                             double* out_save, int* retcode, long long* stats);
 
 /* ---- line-of-sight integration ----------------------------------------------------------- */
-
-/* I_l(k) = Σ_τ w_τ S(τ, k) j_l(k(τ0 - τ)) for every k and tabulated l (los_integrate,
- * src/observables/angular.jl:109-185, non-Limber branch; trapezoid weights over τs, τ0 = τs[n_tau-1]).
- * S(iτ, ik) = S[ik*s_k + iτ*s_tau]; y/dy: j_l and j_l' on the uniform grid x_i = i dx, stored
- * [n_x][n_l] (SphericalBesselCache, angular.jl:9-48); out: [n_k][n_l].  Host buffers; the kernel
- * time is returned in *kernel_ms. */
-int sbz_los_integrate(const double* S, long long s_k, long long s_tau, const double* taus, int n_tau,
-                      const double* ks, int n_k, const double* y, const double* dy, int n_x, double dx, int n_l,
-                      double* out, float* kernel_ms);
 
 /* j_l(x) and j_l'(x) = l/(2l+1) j_{l-1} - (l+1)/(2l+1) j_{l+1} on x_i = i dx (i < n_x - 1) plus one
  * duplicate padding row, computed on the device and returned in y, dy [n_x][n_l]: the table of

@@ -19,9 +19,9 @@ struct SbzSetTable {
 };
 
 struct sbz_model {
-    int n = 0, nnz_lu = 0, nw = 0;
+    int n = 0, nnz_lu = 0;
     hipModule_t module = nullptr;
-    hipFunction_t k_solve = nullptr, k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
+    hipFunction_t k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
     hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
     hipFunction_t k_wave4 = nullptr;   // null when the model's border or chains exceed 16 lanes
     hipFunction_t k_tan = nullptr, k_pk_tan = nullptr;   // null when the tangent kernel does not fit LDS
@@ -52,7 +52,6 @@ struct SbzSolveArgsHost {
     int* retcode;
     long long* stats;
     unsigned int* queue;
-    double* ws;
     int S;
     const double* srcbg;
     const int* xcd_off;
@@ -72,7 +71,7 @@ constexpr int SBZ_ALGO_TANGENT_INTERNAL = 100;
 constexpr int SBZ_N_XCD = 8;
 
 // Launch the persistent solve kernel of `algo` with `args` on the model's stream: grid `units`
-// wavefronts (wave variants, four modes each for SBZ_ALGO_WAVE4) or lanes/64 blocks (lane variant).
+// wavefronts (four modes each for SBZ_ALGO_WAVE4).
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units);
 // Resident wavefronts of a wave variant on the model's device (the persistent grid size).
 int sbz_resident_waves(sbz_model* m, int algo);

@@ -1,15 +1,15 @@
 // Batched Rodas5P perturbation solver for MI355X (gfx950).
 //
-// One k-mode per lane.  A persistent grid pulls (parameter set, k) instances from a device work
-// queue ordered by estimated cost (longest first), so the 10²–10⁵ spread in step counts between
-// modes is balanced by lanes fetching new work as soon as their mode finishes.  Each lane runs
-// the full Rodas5P step (src/solve.jl:339 → OrdinaryDiffEqRosenbrock) with the model functions
-// emitted for this model (RHS, W = I/(γh) - J on the static sparse LU pattern, LU, solves,
-// ∂f/∂τ, ICs, Δ_m), spliced in at the @@SBZ_MODEL@@ marker by symboltz.emitter.
-//
-// Memory layout: every per-mode vector lives in a workspace array indexed [slot][lane], so a
-// wavefront's 64 lanes touch 64 consecutive doubles (fully coalesced 512 B); background spline
-// tables are read through the cache (one table per parameter set, shared by all its modes).
+// A persistent grid pulls (parameter set, k) instances from device work queues (one per XCD,
+// each ordered by estimated cost, longest first), so the 10²–10⁵ spread in step counts between
+// modes is balanced by wavefronts fetching new work as soon as their mode finishes.  Every kernel
+// runs the full Rodas5P step (src/solve.jl:339 → OrdinaryDiffEqRosenbrock) on the model functions
+// emitted for this model and spliced in at the @@SBZ_MODEL@@ marker by symboltz.emitter:
+//   sbz_wave_kernel, sbz_wave_kernel_o2   one mode per 64-lane wavefront (one / two per SIMD)
+//   sbz_wave4_kernel                      four modes per wavefront, 16 lanes each
+//   sbz_wave_tan_kernel                   one mode and its parameter tangent per wavefront
+// plus the source background table, P(k) and output-fill kernels.  Background spline tables (one
+// per parameter set, shared by all its modes) are read through the cache.
 #ifndef __HIPCC_RTC__  // hiprtc provides the HIP device headers itself
 #include <hip/hip_runtime.h>
 #endif
@@ -57,16 +57,8 @@ struct SbzBcRow16 {
 #error "model code missing"
 #endif
 
-// Views into the workspace.  Each wavefront owns a contiguous block laid out [slot][64 lanes]:
-// the lane stride is the compile-time constant 64, so every slot is a distinct constant offset the
-// compiler can disambiguate (no false store→load aliasing), and a wavefront's access to one slot
-// is 64 consecutive doubles (512 B, fully coalesced).
 #define SBZ_WAVE 64
-#define SBZ_OWN ((SBZ_N + 63) / 64)   // state components per lane in the wave kernel
-struct SV {
-    double* p;
-    SBZ_DEV double& operator[](int i) const { return p[i * SBZ_WAVE]; }
-};
+#define SBZ_OWN ((SBZ_N + 63) / 64)   // state components per lane in the one-mode-per-wavefront kernels
 struct CSV {
     const double* p;
     int s;
@@ -94,8 +86,7 @@ struct SbzSolveArgs {
     int* retcode;              // [n_inst]
     long long* stats;          // [n_inst][6]: accepted, rejected, f evals, J evals, LU, solves
     unsigned int* queue;       // work counter (zeroed by the host before launch)
-    double* ws;                // workspace: per wavefront [SBZ_NW][64]
-    int S;                     // number of lanes (multiple of 64)
+    int S;                     // number of resident wavefronts (the persistent grid)
     const double* srcbg;       // [n_sets][n_save][SBZ_SRCBG_STRIDE] background at the save times
                                // (sbz_srcbg_kernel), or null: the wave kernel then evaluates it per mode
     const int* xcd_off;        // [n_xcd + 1] segments of `order`, one work queue per XCD; null: one queue
@@ -130,17 +121,6 @@ SBZ_DEV int sbz_pull(const SbzSolveArgs& a, int x) {
 
 // Row of the per-set source background table: B[0..NB), dB/dτ[0..NB), d²κ̇/dτ² at 2 NB.
 #define SBZ_SRCBG_STRIDE 80
-
-// workspace slots (in units of SBZ_N doubles, then the LU)
-#define WS_U0 0
-#define WS_U1 1
-#define WS_F0 2
-#define WS_F1 3
-#define WS_DT 4
-#define WS_US 5
-#define WS_K1 6
-#define SBZ_NW (14 * SBZ_N + SBZ_NNZ_LU)
-#define WS_W (14 * SBZ_N)
 
 // Background quantities B[] for the perturbation equations at time τ (see symboltz.emitter
 // B_NAMES): splined unknowns a, X_H+, X_He+, ΔT, _κ through the cubic Hermite spline, then the
@@ -290,220 +270,6 @@ SBZ_DEV void sbz_consts_k(double* Cs, double k) {
     Cs[SBZ_C_IK2] = 1.0 / (k * k);
 }
 
-// Rodas5P tables in constant memory: the stage index is uniform across a wavefront, so these
-// are scalar loads.
-__constant__ double R5A[9][8] = {
-    {0}, {0}, {SBZ_R5_A21}, {SBZ_R5_A31, SBZ_R5_A32}, {SBZ_R5_A41, SBZ_R5_A42, SBZ_R5_A43},
-    {SBZ_R5_A51, SBZ_R5_A52, SBZ_R5_A53, SBZ_R5_A54},
-    {SBZ_R5_A61, SBZ_R5_A62, SBZ_R5_A63, SBZ_R5_A64, SBZ_R5_A65},
-    {SBZ_R5_A61, SBZ_R5_A62, SBZ_R5_A63, SBZ_R5_A64, SBZ_R5_A65, 1.0},
-    {SBZ_R5_A61, SBZ_R5_A62, SBZ_R5_A63, SBZ_R5_A64, SBZ_R5_A65, 1.0, 1.0}};
-__constant__ double R5C[9][8] = {
-    {0}, {0}, {SBZ_R5_C21}, {SBZ_R5_C31, SBZ_R5_C32}, {SBZ_R5_C41, SBZ_R5_C42, SBZ_R5_C43},
-    {SBZ_R5_C51, SBZ_R5_C52, SBZ_R5_C53, SBZ_R5_C54},
-    {SBZ_R5_C61, SBZ_R5_C62, SBZ_R5_C63, SBZ_R5_C64, SBZ_R5_C65},
-    {SBZ_R5_C71, SBZ_R5_C72, SBZ_R5_C73, SBZ_R5_C74, SBZ_R5_C75, SBZ_R5_C76},
-    {SBZ_R5_C81, SBZ_R5_C82, SBZ_R5_C83, SBZ_R5_C84, SBZ_R5_C85, SBZ_R5_C86, SBZ_R5_C87}};
-__constant__ double R5c[9] = {0, 0, SBZ_R5_C2, SBZ_R5_C3, SBZ_R5_C4, SBZ_R5_C5, 1.0, 1.0, 1.0};
-__constant__ double R5d[9] = {0, SBZ_R5_D1, SBZ_R5_D2, SBZ_R5_D3, SBZ_R5_D4, SBZ_R5_D5, 0, 0, 0};
-
-extern "C" __global__ void __launch_bounds__(64) sbz_rodas5p_kernel(SbzSolveArgs a) {
-    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.S) return;
-    // this wavefront's workspace block
-    double* wsl = a.ws + (size_t)(lane / SBZ_WAVE) * SBZ_NW * SBZ_WAVE + (lane % SBZ_WAVE);
-    auto vec = [&](int slot) { return SV{wsl + (size_t)slot * SBZ_N * SBZ_WAVE}; };
-    const SV W{wsl + (size_t)WS_W * SBZ_WAVE};
-
-
-    for (;;) {
-        const unsigned int qi = atomicAdd(a.queue, 1u);
-        if (qi >= (unsigned int)a.n_inst) break;
-        const int inst = a.order ? a.order[qi] : (int)qi;
-        const int set = a.inst_set[inst];
-        const double* saveat = a.saveat ? a.saveat + (size_t)set * a.saveat_stride : nullptr;
-        const double* P = a.params + (size_t)set * SBZ_NPAR;
-        const double* kt = a.knots + a.knot_off[set];
-        const int nk = a.knot_n[set];
-        const double k = a.inst_k[inst];
-        const double t1 = kt[nk - 1];
-        double t = a.inst_t0[inst];
-        double Cs[SBZ_NC];
-        sbz_consts(P, Cs);
-        double B[SBZ_NB], dB[SBZ_NB];
-        int kidx = sbz_find_knot(kt, nk, t);
-        long long st_acc = 0, st_rej = 0, st_f = 0, st_j = 0;
-        int ret = SBZ_RET_SUCCESS;
-        int isave = 0;
-        double* osave = a.out_save ? a.out_save + (size_t)inst * a.n_save * SBZ_N : nullptr;
-        int su = WS_U0, su1 = WS_U1, sf0 = WS_F0, sf1 = WS_F1;
-
-        // initial conditions
-        sbz_bg_plain(P, kt, nk, kidx, t, B);
-        {
-            SV u = vec(su);
-            sbz_ic(B, Cs, k, u);
-        }
-        bool finite0 = (k > 0.0) && (k == k);
-        for (int i = 0; i < SBZ_N; ++i) {
-            double v = vec(su)[i];
-            if (!(v == v) || fabs(v) > 1e300) finite0 = false;
-        }
-        if (!finite0) ret = SBZ_RET_INITIALFAILURE;
-        double* osrc = a.out_src ? a.out_src + (size_t)inst * a.n_save * 3 : nullptr;
-        while ((osave || osrc) && isave < a.n_save && saveat[isave] <= t) {
-            if (osave)
-                for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = vec(su)[i];
-            if (osrc && ret == SBZ_RET_SUCCESS) {
-                double S3[3];
-                sbz_sources_at(P, kt, nk, kidx, Cs, k, saveat[isave], vec(su), vec(WS_K1), vec(WS_K1 + 1),
-                               vec(WS_K1 + 2), S3);
-                for (int c = 0; c < 3; ++c) osrc[(size_t)isave * 3 + c] = S3[c];
-            }
-            ++isave;
-        }
-        double dt = 0.0, qold = SBZ_CTL_QOLDINIT;
-        if (ret == SBZ_RET_SUCCESS && t1 > t) {
-            // Hairer–Wanner initial step (OrdinaryDiffEqCore ode_determine_initdt)
-            SV u = vec(su), f0 = vec(sf0), u1 = vec(su1), f1 = vec(sf1);
-            sbz_f(B, Cs, k, u, f0);
-            double d0 = 0.0, d1 = 0.0;
-            for (int i = 0; i < SBZ_N; ++i) {
-                double sk = a.atol + fabs(u[i]) * a.rtol;
-                d0 += (u[i] / sk) * (u[i] / sk);
-                d1 += (f0[i] / sk) * (f0[i] / sk);
-            }
-            d0 = sqrt(d0 / SBZ_N);
-            d1 = sqrt(d1 / SBZ_N);
-            double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
-            dt0 = fmin(dt0, t1 - t);
-            for (int i = 0; i < SBZ_N; ++i) u1[i] = u[i] + dt0 * f0[i];
-            sbz_bg_plain(P, kt, nk, kidx, t + dt0, B);
-            sbz_f(B, Cs, k, u1, f1);
-            double d2 = 0.0;
-            for (int i = 0; i < SBZ_N; ++i) {
-                double sk = a.atol + fabs(u[i]) * a.rtol;
-                double r = (f1[i] - f0[i]) / sk;
-                d2 += r * r;
-            }
-            d2 = sqrt(d2 / SBZ_N) / dt0;
-            double mx = fmax(d1, d2);
-            double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(0.01 / mx, 1.0 / 6.0);
-            dt = fmin(fmin(100.0 * dt0, dt1), t1 - t);
-            st_f += 2;
-        }
-        long long iters = 0;
-        while (ret == SBZ_RET_SUCCESS && t < t1) {
-            if (++iters > a.maxiters) { ret = SBZ_RET_MAXITERS; break; }
-            if (t + dt >= t1 - 1e-12 * fabs(t1)) dt = t1 - t;
-            else if (dt < 1e-14 * fabs(t)) { ret = SBZ_RET_DTLESSTHANMIN; break; }
-            SV u = vec(su), u1 = vec(su1), f0 = vec(sf0), f1 = vec(sf1), dT = vec(WS_DT), us = vec(WS_US);
-            // J, ∂f/∂τ and W at (t, u)
-            sbz_bg_dual(P, kt, nk, kidx, t, B, dB);
-            const double ig = 1.0 / (SBZ_R5_GAMMA * dt);
-            sbz_w(B, Cs, k, ig, W);
-            bool ok = sbz_lu(W);
-            sbz_dfdt(B, dB, Cs, k, u, dT);
-            st_j++;
-            double EEst = __builtin_inf();
-            if (ok) {
-                double tprev = -1.0;
-                for (int s = 1; s <= 8; ++s) {
-                    SV ks = vec(WS_K1 + s - 1);
-                    if (s == 1) {
-                        for (int i = 0; i < SBZ_N; ++i) ks[i] = f0[i] + dt * SBZ_R5_D1 * dT[i];
-                    } else {
-                        for (int i = 0; i < SBZ_N; ++i) {
-                            double acc = u[i];
-                            for (int j = 1; j < s; ++j) acc += R5A[s][j - 1] * vec(WS_K1 + j - 1)[i];
-                            us[i] = acc;
-                        }
-                        const double ts = t + R5c[s] * dt;
-                        if (ts != tprev) {
-                            sbz_bg_plain(P, kt, nk, kidx, ts, B);
-                            tprev = ts;
-                        }
-                        sbz_f(B, Cs, k, us, ks);
-                        st_f++;
-                        const double dd = dt * R5d[s];
-                        for (int i = 0; i < SBZ_N; ++i) {
-                            double acc = ks[i] + dd * dT[i];
-                            for (int j = 1; j < s; ++j) acc += (R5C[s][j - 1] / dt) * vec(WS_K1 + j - 1)[i];
-                            ks[i] = acc;
-                        }
-                    }
-                    sbz_solve(W, ks);
-                }
-                SV k8 = vec(WS_K1 + 7);
-                double ss = 0.0;
-                bool fin = true;
-                for (int i = 0; i < SBZ_N; ++i) {
-                    double un = us[i] + k8[i];
-                    u1[i] = un;
-                    double sc = a.atol + fmax(fabs(u[i]), fabs(un)) * a.rtol;
-                    double r = k8[i] / sc;
-                    ss += r * r;
-                    if (!(un == un) || fabs(un) > 1e300) fin = false;
-                }
-                EEst = fin ? sqrt(ss / SBZ_N) : __builtin_inf();
-            }
-            if (!(EEst <= 1.0)) {
-                double q11 = (EEst < __builtin_inf()) ? pow(EEst, SBZ_CTL_BETA1) : 1.0 / SBZ_CTL_QMIN;
-                dt /= fmin(1.0 / SBZ_CTL_QMIN, q11 / SBZ_CTL_GAMMA);
-                st_rej++;
-                continue;
-            }
-            double q11 = pow(EEst, SBZ_CTL_BETA1);
-            double q = (EEst == 0.0) ? 1.0 / SBZ_CTL_QMAX
-                                     : fmax(1.0 / SBZ_CTL_QMAX,
-                                            fmin(1.0 / SBZ_CTL_QMIN, q11 / pow(qold, SBZ_CTL_BETA2) / SBZ_CTL_GAMMA));
-            qold = fmax(EEst, SBZ_CTL_QOLDINIT);
-            double tn = t + dt;
-            if (tn >= t1 - 1e-14 * fabs(t1)) tn = t1;
-            // f at the new point: next step's first stage and the Hermite dense output
-            sbz_bg_plain(P, kt, nk, kidx, tn, B);
-            sbz_f(B, Cs, k, u1, f1);
-            st_f++;
-            while ((osave || osrc) && isave < a.n_save && saveat[isave] <= tn) {
-                // cubic Hermite dense output over the accepted step
-                double th = (saveat[isave] - t) / dt;
-                double h00 = (1 + 2 * th) * (1 - th) * (1 - th), h10 = th * (1 - th) * (1 - th);
-                double h01 = th * th * (3 - 2 * th), h11 = th * th * (th - 1);
-                SV ys = vec(WS_K1);
-                for (int i = 0; i < SBZ_N; ++i) ys[i] = h00 * u[i] + h10 * dt * f0[i] + h01 * u1[i] + h11 * dt * f1[i];
-                if (osave)
-                    for (int i = 0; i < SBZ_N; ++i) osave[(size_t)isave * SBZ_N + i] = ys[i];
-                if (osrc) {
-                    double S3[3];
-                    sbz_sources_at(P, kt, nk, kidx, Cs, k, saveat[isave], ys, vec(WS_K1 + 1), vec(WS_K1 + 2),
-                                   vec(WS_K1 + 3), S3);
-                    for (int c = 0; c < 3; ++c) osrc[(size_t)isave * 3 + c] = S3[c];
-                }
-                ++isave;
-            }
-            t = tn;
-            int tmp = su; su = su1; su1 = tmp;
-            tmp = sf0; sf0 = sf1; sf1 = tmp;
-            st_acc++;
-            dt = dt / q;
-        }
-        {
-            SV u = vec(su);
-            double* o = a.out_u + (size_t)inst * SBZ_N;
-            for (int i = 0; i < SBZ_N; ++i) o[i] = u[i];
-        }
-        a.retcode[inst] = ret;
-        long long* st = a.stats + (size_t)inst * 6;
-        st[0] = st_acc;
-        st[1] = st_rej;
-        st[2] = st_f;
-        st[3] = st_j;
-        st[4] = st_j;
-        st[5] = 8 * st_j;
-    }
-}
-
-
 // ============================================================================================
 // Wavefront-per-mode Rodas5P kernel.
 //
@@ -645,7 +411,7 @@ SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB,
 
 
 // Rodas5P tableau as compile-time constants, so the stage combinations below fold them into
-// immediates (sbz_rodas5p.h; the __constant__ tables above stay for the lane-per-mode kernel).
+// immediates (sbz_rodas5p.h).
 struct SbzR5 {
     static constexpr double A[9][8] = {
         {0}, {0}, {SBZ_R5_A21}, {SBZ_R5_A31, SBZ_R5_A32}, {SBZ_R5_A41, SBZ_R5_A42, SBZ_R5_A43},

@@ -77,8 +77,8 @@ extern "C" int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const 
     }
     const int algo = opts->algo ? opts->algo : SBZ_ALGO_WAVE;
     if (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2) {
-        sbz_set_error("sbz_solve_batch_noalloc: only the wavefront kernels (SBZ_ALGO_WAVE, SBZ_ALGO_WAVE2) run "
-                      "without allocation (the lane kernel needs a per-lane workspace)");
+        sbz_set_error("sbz_solve_batch_noalloc: only the one-mode wavefront kernels (SBZ_ALGO_WAVE, SBZ_ALGO_WAVE2) "
+                      "run without allocation");
         return SBZ_ERR_ARG;
     }
     if (opts->save & ~SBZ_SAVE_STATES) {
@@ -184,7 +184,6 @@ extern "C" int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const 
     a.retcode = reinterpret_cast<int*>(base + o_ret);
     a.stats = reinterpret_cast<long long*>(base + o_stats);
     a.queue = reinterpret_cast<unsigned int*>(base + o_queue);
-    a.ws = nullptr;
     const int waves = std::max(1, std::min(n_inst, opts->lanes > 0 ? opts->lanes : sbz_resident_waves(m, algo)));
     a.S = waves;
     a.srcbg = nullptr;

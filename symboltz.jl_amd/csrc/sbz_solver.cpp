@@ -30,12 +30,12 @@ static constexpr int kSrcBgStride = 80;
 
 struct sbz_plan {
     sbz_model* m = nullptr;
-    int n_inst = 0, n_sets = 0, n_save = 0, saveat_stride = 0, lanes = 0, block = 64;
+    int n_inst = 0, n_sets = 0, n_save = 0, saveat_stride = 0, lanes = 0;
     double rtol = 1e-5, atol = 1e-5;
     long long maxiters = 100000;
     // device buffers
     double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
-    double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_ws = nullptr, *d_pk = nullptr;
+    double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_pk = nullptr;
     double* d_srcbg = nullptr;   // [n_sets][n_save] background rows for the wave kernels' sources
     int save_flags = 0;
     int algo = SBZ_ALGO_WAVE;
@@ -105,7 +105,7 @@ extern "C" int sbz_compile(const char* hip_src, const char* out_path) {
 
 extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc, sbz_model** out) {
     sbz_clear_error();
-    if (!desc || !out || desc->n <= 0 || desc->nw <= 0) {
+    if (!desc || !out || desc->n <= 0) {
         sbz_set_error("sbz_model_create: invalid descriptor");
         return SBZ_ERR_ARG;
     }
@@ -133,7 +133,6 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
     sbz_model* m = new sbz_model();
     m->n = desc->n;
     m->nnz_lu = desc->nnz_lu;
-    m->nw = desc->nw;
     (void)hipGetDevice(&m->device);
     hipError_t e = hipModuleLoadData(&m->module, code.data());
     if (e != hipSuccess) {
@@ -141,8 +140,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         delete m;
         return SBZ_ERR_COMPILE;
     }
-    if (hipModuleGetFunction(&m->k_solve, m->module, "sbz_rodas5p_kernel") != hipSuccess ||
-        hipModuleGetFunction(&m->k_wave, m->module, "sbz_wave_kernel") != hipSuccess ||
+    if (hipModuleGetFunction(&m->k_wave, m->module, "sbz_wave_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_wave2, m->module, "sbz_wave_kernel_o2") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk_modes, m->module, "sbz_pk_modes_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_srcbg, m->module, "sbz_srcbg_kernel") != hipSuccess ||
@@ -203,7 +201,7 @@ extern "C" int sbz_bg_upload(sbz_model* m, int set, const double* t, const doubl
 }
 
 static void plan_free(sbz_plan* p) {
-    void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src, p->d_ws,
+    void* bufs[] = {p->d_params, p->d_knots, p->d_k, p->d_t0, p->d_saveat, p->d_out, p->d_save, p->d_src,
                     p->d_pk, p->d_koff, p->d_stats, p->d_knotn, p->d_set, p->d_order, p->d_ret, p->d_queue,
                     p->d_xcd_off, p->d_setp, p->d_setm, p->d_tscale, p->d_s, p->d_fend, p->d_dlogpk,
                     p->d_srcbg};
@@ -224,13 +222,11 @@ static hipError_t upload(T** dst, const T* src, size_t n) {
 
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units) {
     void* kargs[] = {args};
-    hipFunction_t kf = (algo == SBZ_ALGO_LANE)              ? m->k_solve
-                       : (algo == SBZ_ALGO_WAVE2)           ? m->k_wave2
+    hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)             ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
                        : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
                                                             : m->k_wave;
-    const unsigned grid = (algo == SBZ_ALGO_LANE) ? units / 64 : units;
-    return hipModuleLaunchKernel(kf, grid, 1, 1, 64, 1, 1, 0, m->stream, kargs, nullptr);
+    return hipModuleLaunchKernel(kf, units, 1, 1, 64, 1, 1, 0, m->stream, kargs, nullptr);
 }
 
 int sbz_resident_waves(sbz_model* m, int algo) {
@@ -250,12 +246,11 @@ int sbz_resident_waves(sbz_model* m, int algo) {
 extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     sbz_clear_error();
     if (!m || !capacity ||
-        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_LANE && algo != SBZ_ALGO_WAVE4)) {
+        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_WAVE4)) {
         sbz_set_error("sbz_model_capacity: invalid argument");
         return SBZ_ERR_ARG;
     }
-    hipFunction_t kf = (algo == SBZ_ALGO_LANE)    ? m->k_solve
-                       : (algo == SBZ_ALGO_WAVE2) ? m->k_wave2
+    hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)   ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4
                                                   : m->k_wave;
     if (!kf) {
@@ -265,7 +260,7 @@ extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     int per_cu = 0, ncu = 0;
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 64, 0));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
-    *capacity = per_cu * ncu * (algo == SBZ_ALGO_LANE ? 64 : algo == SBZ_ALGO_WAVE4 ? 4 : 1);
+    *capacity = per_cu * ncu * (algo == SBZ_ALGO_WAVE4 ? 4 : 1);
     return SBZ_OK;
 }
 
@@ -310,8 +305,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->save_flags = n_save > 0 ? opts->save : 0;
     if (n_save > 0 && !(p->save_flags & (SBZ_SAVE_STATES | SBZ_SAVE_SOURCES))) p->save_flags |= SBZ_SAVE_STATES;
     p->saveat_stride = per_set ? n_save : 0;
-    if (opts->algo != 0 && opts->algo != SBZ_ALGO_LANE && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2 &&
-        opts->algo != SBZ_ALGO_WAVE4) {
+    if (opts->algo != 0 && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2 && opts->algo != SBZ_ALGO_WAVE4) {
         sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
         delete p;
         return SBZ_ERR_ARG;
@@ -323,17 +317,15 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
         return SBZ_ERR_ARG;
     }
     p->algo = opts->algo ? opts->algo : SBZ_ALGO_WAVE;
-    p->block = 64;
-    // lane-per-mode: `lanes` concurrent modes; wave-per-mode: `lanes` concurrent wavefronts
+    // `lanes`: concurrently resident wavefronts (the persistent grid)
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
-    if (opts->lanes <= 0 && p->algo != SBZ_ALGO_LANE) {
+    if (opts->lanes <= 0) {
         // persistent grid: as many wavefronts as can be resident at once, each pulling instances
         // (four at a time for SBZ_ALGO_WAVE4)
         const int need = (p->algo == SBZ_ALGO_WAVE4) ? (n_inst + 3) / 4 : n_inst;
         lanes = std::min(need, sbz_resident_waves(m, p->algo));
     }
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));
-    if (p->algo == SBZ_ALGO_LANE) lanes = (lanes + p->block - 1) / p->block * p->block;
     p->lanes = lanes;
     // concatenated background tables
     std::vector<double> params((size_t)n_sets * SBZ_NPAR), knots;
@@ -356,7 +348,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
         for (int i = 0; i < n_inst; ++i)
             if (!seen[inst_set[i]]) { seen[inst_set[i]] = 1; ++used_sets; }
     }
-    p->n_xcd = (p->algo != SBZ_ALGO_LANE && used_sets >= SBZ_N_XCD) ? SBZ_N_XCD : 1;
+    p->n_xcd = (used_sets >= SBZ_N_XCD) ? SBZ_N_XCD : 1;
     std::vector<int> xcd_off(SBZ_N_XCD + 1, 0);
     if (p->n_xcd > 1) {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -387,14 +379,12 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
         e = hipMalloc((void**)&p->d_save, sizeof(double) * (size_t)n_inst * n_save * m->n);
     if (e == hipSuccess && (p->save_flags & SBZ_SAVE_SOURCES))
         e = hipMalloc((void**)&p->d_src, sizeof(double) * (size_t)n_inst * n_save * 3);
-    if (e == hipSuccess && (p->save_flags & SBZ_SAVE_SOURCES) && p->algo != SBZ_ALGO_LANE)
+    if (e == hipSuccess && (p->save_flags & SBZ_SAVE_SOURCES))
         e = hipMalloc((void**)&p->d_srcbg, sizeof(double) * kSrcBgStride * (size_t)n_sets * n_save);
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_ret, sizeof(int) * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_stats, sizeof(long long) * 6 * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_pk, sizeof(double) * std::max(n_inst, 1));
     if (e == hipSuccess) e = hipMalloc((void**)&p->d_queue, sizeof(unsigned int) * SBZ_N_XCD);
-    if (e == hipSuccess && p->algo == SBZ_ALGO_LANE)
-        e = hipMalloc((void**)&p->d_ws, sizeof(double) * (size_t)m->nw * lanes);
     if (e == hipSuccess) e = hipEventCreate(&p->ev0);
     if (e == hipSuccess) e = hipEventCreate(&p->ev1);
     if (e != hipSuccess) {
@@ -478,7 +468,6 @@ static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
     a.retcode = p->d_ret;
     a.stats = p->d_stats;
     a.queue = p->d_queue;
-    a.ws = p->d_ws;
     a.S = p->lanes;
     a.srcbg = p->d_srcbg;
     a.xcd_off = p->d_xcd_off;

@@ -23,8 +23,8 @@ import numpy as np
 
 from . import _lib
 
-ALGO_WAVE, ALGO_LANE, ALGO_WAVE2, ALGO_WAVE4 = 1, 2, 3, 4
-NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4", ALGO_LANE: "LANE"}
+ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4 = 1, 3, 4
+NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4"}
 _cache = {}
 last_times = {}
 

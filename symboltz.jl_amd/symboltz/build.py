@@ -22,7 +22,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-f
 def build_library(verbose=False):
     LIBDIR.mkdir(exist_ok=True)
     out = LIBDIR / "libsbz_hip.so"
-    srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_los.hip", "sbz_cmb.hip",
+    srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_cmb.hip",
                                     "sbz_bgsolve.hip",
                                     "sbz_mlp.hip", "sbz_noalloc.hip")]
     cmd = ["hipcc", "-fPIC", "-shared", "-I", str(INCLUDE)] + HIP_FLAGS + srcs + ["-lhiprtc", "-o", str(out)]

@@ -54,7 +54,7 @@ class _Stored:
 
 
 class _Desc(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int), ("nnz_lu", ctypes.c_int), ("nw", ctypes.c_int), ("cache_path", ctypes.c_char_p)]
+    _fields_ = [("n", ctypes.c_int), ("nnz_lu", ctypes.c_int), ("cache_path", ctypes.c_char_p)]
 
 
 class _Opts(ctypes.Structure):
@@ -63,7 +63,6 @@ class _Opts(ctypes.Structure):
 
 
 ALGO_WAVE = 1
-ALGO_LANE = 2
 ALGO_WAVE2 = 3
 ALGO_WAVE4 = 4   # four modes per wavefront (models whose border and chains fit 16 lanes; no saves)
 ALGO_AUTO = -1   # host-side: symboltz.autotune picks among the wavefront variants
@@ -101,10 +100,9 @@ class GpuModel:
         em = model_source(M)
         self.em = em
         self.n = em.n
-        self.nw = 14 * em.n + em.nnz_LU
         KCACHE.mkdir(exist_ok=True)
         self.cache_path = KCACHE / f"{em.full_hash}.hsaco"
-        desc = _Desc(em.n, em.nnz_LU, self.nw, str(self.cache_path).encode())
+        desc = _Desc(em.n, em.nnz_LU, str(self.cache_path).encode())
         h = ctypes.c_void_p()
         _lib.check(self.L.sbz_model_create(em.full_src.encode(), ctypes.byref(desc), ctypes.byref(h)),
                    "sbz_model_create")
@@ -121,8 +119,8 @@ class GpuModel:
         self.n_sets = max(self.n_sets, set_index + 1)
 
     def plan(self, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, lanes=0, save=0, algo=0):
-        """algo: 0 (library default, one wavefront per SIMD), ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_LANE
-        or ALGO_AUTO (timed choice among the wavefront variants, symboltz.autotune)."""
+        """algo: 0 (library default, one wavefront per SIMD), ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4 or
+        ALGO_AUTO (timed choice among the variants, symboltz.autotune)."""
         if algo == ALGO_AUTO:
             from .autotune import choose
             algo = choose(self, inst_set, ks, t0, rtol, atol, maxiters, saveat, save)

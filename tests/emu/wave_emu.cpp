@@ -226,7 +226,6 @@ extern "C" int emu_wave_solve(const double* params, const double* knots, const l
     a.retcode = retcode;
     a.stats = stats;
     a.queue = &queue;
-    a.ws = nullptr;
     a.S = 1;
     a.xcd_off = nullptr;
     a.n_xcd = 1;

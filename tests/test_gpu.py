@@ -57,18 +57,6 @@ def test_fixture_model_matches_oracle(fixture5):
     assert np.max(np.abs(g["stats"][:, 0] - o["stats"][:, 0]) / o["stats"][:, 0]) < 0.05
 
 
-def test_lane_per_mode_kernel_matches_oracle(fixture5):
-    from symboltz.solver import ALGO_LANE
-    S = fixture5
-    say("case: lane-per-mode kernel, ΛCDM lmax=5, 8 modes")
-    ks = 10 ** np.linspace(np.log10(0.05), np.log10(2000.0), 8)
-    t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
-    g = gpu_solve(S, ks, t0, algo=ALGO_LANE)
-    o = S.O.solve(ks, t0)
-    assert np.all(g["retcode"] == 0)
-    assert rel_state_err(g["u"], o["u"]) < 1e-4
-
-
 def test_class_pk_fixture_on_gpu(fixture5):
     S = fixture5
     say("case: CLASS P(k) fixture, 516 modes")

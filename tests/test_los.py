@@ -86,23 +86,6 @@ def test_limber_limit_of_the_oracle():
 
 
 @pytest.mark.gpu
-def test_los_gpu_matches_oracle():
-    from symboltz.bessel import SphericalBesselCache
-    from symboltz.los import los_integrate
-    print("case: LOS 300 τ x 64 k x 40 l", flush=True)
-    rng = np.random.default_rng(1)
-    ls = np.unique(np.geomspace(2, 1500, 40).astype(int))
-    taus = np.sort(np.concatenate([[0.01], rng.uniform(0.01, 3.1, 298), [3.1]]))
-    ks = np.sort(rng.uniform(1.0, 500.0, 64))
-    xmax = max(20 * ls[-1], ks[-1] * 3.1 * 1.01)
-    jl = SphericalBesselCache(ls, xmax=xmax)
-    S = rng.standard_normal((len(taus), len(ks)))
-    g = los_integrate(S, ls, taus, ks, jl)
-    o = los_oracle(S, taus, ks, oc.bessel_table(ls, xmax=xmax))
-    assert np.max(np.abs(g - o)) <= 1e-12 * np.abs(o).max()
-
-
-@pytest.mark.gpu
 def test_gpu_bessel_table_matches_oracle():
     from symboltz.bessel import SphericalBesselCache
     print("case: Bessel table l ≤ 2500 on the GPU", flush=True)

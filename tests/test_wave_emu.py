@@ -1,6 +1,6 @@
 """The wavefront-per-mode kernel (sbz_wave_kernel) run on the CPU through the lane emulator
 (tests/emu) against the oracle: final states, saved states and CMB sources ≤ 1e-4 relative, and
-the failure return codes (k = 0, k = NaN give InitialFailure, as the lane-per-mode kernel)."""
+the failure return codes (k = 0, k = NaN give InitialFailure)."""
 import numpy as np
 import pytest
 
