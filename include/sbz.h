@@ -196,6 +196,30 @@ int sbz_solve_batch_noalloc(sbz_model* m, const int* inst_set, const double* ins
                             int n_inst, const sbz_opts* opts, const double* saveat, int n_save, double* out_u,
                             double* out_save, int* retcode, long long* stats);
 
+/* ---- mixed-model launches ---------------------------------------------------------------- */
+
+/* Several models in one persistent launch.  Replaces the per-model solve loops of a stream of
+ * requests over several cosmological models (solve(prob, ks) per CosmologyProblem,
+ * src/solve.jl:543-569, called once per model): the solves of every model run in one launch
+ * whose wavefronts pull (model, instance) entries from one queue ordered longest first across the
+ * models, so the whole batch has one tail.  No reference counterpart beyond that loop.
+ * hip_src is the mixed-model source (symboltz.multi: each model's device code in its own namespace
+ * and the kernel sbz_multi_kernel), loaded from cache_path when that file exists, else compiled
+ * with hiprtc and written there; models[0..n_models) are the model handles of its families, in its
+ * order (at most 8, one device; their state sizes are checked against the code object).
+ * Calls on one handle must not overlap. */
+typedef struct sbz_multi sbz_multi;
+int sbz_multi_create(const char* hip_src, const char* cache_path, sbz_model* const* models, int n_models,
+                     sbz_multi** out);
+/* Run plans[f] of models[f] for every family f together (one plan per model, in order; a plan may
+ * hold no instances).  Plans save no states or sources (final states only) and are not tangent
+ * plans.  Results land in each plan's buffers: sbz_plan_fetch and sbz_plan_pk read them as after
+ * sbz_plan_run.  *kernel_ms receives the launch's duration. */
+int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans, float* kernel_ms);
+/* Resident wavefronts of the mixed-model kernel (modes that run at once). */
+int sbz_multi_capacity(sbz_multi* mm, int* capacity);
+int sbz_multi_destroy(sbz_multi* mm);
+
 /* ---- line-of-sight integration ----------------------------------------------------------- */
 
 /* j_l(x) and j_l'(x) = l/(2l+1) j_{l-1} - (l+1)/(2l+1) j_{l+1} on x_i = i dx (i < n_x - 1) plus one

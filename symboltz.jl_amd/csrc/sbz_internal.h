@@ -30,6 +30,8 @@ struct sbz_model {
     std::vector<SbzSetTable> sets;
 };
 
+#include <cstddef>
+
 // Mirrors struct SbzSolveArgs of sbz_kernel.hip (passed by value as the kernel argument).
 struct SbzSolveArgsHost {
     const double* params;
@@ -75,3 +77,43 @@ constexpr int SBZ_N_XCD = 8;
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units);
 // Resident wavefronts of a wave variant on the model's device (the persistent grid size).
 int sbz_resident_waves(sbz_model* m, int algo);
+
+// A solve plan: instances, parameter tables and outputs resident on the device.
+struct sbz_plan {
+    sbz_model* m = nullptr;
+    int n_inst = 0, n_sets = 0, n_save = 0, saveat_stride = 0, lanes = 0;
+    double rtol = 1e-5, atol = 1e-5;
+    long long maxiters = 100000;
+    // device buffers
+    double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
+    double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_pk = nullptr;
+    double* d_srcbg = nullptr;   // [n_sets][n_save] background rows for the wave kernels' sources
+    int save_flags = 0;
+    int algo = SBZ_ALGO_WAVE;
+    long long *d_koff = nullptr, *d_stats = nullptr;
+    int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
+    int* d_xcd_off = nullptr;   // per-XCD segments of d_order (null: one queue)
+    // tangent plans: the sets of log θ ± h, 1/(2h), tangents and f(τ0) per instance, d log P
+    int *d_setp = nullptr, *d_setm = nullptr;
+    double *d_tscale = nullptr, *d_s = nullptr, *d_fend = nullptr, *d_dlogpk = nullptr;
+    int n_xcd = 1;
+    unsigned int* d_queue = nullptr;   // one counter per queue
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    // sbz_plan_run_graph: the run's launches (output fill, source background, queue reset, solve,
+    // P(k)) captured once into a graph and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool pk_ready = false;   // d_pk holds the P(k) of the last run (graph runs compute it)
+    // host copies of the processing order and the wavenumbers (mixed-model launches merge the
+    // orders of several plans by estimated cost)
+    std::vector<int> h_order;
+    std::vector<double> h_k;
+};
+
+// The kernel arguments of a plan's solve launch.
+void sbz_plan_args(const sbz_plan* p, SbzSolveArgsHost* out);
+// Read a whole file (false when missing or empty).
+bool sbz_read_file(const char* path, std::vector<char>& out);
+// Compile HIP source with hiprtc for the current device's architecture into a code object.
+int sbz_compile_hiprtc(const char* src, std::vector<char>& code);

@@ -510,11 +510,10 @@ SBZ_DEV double sbz_wave_sum(double v) {
     return (sbz_readlane(v, 0) + sbz_readlane(v, 16)) + (sbz_readlane(v, 32) + sbz_readlane(v, 48));
 }
 
-template <bool FUSED>
-SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
-    __shared__ WaveLDS L;
+// The lane roles of a 64-lane wavefront that integrates one mode.
+SBZ_DEV WaveCtx sbz_wave_ctx(int lane) {
     WaveCtx c;
-    c.lane = threadIdx.x;
+    c.lane = lane;
     c.isB = c.lane < SBZ_NBORD;
     c.bcomp = c.isB ? sbz_bord_comp[c.lane] : 0;
     c.bchain = c.isB ? sbz_bord_chain[c.lane] : -1;
@@ -526,19 +525,23 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     c.catt = sbz_chain_att[ci];
     c.cattc = sbz_chain_attc[ci];
     c.cbase = sbz_chain_base[ci];
+    return c;
+}
+
+// The border staging's structural zeros: the W assembly never writes them.  Once per workgroup
+// before its first mode (and again whenever the LDS held another model's layout meanwhile).
+SBZ_DEV void sbz_wave_lds_init(WaveLDS& L, int lane) {
+    for (int i = lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += 64) L.M[0][i] = 0.0;
+}
+
+// Integrate instance `inst` from its start to τ0 (saves and sources on the way) and write its
+// final state, return code and statistics; every lane of the wavefront calls it.
+template <bool FUSED>
+SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, int inst) {
     double* const DT = L.v[4];
     double* const US = L.v[5];
     double* const TMP = L.v[14];
-    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += 64) L.M[0][i] = 0.0;   // structural zeros, never written
-    __syncthreads();
-
-    for (;;) {
-        if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
-        __syncthreads();
-        const int qi = L.mode;
-        __syncthreads();
-        if (qi < 0) break;
-        const int inst = a.order ? a.order[qi] : qi;
+    {
         const int set = a.inst_set[inst];
         const double* saveat = a.saveat ? a.saveat + (size_t)set * a.saveat_stride : nullptr;
         const double* P = a.params + (size_t)set * SBZ_NPAR;
@@ -873,6 +876,22 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
             st[5] = 8 * st_j;
         }
         __syncthreads();
+    }
+}
+
+template <bool FUSED>
+SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
+    __shared__ WaveLDS L;
+    const WaveCtx c = sbz_wave_ctx(threadIdx.x);
+    sbz_wave_lds_init(L, c.lane);
+    __syncthreads();
+    for (;;) {
+        if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
+        __syncthreads();
+        const int qi = L.mode;
+        __syncthreads();
+        if (qi < 0) break;
+        sbz_wave_mode<FUSED>(a, L, c, a.order ? a.order[qi] : qi);
     }
 }
 
@@ -1634,6 +1653,9 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
 }
 #endif
 
+// The kernels of a single-model code object (a mixed-model code object compiles each model's
+// device code in its own namespace and defines its own kernel, sbz_multi_kernel).
+#ifndef SBZ_MULTI_FAMILY
 // The same integrator compiled twice: for one resident wavefront per SIMD (all 512 registers
 // per lane, fewest spills: lowest latency per mode) and for two (half the registers, a few more
 // spills: the second wavefront hides the first's latency when there are more instances than
@@ -1774,6 +1796,8 @@ extern "C" __global__ void sbz_pk_modes_kernel(const double* __restrict__ params
         out[((size_t)m * n_t + it) * n_inst + i] = p0 * d * d;
     }
 }
+
+#endif  // SBZ_MULTI_FAMILY
 
 #ifdef SBZ_HOST_PROBE
 // Host entry points into the emitted model functions, compiled only into the test probe library

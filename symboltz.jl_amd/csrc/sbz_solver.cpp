@@ -28,42 +28,15 @@
 // Row stride of the source background table (SBZ_SRCBG_STRIDE in sbz_kernel.hip).
 static constexpr int kSrcBgStride = 80;
 
-struct sbz_plan {
-    sbz_model* m = nullptr;
-    int n_inst = 0, n_sets = 0, n_save = 0, saveat_stride = 0, lanes = 0;
-    double rtol = 1e-5, atol = 1e-5;
-    long long maxiters = 100000;
-    // device buffers
-    double *d_params = nullptr, *d_knots = nullptr, *d_k = nullptr, *d_t0 = nullptr, *d_saveat = nullptr;
-    double *d_out = nullptr, *d_save = nullptr, *d_src = nullptr, *d_pk = nullptr;
-    double* d_srcbg = nullptr;   // [n_sets][n_save] background rows for the wave kernels' sources
-    int save_flags = 0;
-    int algo = SBZ_ALGO_WAVE;
-    long long *d_koff = nullptr, *d_stats = nullptr;
-    int *d_knotn = nullptr, *d_set = nullptr, *d_order = nullptr, *d_ret = nullptr;
-    int* d_xcd_off = nullptr;   // per-XCD segments of d_order (null: one queue)
-    // tangent plans: the sets of log θ ± h, 1/(2h), tangents and f(τ0) per instance, d log P
-    int *d_setp = nullptr, *d_setm = nullptr;
-    double *d_tscale = nullptr, *d_s = nullptr, *d_fend = nullptr, *d_dlogpk = nullptr;
-    int n_xcd = 1;
-    unsigned int* d_queue = nullptr;   // one counter per queue
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    float last_ms = 0.0f;
-    // sbz_plan_run_graph: the run's launches (output fill, source background, queue reset, solve,
-    // P(k)) captured once into a graph and replayed
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    bool pk_ready = false;   // d_pk holds the P(k) of the last run (graph runs compute it)
-};
 
-static bool read_file(const char* path, std::vector<char>& out) {
+bool sbz_read_file(const char* path, std::vector<char>& out) {
     std::ifstream f(path, std::ios::binary);
     if (!f) return false;
     out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
     return !out.empty();
 }
 
-static int compile_hiprtc(const char* src, std::vector<char>& code) {
+int sbz_compile_hiprtc(const char* src, std::vector<char>& code) {
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src, "sbz_model.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
         sbz_set_error("hiprtcCreateProgram failed");
@@ -96,7 +69,7 @@ extern "C" int sbz_compile(const char* hip_src, const char* out_path) {
         return SBZ_ERR_ARG;
     }
     std::vector<char> code;
-    int st = compile_hiprtc(hip_src, code);
+    int st = sbz_compile_hiprtc(hip_src, code);
     if (st != SBZ_OK) return st;
     std::ofstream f(out_path, std::ios::binary);
     f.write(code.data(), (std::streamsize)code.size());
@@ -116,14 +89,14 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         return SBZ_ERR_NODEV;
     }
     std::vector<char> code;
-    bool have = desc->cache_path && read_file(desc->cache_path, code);
+    bool have = desc->cache_path && sbz_read_file(desc->cache_path, code);
     if (!have) {
         if (!hip_src) {
             sbz_set_error("sbz_model_create: no cached code object at %s and no source",
                           desc->cache_path ? desc->cache_path : "(null)");
             return SBZ_ERR_COMPILE;
         }
-        int st = compile_hiprtc(hip_src, code);
+        int st = sbz_compile_hiprtc(hip_src, code);
         if (st != SBZ_OK) return st;
         if (desc->cache_path) {
             std::ofstream f(desc->cache_path, std::ios::binary);
@@ -360,6 +333,8 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     } else {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst_k[a] > inst_k[b]; });
     }
+    p->h_order = order;
+    p->h_k.assign(inst_k, inst_k + n_inst);
     hipError_t e = hipSuccess;
 #define UP(dst, src, n) \
     if (e == hipSuccess) e = upload(dst, src, n)
@@ -419,34 +394,9 @@ static hipError_t launch_pk(sbz_plan* p) {
     return hipModuleLaunchKernel(m->k_pk, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, m->stream, args, nullptr);
 }
 
-// The launches of one run on the model's stream: NaN into the save/source outputs, the source
-// background table, the queue reset and the persistent solve kernel (bracketed by ev0/ev1 when
-// `timed`), then the P(k) kernel when `with_pk`.  Used eagerly and under stream capture.
-static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
-    sbz_model* m = p->m;
-    hipError_t e = hipSuccess;
-    const double nan = __builtin_nan("");
-    if (p->d_save) e = launch_fill(m, p->d_save, (long long)p->n_inst * p->n_save * m->n, nan);
-    if (e == hipSuccess && p->d_src) e = launch_fill(m, p->d_src, (long long)p->n_inst * p->n_save * 3, nan);
-    if (e == hipSuccess && p->d_srcbg) {
-        // the background at every save time of every set, shared by all modes of the set
-        const double* params = p->d_params;
-        const double* knots = p->d_knots;
-        const long long* koff = p->d_koff;
-        const int* kn = p->d_knotn;
-        const double* sv = p->d_saveat;
-        int stride = p->saveat_stride, n_save = p->n_save;
-        long long n_rows = (long long)p->n_sets * p->n_save;
-        double* out = p->d_srcbg;
-        void* bargs[] = {&params, &knots, &koff, &kn, &sv, &stride, &n_save, &n_rows, &out};
-        e = hipModuleLaunchKernel(m->k_srcbg, (unsigned)((n_rows + 63) / 64), 1, 1, 64, 1, 1, 0, m->stream, bargs,
-                                  nullptr);
-    }
-    unsigned int* q = p->d_queue;
-    int nq = p->n_xcd;
-    void* qargs[] = {&q, &nq};
-    if (e == hipSuccess) e = hipModuleLaunchKernel(m->k_qreset, 1, 1, 1, 64, 1, 1, 0, m->stream, qargs, nullptr);
-    SbzSolveArgsHost a;
+// The kernel arguments of a plan's solve launch.
+void sbz_plan_args(const sbz_plan* p, SbzSolveArgsHost* out) {
+    SbzSolveArgsHost& a = *out;
     a.params = p->d_params;
     a.knots = p->d_knots;
     a.knot_off = p->d_koff;
@@ -477,6 +427,37 @@ static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
     a.inst_tscale = p->d_tscale;
     a.out_s = p->d_s;
     a.out_fend = p->d_fend;
+}
+
+// The launches of one run on the model's stream: NaN into the save/source outputs, the source
+// background table, the queue reset and the persistent solve kernel (bracketed by ev0/ev1 when
+// `timed`), then the P(k) kernel when `with_pk`.  Used eagerly and under stream capture.
+static hipError_t plan_enqueue(sbz_plan* p, bool timed, bool with_pk) {
+    sbz_model* m = p->m;
+    hipError_t e = hipSuccess;
+    const double nan = __builtin_nan("");
+    if (p->d_save) e = launch_fill(m, p->d_save, (long long)p->n_inst * p->n_save * m->n, nan);
+    if (e == hipSuccess && p->d_src) e = launch_fill(m, p->d_src, (long long)p->n_inst * p->n_save * 3, nan);
+    if (e == hipSuccess && p->d_srcbg) {
+        // the background at every save time of every set, shared by all modes of the set
+        const double* params = p->d_params;
+        const double* knots = p->d_knots;
+        const long long* koff = p->d_koff;
+        const int* kn = p->d_knotn;
+        const double* sv = p->d_saveat;
+        int stride = p->saveat_stride, n_save = p->n_save;
+        long long n_rows = (long long)p->n_sets * p->n_save;
+        double* out = p->d_srcbg;
+        void* bargs[] = {&params, &knots, &koff, &kn, &sv, &stride, &n_save, &n_rows, &out};
+        e = hipModuleLaunchKernel(m->k_srcbg, (unsigned)((n_rows + 63) / 64), 1, 1, 64, 1, 1, 0, m->stream, bargs,
+                                  nullptr);
+    }
+    unsigned int* q = p->d_queue;
+    int nq = p->n_xcd;
+    void* qargs[] = {&q, &nq};
+    if (e == hipSuccess) e = hipModuleLaunchKernel(m->k_qreset, 1, 1, 1, 64, 1, 1, 0, m->stream, qargs, nullptr);
+    SbzSolveArgsHost a;
+    sbz_plan_args(p, &a);
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev0, m->stream);
     if (e == hipSuccess) e = sbz_launch_solve(m, p->algo, &a, (unsigned)p->lanes);
     if (e == hipSuccess && timed) e = hipEventRecord(p->ev1, m->stream);

@@ -24,7 +24,7 @@ def build_library(verbose=False):
     out = LIBDIR / "libsbz_hip.so"
     srcs = [str(CSRC / f) for f in ("sbz_api.cpp", "sbz_background.cpp", "sbz_solver.cpp", "sbz_cmb.hip",
                                     "sbz_bgsolve.hip",
-                                    "sbz_mlp.hip", "sbz_noalloc.hip")]
+                                    "sbz_mlp.hip", "sbz_noalloc.hip", "sbz_multi.cpp")]
     cmd = ["hipcc", "-fPIC", "-shared", "-I", str(INCLUDE)] + HIP_FLAGS + srcs + ["-lhiprtc", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -80,10 +80,19 @@ def standard_models():
     ]
 
 
-def build_models(models=None, jobs=4, verbose=False):
+def standard_multis():
+    """Model lists compiled ahead of time into one mixed-model code object each (symboltz.multi)."""
+    from .workloads import mixed_families
+    return [mixed_families()]
+
+
+def build_models(models=None, jobs=4, verbose=False, with_multi=None):
     from concurrent.futures import ThreadPoolExecutor
     from .solver import model_source
-    models = standard_models() if models is None else models
+    standard = models is None
+    if with_multi is None:
+        with_multi = standard
+    models = standard_models() if standard else models
     KCACHE.mkdir(exist_ok=True)
     work = []
     for M in models:
@@ -91,13 +100,24 @@ def build_models(models=None, jobs=4, verbose=False):
         path = KCACHE / f"{em.full_hash}.hsaco"
         if not path.exists():
             work.append((em.full_src, path))
+    multis = []
+    if with_multi:   # the mixed-model code objects of the standard model lists
+        from .multi import multi_source
+        import hashlib
+        for fams in standard_multis():
+            src = multi_source(fams)
+            path = KCACHE / f"multi_{hashlib.sha1(src.encode()).hexdigest()[:16]}.hsaco"
+            multis.append(path.name)
+            if not path.exists():
+                work.append((src, path))
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(work) or 1))) as ex:
         for p in ex.map(lambda w: compile_code_object(*w), work):
             if verbose:
                 print("built", p)
     keep = {f"{model_source(M).full_hash}.hsaco" for M in models}
     keep |= {f"src_{getattr(model_source(M), 'tag', '')}.json" for M in models}
-    for f in KCACHE.iterdir():   # drop code objects and emitted sources of older kernel versions
+    keep |= set(multis)
+    for f in (KCACHE.iterdir() if standard else []):   # drop code objects and sources of older kernel versions
         if f.name not in keep and (f.suffix == ".hsaco" or f.name.startswith("src_")):
             f.unlink()
     return [KCACHE / f"{model_source(M).full_hash}.hsaco" for M in models]

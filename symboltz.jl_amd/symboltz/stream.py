@@ -1,0 +1,111 @@
+"""The mixed-model request stream: spectra for requests across several model families.
+
+Each request names a model family and its parameters.  The learned surrogate answers the requests
+of the family it was trained on when they lie inside its training box and its members agree
+within `tol` in log P; every other request (other families, out of the box, or disputed) is solved
+by the plain GPU solver, and all those solves, of every family and state size, run in one
+persistent launch (`symboltz.multi`): one queue, longest modes first across the families, one
+tail for the whole batch.
+
+No reference counterpart: the reference solves one cosmology of one model per call; this is the
+batched form of calling `solve(prob, ks)` and `spectrum_matter` for every request.
+"""
+import time
+
+import numpy as np
+
+from .constants import k_dimensionless
+from .multi import MultiModel
+from .problem import CosmologyProblem, solvebg_sets
+from .solver import GpuModel
+
+
+class MixedStream:
+    """families: model descriptions (at most 8); ks_mpc: the k grid of every spectrum (1/Mpc).
+    surrogate / surrogate_family: a trained `symboltz.surrogate.Surrogate` on this k grid and the
+    index of the family whose requests it may answer (None: every request is solved)."""
+
+    def __init__(self, families, ks_mpc, surrogate=None, surrogate_family=None, tol=2e-3, rtol=1e-5, atol=1e-5):
+        self.families = list(families)
+        self.ks_mpc = np.asarray(ks_mpc, dtype=np.float64)
+        self.surrogate = surrogate
+        self.sfam = surrogate_family
+        self.tol = tol
+        self.rtol, self.atol = rtol, atol
+        self.gms = [GpuModel(M) for M in self.families]
+        self.multi = MultiModel(self.gms)
+
+    def _plans(self, groups):
+        """One plan per family over its requests' sets × k (sets uploaded as 0..m-1)."""
+        plans, shapes = [], []
+        for f, pars in enumerate(groups):
+            gm, M = self.gms[f], self.families[f]
+            bgs = solvebg_sets([CosmologyProblem(M, p) for p in pars]) if pars else []
+            iset, kk, tt = [], [], []
+            for s, (p, bg) in enumerate(zip(pars, bgs)):
+                gm.upload_background(s, bg.t, bg.u, bg.du, bg.P)
+                k = k_dimensionless(self.ks_mpc, p["h"])
+                iset.append(np.full(len(k), s, np.int32))
+                kk.append(k)
+                tt.append(np.clip(np.minimum(1e-2 / k, 1e-4), bg.t[0], bg.t[-1]))
+            cat = (lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt))
+            plans.append(gm.plan(cat(iset, np.int32), cat(kk, np.float64), cat(tt, np.float64),
+                                 rtol=self.rtol, atol=self.atol))
+            shapes.append(len(pars))
+        return plans, shapes
+
+    def solve(self, requests, one_launch=True):
+        """requests: (family index, parameter dict, surrogate input row or None) per request.
+        Returns log P [n_req][n_k] (P in (c/H0)³), the mask of requests the surrogate answered,
+        the per-request return code (0: every mode succeeded) and timings in ms (surrogate
+        inference, background solves and uploads, solve kernel(s), P(k)).  one_launch=False runs
+        the families' plans one launch after the other instead (for comparison)."""
+        n, nk = len(requests), len(self.ks_mpc)
+        logp = np.full((n, nk), np.nan)
+        answered = np.zeros(n, bool)
+        ret = np.zeros(n, np.int32)
+        tm = {"surrogate_ms": 0.0, "prepare_ms": 0.0, "solve_ms": 0.0, "pk_ms": 0.0}
+        if self.surrogate is not None and self.sfam is not None:
+            idx = [i for i, (f, _, x) in enumerate(requests) if f == self.sfam and x is not None]
+            if idx:
+                X = np.array([requests[i][2] for i in idx], dtype=np.float64)
+                mean, dis, inbox, ms = self.surrogate.predict(X)
+                tm["surrogate_ms"] = ms
+                ok = inbox & (dis <= self.tol)
+                for j, i in enumerate(idx):
+                    if ok[j]:
+                        logp[i] = mean[j]
+                        answered[i] = True
+        groups = [[] for _ in self.families]
+        where = [[] for _ in self.families]
+        for i, (f, p, _) in enumerate(requests):
+            if not answered[i]:
+                groups[f].append(p)
+                where[f].append(i)
+        t0 = time.perf_counter()
+        plans, _ = self._plans(groups)
+        tm["prepare_ms"] = (time.perf_counter() - t0) * 1e3
+        try:
+            if one_launch:
+                tm["solve_ms"] = self.multi.run(plans)
+            else:
+                tm["solve_ms"] = sum(p.run() for p in plans if p.n_inst)
+            t0 = time.perf_counter()
+            for f, p in enumerate(plans):
+                if not p.n_inst:
+                    continue
+                pk = p.pk().reshape(len(where[f]), nk)
+                rc = p.fetch()["retcode"].reshape(len(where[f]), nk)
+                for j, i in enumerate(where[f]):
+                    logp[i] = np.log(pk[j])
+                    ret[i] = int(np.max(rc[j]))
+            tm["pk_ms"] = (time.perf_counter() - t0) * 1e3
+        finally:
+            for p in plans:
+                p.close()
+        return logp, answered, ret, tm
+
+    def close(self):
+        self.multi.close()
+        for gm in self.gms:
+            gm.close()

@@ -16,6 +16,11 @@ A_s = 2.1e-9, n_s = 0.966, hydrogen reionization at z = 7.7.
 
 The reference has no τ_reion parameter (reionization is fixed at z = 7.6711 in baryons.jl:202-203);
 z_re is this build's parameter, and τ_reion is converted by `problem.z_re_for_optical_depth`.
+
+- "mixed": the mixed-model stream (`mixed_families`, `mixed_requests`): requests spread over five
+  model families with 38 to 236 unknowns (ΛCDM with massless ν; three degenerate massive ν, the
+  surrogate's family; the w0wa fluid; one massive ν at lmax 16; three massive ν at lmax 32), each
+  with Latin-hypercube parameters and one k grid.
 """
 import math
 
@@ -86,3 +91,54 @@ def workload(name):
         desc = "LCDM massless nu lmax 10, n=38; 100 k log-spaced in [1e-4, 1]/Mpc; rtol=atol=1e-5"
         return M, [dict(BASE, Neff=3.046)], ks, desc
     raise ValueError(f"unknown workload {name}")
+
+
+def mixed_families():
+    """The model families of the mixed-model stream, by state size: ΛCDM with massless ν (lmax 10,
+    n = 38), ΛCDM with three degenerate massive ν (lmax 6, n = 54; the surrogate's family), w0waCDM
+    (lmax 6, n = 56), ΛCDM with one massive ν (photon/massless-ν lmax 16, lmax 10 per bin, n = 100)
+    and ΛCDM at lmax 32 (n = 236)."""
+    from . import models
+    return [models.LCDM(lmax=10, h=False), models.LCDM(lmax=6), models.w0waCDM(lmax=6),
+            models.LCDM(lmax_g=16, lmax_nu=16, lmax_h=10, nx=4), models.LCDM(lmax=32)]
+
+
+MIXED_SURROGATE_FAMILY = 1
+
+
+def mixed_requests(n, seed=0, out_of_box=0.0):
+    """n requests (family, parameters, surrogate input or None), families in turn.  Parameters from
+    one Latin hypercube over h, ω_b, ω_c, ln 10¹⁰A_s, n_s, z_re, Σm_ν ∈ [0.06, 0.30] eV, w0 ∈ [-1.2,
+    -0.8], wa ∈ [-0.5, 0.5]; w0wa draws whose w(a) = w0 + wa (1 - a) would cross -1 take -wa (the
+    fluid equations divide by 1 + w, dark_energy.jl:55-57).  A fraction `out_of_box` of the
+    surrogate family's requests get Σm_ν = 0.45 eV, outside the surrogate's box."""
+    from scipy.stats import qmc
+    from . import models
+    fams = mixed_families()
+    lo = np.array([0.60, 0.020, 0.10, 2.9, 0.92, 6.2, 0.06, -1.2, -0.5])
+    hi = np.array([0.75, 0.024, 0.14, 3.2, 1.00, 10.3, 0.30, -0.8, 0.5])
+    X = qmc.scale(qmc.LatinHypercube(d=len(lo), seed=seed).random(n), lo, hi)
+    rng = np.random.default_rng(seed)
+    out = []
+    for i, (h, wb, wc, la, ns, zre, mnu, w0, wa) in enumerate(X):
+        f = i % len(fams)
+        M = fams[f]
+        p = dict(models.parameters_Planck18(M), h=h, Omega_b0=wb / h**2, Omega_c0=wc / h**2, ln_As1e10=la, ns=ns,
+                 z_re=zre)
+        x = None
+        if M.de == "w0wa":
+            if (w0 + 1.0) * (w0 + wa + 1.0) < 0:
+                wa = -wa
+            p.update(w0=w0, wa=wa)
+        if M.has_h:
+            if f == MIXED_SURROGATE_FAMILY:
+                if rng.random() < out_of_box:
+                    mnu = 0.45
+                p.update(N=3, m_eV=mnu / 3)
+                x = np.array([h, wb, wc, la, ns, zre, mnu])
+            elif M.lmax_h == 10 and M.lmax_g == 16:
+                p.update(N=1, m_eV=mnu, Neff=2.046)
+            else:
+                p.update(N=3, m_eV=mnu / 3)
+        out.append((f, p, x))
+    return out
