@@ -116,6 +116,89 @@ def bench_bg(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mixed(args):
+    """Mixed-model stream: requests spread over five model families (n = 38, 54, 56, 100, 236;
+    symboltz.workloads.mixed_requests), every request solved: per step, the background solves of all
+    requests, one plan per family and one persistent launch over all families (sbz_multi_kernel),
+    then P(k) on 256 k in [1e-4, 5]/Mpc.  The surrogate, which answers the in-box requests of its
+    family in the full stream, is measured on its own (spectra/s of inference); here every request
+    takes the solver path, the stream's fallback.  The same requests are also run with one launch
+    per family, for the launch-tail comparison."""
+    from symboltz import surrogate as sg, workloads
+    from symboltz.constants import k_dimensionless
+    from symboltz.stream import MixedStream
+    fams = workloads.mixed_families()
+    ks = sg.k_grid()
+    reqs = workloads.mixed_requests(args.mixed_sets, seed=0)
+    st = MixedStream(fams, ks)
+    for w in range(args.warmup):
+        _, _, ret, tm = st.solve(reqs)
+        log(f"warmup {w}: one launch {tm['solve_ms']:.1f} ms")
+    tms = []
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        lp, _, ret, tm = st.solve(reqs)
+        tms.append(tm)
+        log(f"step {s}: prepare {tm['prepare_ms']:.1f} ms, one launch {tm['solve_ms']:.1f} ms, P(k) {tm['pk_ms']:.1f} ms")
+    elapsed = (time.perf_counter() - t_start) / args.steps
+    _, _, ret2, tm2 = st.solve(reqs, mode="serial")
+    _, _, ret3, tm3 = st.solve(reqs, mode="concurrent")
+    kern_ms = float(np.mean([t["solve_ms"] for t in tms]))
+    flops = tms[-1]["flops"]
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    ok = bool(np.all(ret == 0))
+    n_modes = len(reqs) * len(ks)
+    log(f"one launch {kern_ms:.1f} ms vs one launch per family {tm2['solve_ms']:.1f} ms (serial), "
+        f"{tm3['solve_ms']:.1f} ms (concurrent streams); all success {ok}; "
+        f"{achieved:.3f} TFLOP/s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        from symboltz.problem import CosmologyProblem, solvebg
+        cores, cpu_model = host_cpu()
+        tc, par, n_cpu = 0.0, 0.0, 0
+        for f in range(len(fams)):   # the first request of every family, all its k
+            i = next(j for j, r in enumerate(reqs) if r[0] == f)
+            M, p = fams[f], reqs[i][1]
+            b = solvebg(CosmologyProblem(M, p))
+            O = oracle.Oracle(M, b.P, b.t, b.u, b.du)
+            k = k_dimensionless(ks, p["h"])
+            t0 = np.clip(np.minimum(1e-2 / k, 1e-4), b.t[0], b.t[-1])
+            t_c = time.perf_counter()
+            o = O.solve(k, t0, nthreads=cores, sparse=True)
+            tc += time.perf_counter() - t_c
+            po = np.array([O.p0(kk) * O.delta_m(kk, b.t[-1], o["u"][j]) ** 2 for j, kk in enumerate(k)])
+            par = max(par, float(np.max(np.abs(np.exp(lp[i]) / po - 1))))
+            n_cpu += 1
+        cpu = {"value": n_cpu / tc, "unit": "spectra/s", "cores": cores, "kind": "port", "solver": "sparse",
+               "cpu": cpu_model, "nproc": os.cpu_count(),
+               "sample": f"the first request of each of the {len(fams)} families ({n_cpu} spectra x {len(ks)} k), "
+                         f"{tc:.1f} s on {cores} host threads ({cpu_model}); oracle sparse path; "
+                         f"max |P_gpu/P_cpu - 1| = {par:.2e}"}
+        log(f"cpu baseline: {cpu['value']:.3f} spectra/s, parity {par:.2e}")
+    out = {
+        "metric": "spectra/s of the mixed-model stream (every request solved, P(k) on 256 k)",
+        "value": len(reqs) / elapsed, "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "mixed",
+                   "description": f"{len(reqs)} requests over {len(fams)} model families "
+                                  f"(n = {[M.n_unknowns for M in fams]}), Latin-hypercube parameters (seed 0), "
+                                  f"256 k log-spaced in [1e-4, 5]/Mpc: {n_modes} k-modes in one launch; rtol=atol=1e-5",
+                   "k_mode_solves_per_s": n_modes / elapsed, "all_success": ok,
+                   "one_launch_kernel_ms": kern_ms, "per_family_launches_kernel_ms": tm2["solve_ms"],
+                   "per_family_concurrent_ms": tm3["solve_ms"],
+                   "stage_ms_per_step": {k: float(np.mean([t[k] for t in tms])) for k in
+                                         ("prepare_ms", "solve_ms", "pk_ms")}},
+        "roofline": {"bound": BOUND, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "sbz_multi_kernel",
+                     "kernel_ms": kern_ms, "flops_per_launch": flops, "steps_per_launch": tms[-1]["steps"]},
+        "cpu_baseline": cpu,
+    }
+    st.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_cmb(args):
     """C_l workload: spectrum_cmb([TT, EE, ψψ]) of the reference's CLASS-comparison model (ΛCDM
     lmax 5; test/runtests.jl:14-16,879-886) for a batch of cosmologies: set 0 is Planck18 (checked
@@ -246,7 +329,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massless10", "cmb", "bg"])
+    ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massless10", "cmb", "bg", "mixed"])
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--algo", choices=["auto", "wave", "wave2", "wave4"], default="auto",
                     help="auto: timed choice of the kernel variant (default); wave: one wavefront per "
@@ -264,6 +347,7 @@ def main():
     ap.add_argument("--n-tau", type=int, default=2000, help="points of the source time grid")
     ap.add_argument("--cmb-sets", type=int, default=16, help="cosmologies per step of the cmb workload")
     ap.add_argument("--bg-sets", type=int, default=16384, help="cosmologies per step of the bg workload")
+    ap.add_argument("--mixed-sets", type=int, default=320, help="requests per step of the mixed workload")
     ap.add_argument("--pmc-copy", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -292,6 +376,8 @@ def main():
         return bench_cmb(args)
     if args.workload == "bg":
         return bench_bg(args)
+    if args.workload == "mixed":
+        return bench_mixed(args)
     M, sets, ks_mpc, desc = workloads.workload(args.workload)
     gm = GpuModel(M)
     bgs, inst_set, ks, t0, t_bg = prepare(args, M, sets, ks_mpc, gm)

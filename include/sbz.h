@@ -126,6 +126,13 @@ int sbz_plan_create(sbz_model* m, const int* inst_set, const double* inst_k, con
 /* Launch the persistent solver kernel and wait; *kernel_ms receives its duration (HIP events on
  * the model's stream). */
 int sbz_plan_run(sbz_plan* p, float* kernel_ms);
+/* sbz_plan_run in two halves: sbz_plan_launch enqueues the run on the model's stream and returns at
+ * once; sbz_plan_wait waits for it and gives its duration.  Plans of different models (each model
+ * has its own stream) launched one after the other run concurrently on the device, the next
+ * kernel's wavefronts filling the CUs that the previous kernel's tail frees.  Replaces the same
+ * per-mode loop as sbz_plan_run (src/solve.jl:543-569). */
+int sbz_plan_launch(sbz_plan* p);
+int sbz_plan_wait(sbz_plan* p, float* kernel_ms);
 /* The same run as one HIP graph launch: on the first call the run's kernels (NaN fill of the
  * save/source outputs, source background table, queue reset, solve, and the P(k) reduction of
  * sbz_plan_pk) are captured from the model's stream into a graph, which every call then replays;

@@ -284,17 +284,31 @@ SBZ_DEV void sbz_consts_k(double* Cs, double k) {
 // ============================================================================================
 __constant__ int sbz_stage_bg[10] = {0, 0, 1, 2, 3, 4, 5, 5, 5, 5};  // stage -> background slot
 
+// The border staging (A, α, β, p, q) lives in the stage vectors K1..K8 when it fits there: they
+// are dead while W is assembled at the start of a step attempt, and the staging is dead once the
+// rows are loaded (its loads mask every entry the assembly does not write).
+#define SBZ_M0_IN_K (SBZ_NBORD * SBZ_WSTRIDE <= 8 * SBZ_N)
 struct WaveLDS {
-    double v[15][SBZ_N];           // 0,1 U (ping-pong), 2,3 F (ping-pong), 4 DT, 5 US, 6..13 K1..K8, 14 tmp
+    double v[14][SBZ_N];           // 0,1 U (ping-pong), 2,3 F (ping-pong), 4 DT, 5 US, 6..13 K1..K8
     double B[7][SBZ_NB];           // background at the distinct stage times (0..5) and a spare slot (6)
     double dB[2][SBZ_NB];          // d/dτ of the background at t_n (0) and at the spare time (1)
     double Cs[SBZ_NC];             // per-mode constants (momentum nodes, weights, d ln f0/d ln x, c_s², 1/x, 1/k)
-    double M[3][SBZ_NBORD * SBZ_WSTRIDE];  // border block: A, α, β, p, q staging (0); L (1) and U (2) factors
+#if !SBZ_M0_IN_K
+    double M0[SBZ_NBORD * SBZ_WSTRIDE];   // border staging
+#endif
+    double LU[(1 + SBZ_LU_SPLIT) * SBZ_NBORD * SBZ_WSTRIDE];   // factored border block (sbz_wb_store)
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
     double ddkd;
     int mode;
 };
+SBZ_DEV double* sbz_wave_m0(WaveLDS& L) {
+#if SBZ_M0_IN_K
+    return &L.v[6][0];
+#else
+    return L.M0;
+#endif
+}
 
 // select element `i` of an 8-vector with compile-time indices (no scratch for runtime indexing)
 SBZ_DEV double sbz_pick8(const double* v, int i) {
@@ -528,19 +542,13 @@ SBZ_DEV WaveCtx sbz_wave_ctx(int lane) {
     return c;
 }
 
-// The border staging's structural zeros: the W assembly never writes them.  Once per workgroup
-// before its first mode (and again whenever the LDS held another model's layout meanwhile).
-SBZ_DEV void sbz_wave_lds_init(WaveLDS& L, int lane) {
-    for (int i = lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += 64) L.M[0][i] = 0.0;
-}
-
 // Integrate instance `inst` from its start to τ0 (saves and sources on the way) and write its
 // final state, return code and statistics; every lane of the wavefront calls it.
 template <bool FUSED>
 SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, int inst) {
     double* const DT = L.v[4];
     double* const US = L.v[5];
-    double* const TMP = L.v[14];
+    double* const TMP = L.v[8];   // source scratch: K3 is dead between step attempts (rewritten at stage 3)
     {
         const int set = a.inst_set[inst];
         const double* saveat = a.saveat ? a.saveat + (size_t)set * a.saveat_stride : nullptr;
@@ -721,12 +729,12 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, 
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
             double al, be;
-            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
+            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, sbz_wave_m0(L), row, cpl, al, be, wb);
             double rpo;   // reciprocal pivot of this lane's row
             bool ok = sbz_wb_lu<SbzBcWave>(c.lane, row, rpo);
-            sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
+            sbz_wb_store(c.lane, row, rpo, L.LU);
             // Woodbury data: z = A^-1 [α β] and C^-1 with C = I - [p q]^T z (2 x 2)
-            sbz_wb_tri2<SbzBcWave>(c.lane, L.M[1], L.M[2], al, be);
+            sbz_wb_tri2<SbzBcWave>(c.lane, L.LU, al, be);
             wb.z1 = c.isB ? al : 0.0;
             wb.z2 = c.isB ? be : 0.0;
             {
@@ -781,7 +789,7 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, 
                     {
                         double b = c.isB ? Ks[c.bcomp] : 0.0;
                         if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                        sbz_wb_tri<SbzBcWave>(c.lane, L.M[1], L.M[2], b);   // y = A^-1 b
+                        sbz_wb_tri<SbzBcWave>(c.lane, L.LU, b);   // y = A^-1 b
                         const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
                         const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                         if (c.isB) {
@@ -883,8 +891,6 @@ template <bool FUSED>
 SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     __shared__ WaveLDS L;
     const WaveCtx c = sbz_wave_ctx(threadIdx.x);
-    sbz_wave_lds_init(L, c.lane);
-    __syncthreads();
     for (;;) {
         if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
         __syncthreads();
@@ -952,7 +958,6 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
     c.cbase = sbz_chain_base[ci];
     double* const DT = L.v[4];
     double* const US = L.v[5];
-    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += LN) L.M[i] = 0.0;   // structural zeros
     // per-row state; a row without a mode keeps the tables of a valid set so its steps stay finite
     bool have = false, exhausted = false;
     int inst = 0;
@@ -1286,7 +1291,8 @@ struct WaveTanLDS {
     double B[3][7][SBZ_NB];        // background of the sets θ, θ+, θ- at the stage times (slot 6: spare)
     double dB[3][SBZ_NB];          // their τ-derivatives at t_n
     double Cs[SBZ_NC];
-    double M[3][SBZ_NBORD * SBZ_WSTRIDE];
+    double M0[SBZ_NBORD * SBZ_WSTRIDE];   // border staging
+    double LU[(1 + SBZ_LU_SPLIT) * SBZ_NBORD * SBZ_WSTRIDE];   // factored border block (sbz_wb_store)
     double X[SBZ_NBORD];
     double C[SBZ_NCHAIN];
     int mode;
@@ -1313,7 +1319,6 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
     double* const DTs = L.w[4];
     double* const SS = L.w[5];
     double* const TMPs = L.w[14];
-    for (int i = c.lane; i < SBZ_NBORD * SBZ_WSTRIDE; i += 64) L.M[0][i] = 0.0;   // structural zeros
     __syncthreads();
     for (;;) {
         if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
@@ -1453,7 +1458,7 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
             {
                 double b = c.isB ? bv[c.bcomp] : 0.0;
                 if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                sbz_wb_tri<SbzBcWave>(c.lane, L.M[1], L.M[2], b);
+                sbz_wb_tri<SbzBcWave>(c.lane, L.LU, b);
                 const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
                 const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                 if (c.isB) {
@@ -1515,11 +1520,11 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
             __syncthreads();
             const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
             double al, be;
-            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M[0], row, cpl, al, be, wb);
+            sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M0, row, cpl, al, be, wb);
             double rpo;
             bool ok = sbz_wb_lu<SbzBcWave>(c.lane, row, rpo);
-            sbz_wb_store(c.lane, row, rpo, L.M[1], L.M[2]);
-            sbz_wb_tri2<SbzBcWave>(c.lane, L.M[1], L.M[2], al, be);
+            sbz_wb_store(c.lane, row, rpo, L.LU);
+            sbz_wb_tri2<SbzBcWave>(c.lane, L.LU, al, be);
             wb.z1 = c.isB ? al : 0.0;
             wb.z2 = c.isB ? be : 0.0;
             {

@@ -480,6 +480,28 @@ extern "C" int sbz_plan_run(sbz_plan* p, float* kernel_ms) {
     return SBZ_OK;
 }
 
+extern "C" int sbz_plan_launch(sbz_plan* p) {
+    sbz_clear_error();
+    if (!p) return SBZ_ERR_ARG;
+    p->pk_ready = false;
+    if (p->n_inst == 0) return SBZ_OK;
+    HIPCHK(plan_enqueue(p, true, false));
+    return SBZ_OK;
+}
+
+extern "C" int sbz_plan_wait(sbz_plan* p, float* kernel_ms) {
+    sbz_clear_error();
+    if (!p) return SBZ_ERR_ARG;
+    if (p->n_inst == 0) {
+        if (kernel_ms) *kernel_ms = 0.0f;
+        return SBZ_OK;
+    }
+    HIPCHK(hipEventSynchronize(p->ev1));
+    HIPCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));
+    if (kernel_ms) *kernel_ms = p->last_ms;
+    return SBZ_OK;
+}
+
 extern "C" int sbz_plan_run_graph(sbz_plan* p, float* graph_ms) {
     sbz_clear_error();
     if (!p) return SBZ_ERR_ARG;

@@ -765,7 +765,7 @@ class Emitted:
                f"#define SBZ_WAVE4_OK {1 if (nb <= 16 and len(chains) <= 16) else 0}"]
         # the tangent kernel when its LDS (primal and tangent vectors, three backgrounds, border
         # staging and factors) fits one workgroup's 64 KB
-        tan_lds = 8 * (30 * n + 24 * NB + len(C_NAMES) + 3 * nb * (nb + 6) + nb + len(chains) + 1)
+        tan_lds = 8 * (30 * n + 24 * NB + len(C_NAMES) + 2 * nb * (nb + 6) + nb + len(chains) + 1)
         out.append(f"#define SBZ_TAN_OK {1 if tan_lds <= 65536 else 0}")
         out.append("#define SBZ_CHAIN_GROUPS(X) " + " ".join(f"X({a}, {b}, {n_})" for a, b, n_ in groups))
         out.append("__constant__ int sbz_bord_comp[SBZ_NBORD] = {" + ", ".join(map(str, bcomp)) + "};")
@@ -813,8 +813,8 @@ class Emitted:
         out.append("    }")
         out.append("}")
         # A, α, β, p, q of the border, computed once per step by the whole wavefront (one CSE over
-        # all entries) and staged through LDS (structural zeros are never written and stay zero),
-        # from where every lane loads its own row.  Row r: A[r][0..nb), then its coupling to the
+        # all entries) and staged through LDS (only nonzero entries are written: the row loads
+        # mask the others to zero), from where every lane loads its own row.  Row r: A[r][0..nb), then its coupling to the
         # bottom of an attached chain (whose Schur complement xc = W[c0][p]/d'_0 is folded into
         # the diagonal), α_r, β_r, p_r, q_r.
         att_bottom = {lane_of[ch[4]]: ch[3][0] for ch in chains}
@@ -839,6 +839,12 @@ class Emitted:
         out.append("    double p, q, z1, z2;   // p_r, q_r; (A^-1 α)_r, (A^-1 β)_r")
         out.append("    double c11, c12, c21, c22;   // (I - [p q]^T A^-1 [α β])^-1, uniform")
         out.append("};")
+        # rows that hold a written entry in each staging column: every other entry loads as zero,
+        # so the staging needs no zeros of its own (it may share memory with dead stage vectors)
+        colmask = [0] * (nb + 5)
+        for lhs, _ in items:
+            idx = int(lhs[3:-1])
+            colmask[idx % ws] |= 1 << (idx // ws)
         out.append("SBZ_DEV void sbz_wb_w(int lane, const double* __restrict__ B, const double* __restrict__ Cs, double k, double ig, double xc, double* sW, double* row, double& cpl, double& al, double& be, SbzWb& wb) {")
         out.append("    if (lane == 0) {")
         out += ["    " + ln for ln in lines]
@@ -846,14 +852,11 @@ class Emitted:
         out.append("    __syncthreads();")
         out.append("    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;")
         out.append("    const double* w = sW + r * SBZ_WSTRIDE;")
-        out.append("    const bool own = lane < SBZ_NBORD;")
+        out.append("    const unsigned long long bit = (lane < SBZ_NBORD) ? (1ull << lane) : 0ull;")
         for j in range(nb):
-            out.append(f"    row[{j}] = w[{j}];")
-        out.append("    cpl = own ? w[SBZ_NBORD] : 0.0;")
-        out.append("    al = own ? w[SBZ_NBORD + 1] : 0.0;")
-        out.append("    be = own ? w[SBZ_NBORD + 2] : 0.0;")
-        out.append("    wb.p = own ? w[SBZ_NBORD + 3] : 0.0;")
-        out.append("    wb.q = own ? w[SBZ_NBORD + 4] : 0.0;")
+            out.append(f"    row[{j}] = (0x{colmask[j]:x}ull & bit) ? w[{j}] : 0.0;")
+        for off, nm in ((0, "cpl"), (1, "al"), (2, "be"), (3, "wb.p"), (4, "wb.q")):
+            out.append(f"    {nm} = (0x{colmask[nb + off]:x}ull & bit) ? w[SBZ_NBORD + {off}] : 0.0;")
         for i in sorted(att_bottom):   # selects on values, never on row positions: row stays in registers
             out.append(f"    row[{i}] -= (lane == {i}) ? cpl * xc : 0.0;")
         out.append("    __syncthreads();   // sW is rewritten by the next step")
@@ -896,15 +899,31 @@ class Emitted:
         lu.append("    return ok;")
         lu.append("}")
         out += lu
-        # the factors go to LDS as separate strictly-lower (sL) and strictly-upper (sU) rows, with
-        # the reciprocal pivot in column nb of sU, so the solves need no per-lane masks
-        st = ["SBZ_DEV void sbz_wb_store(int lane, const double* row, double rpo, double* sL, double* sU) {",
-              "    if (lane < SBZ_NBORD) {",
-              "        double* lr = sL + lane * SBZ_WSTRIDE;",
-              "        double* ur = sU + lane * SBZ_WSTRIDE;"]
-        for j in range(nb):
-            st.append(f"        lr[{j}] = ({j} < lane) ? row[{j}] : 0.0; ur[{j}] = ({j} > lane) ? row[{j}] : 0.0;")
-        st.append("        ur[SBZ_NBORD] = rpo;")
+        # The factors go to LDS in one of two forms, whichever keeps the one-mode kernel's LDS at
+        # the most resident workgroups per CU (4, one per SIMD, needs <= 40 KB):
+        #   split (SBZ_LU_SPLIT 1): strictly-lower rows (sL) and strictly-upper rows plus the
+        #     reciprocal pivot in column nb (sU = sL + nb·ws), masked once when stored, so the
+        #     solves (nine per step) load them unmasked;
+        #   combined (0): one row per lane, L left of the diagonal, U right of it, 1/pivot in column
+        #     nb: half the LDS, the solves mask each entry as they load it.
+        wl_doubles = 14 * n + 9 * NB + len(C_NAMES) + nb + len(chains) + 1 + (0 if nb * ws <= 8 * n else nb * ws)
+        occ = lambda b: min(4, 163840 // b)
+        split = occ(8 * (wl_doubles + 2 * nb * ws) + 16) >= occ(8 * (wl_doubles + nb * ws) + 16)
+        self.wave["lu_split"] = split
+        out.append(f"#define SBZ_LU_SPLIT {1 if split else 0}")
+        st = ["SBZ_DEV void sbz_wb_store(int lane, const double* row, double rpo, double* sLU) {",
+              "    if (lane < SBZ_NBORD) {"]
+        if split:
+            st += ["        double* lr = sLU + lane * SBZ_WSTRIDE;",
+                   "        double* ur = sLU + (SBZ_NBORD + lane) * SBZ_WSTRIDE;"]
+            for j in range(nb):
+                st.append(f"        lr[{j}] = ({j} < lane) ? row[{j}] : 0.0; ur[{j}] = ({j} > lane) ? row[{j}] : 0.0;")
+            st.append("        ur[SBZ_NBORD] = rpo;")
+        else:
+            st.append("        double* mr = sLU + lane * SBZ_WSTRIDE;")
+            for j in range(nb):
+                st.append(f"        mr[{j}] = row[{j}];")
+            st.append("        mr[SBZ_NBORD] = rpo;")
         st.append("    }")
         st.append("    __syncthreads();")
         st.append("}")
@@ -916,14 +935,27 @@ class Emitted:
         def tri(name, nr):
             args = ", ".join(f"double& b{q}" for q in range(nr))
             so = ["template <class BC>",
-                  f"SBZ_DEV void {name}(int lane, const double* __restrict__ sL, const double* __restrict__ sU, {args}) {{",
-                  "    const int r = lane < SBZ_NBORD ? lane : SBZ_NBORD - 1;",
-                  "    const double* lr = sL + r * SBZ_WSTRIDE;",
-                  "    const double* ur = sU + r * SBZ_WSTRIDE;"]
+                  f"SBZ_DEV void {name}(int lane, const double* __restrict__ sLU, {args}) {{",
+                  "    const bool own = lane < SBZ_NBORD;   // lanes past the border use zero factors",
+                  "    const int r = own ? lane : SBZ_NBORD - 1;"]
+            if split:
+                so += ["    const double* lr = sLU + r * SBZ_WSTRIDE;",
+                       "    const double* ur = sLU + (SBZ_NBORD + r) * SBZ_WSTRIDE;",
+                       "    const double* sU = sLU + SBZ_NBORD * SBZ_WSTRIDE;"]
+                lval = lambda j: f"lr[{j}]"
+                uval = lambda j: f"ur[{j}]"
+                rpv = lambda j: f"sU[{j} * SBZ_WSTRIDE + SBZ_NBORD]"
+                rpr = "ur[SBZ_NBORD]"
+            else:
+                so += ["    const double* mr = sLU + r * SBZ_WSTRIDE;"]
+                lval = lambda j: f"(own && {j} < lane) ? mr[{j}] : 0.0"
+                uval = lambda j: f"(own && {j} > lane) ? mr[{j}] : 0.0"
+                rpv = lambda j: f"sLU[{j} * SBZ_WSTRIDE + SBZ_NBORD]"
+                rpr = "mr[SBZ_NBORD]"
             # every factor entry of a pass is loaded before the pass starts (the empty asm is a
             # compiler barrier for memory accesses), so the rounds never wait on LDS latency
             fwd = [j for j in range(nb) if lower[j]]
-            so += [f"    const double l{j} = lr[{j}];" for j in fwd]
+            so += [f"    const double l{j} = {lval(j)};" for j in fwd]
             so.append('    asm volatile("" ::: "memory");')
             for L_ in range(nlev):
                 piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
@@ -937,8 +969,8 @@ class Emitted:
                     so.append(f"        b{q} -= " + " + ".join(f"l{j} * x{q}_{j}" for j in piv) + ";")
                 so.append("    }")
             bwd = [j for j in range(nb) if ucol[j]]
-            so += [f"    const double u{j} = ur[{j}], rp{j} = sU[{j} * SBZ_WSTRIDE + SBZ_NBORD];" for j in bwd]
-            so.append("    const double rpr = ur[SBZ_NBORD];")
+            so += [f"    const double u{j} = {uval(j)}, rp{j} = {rpv(j)};" for j in bwd]
+            so.append(f"    const double rpr = {rpr};")
             so.append('    asm volatile("" ::: "memory");')
             for L_ in range(nlevb):
                 piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]

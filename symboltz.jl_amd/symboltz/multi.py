@@ -65,7 +65,6 @@ extern "C" __global__ void __launch_bounds__(64) sbz_multi_kernel(SbzMultiArgs a
     // the LDS of whichever model the wavefront runs: each family lays out its own WaveLDS here
     __shared__ __align__(16) unsigned char lds[{lds}];
     __shared__ int pulled;
-    int last = -1;
     for (;;) {{
         if (threadIdx.x == 0) {{
             const unsigned int g = atomicAdd(a.queue, 1u);
@@ -82,17 +81,12 @@ extern "C" __global__ void __launch_bounds__(64) sbz_multi_kernel(SbzMultiArgs a
         out.append(f"""        case {f}: {{
             sbzf{f}::WaveLDS& L = *reinterpret_cast<sbzf{f}::WaveLDS*>(lds);
             const sbzf{f}::WaveCtx c = sbzf{f}::sbz_wave_ctx(threadIdx.x);
-            if (last != {f}) {{
-                sbzf{f}::sbz_wave_lds_init(L, c.lane);
-                __syncthreads();
-            }}
             sbzf{f}::sbz_wave_mode<true>(reinterpret_cast<const sbzf{f}::SbzSolveArgs&>(a.fam[{f}]), L, c, inst);
             break;
         }}""")
     out.append("""        default:
             break;
         }
-        last = f;
     }
 }""")
     return "\n".join(out) + "\n"

@@ -85,6 +85,8 @@ def _declare(L):
                                   ctypes.POINTER(vp)]
     L.sbz_plan_run.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.sbz_plan_run_graph.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.sbz_plan_launch.argtypes = [vp]
+    L.sbz_plan_wait.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.sbz_plan_fetch.argtypes = [vp, dp, dp, ip, llp]
     L.sbz_plan_pk.argtypes = [vp, dp]
     L.sbz_plan_fetch_sources.argtypes = [vp, dp]
@@ -174,6 +176,15 @@ class Plan:
     def run(self):
         ms = ctypes.c_float(0.0)
         _lib.check(self.gm.L.sbz_plan_run(self.h, ctypes.byref(ms)), "sbz_plan_run")
+        return ms.value
+
+    def launch(self):
+        """Enqueue the run and return at once (sbz_plan_launch); `wait()` gives its kernel ms."""
+        _lib.check(self.gm.L.sbz_plan_launch(self.h), "sbz_plan_launch")
+
+    def wait(self):
+        ms = ctypes.c_float(0.0)
+        _lib.check(self.gm.L.sbz_plan_wait(self.h, ctypes.byref(ms)), "sbz_plan_wait")
         return ms.value
 
     def run_graph(self):

@@ -54,12 +54,17 @@ class MixedStream:
             shapes.append(len(pars))
         return plans, shapes
 
-    def solve(self, requests, one_launch=True):
+    def solve(self, requests, mode="one_launch"):
         """requests: (family index, parameter dict, surrogate input row or None) per request.
         Returns log P [n_req][n_k] (P in (c/H0)³), the mask of requests the surrogate answered,
         the per-request return code (0: every mode succeeded) and timings in ms (surrogate
-        inference, background solves and uploads, solve kernel(s), P(k)).  one_launch=False runs
-        the families' plans one launch after the other instead (for comparison)."""
+        inference, background solves and uploads, solve kernel(s), P(k)), with the step attempts
+        and their algorithmic flops of the solves.  `mode` picks how the solves run:
+        "one_launch" (sbz_multi_kernel over all families), "concurrent" (one launch per family, each
+        on its model's stream, all enqueued before any is waited for: the kernels overlap on the
+        device, each at its own occupancy) or "serial" (one launch per family, each waited for).
+        solve_ms is the one launch's kernel time, the longest of the concurrent launches from the
+        first enqueue (wall clock), or the sum of the serial ones."""
         n, nk = len(requests), len(self.ks_mpc)
         logp = np.full((n, nk), np.nan)
         answered = np.zeros(n, bool)
@@ -86,16 +91,32 @@ class MixedStream:
         plans, _ = self._plans(groups)
         tm["prepare_ms"] = (time.perf_counter() - t0) * 1e3
         try:
-            if one_launch:
+            if mode == "one_launch":
                 tm["solve_ms"] = self.multi.run(plans)
-            else:
+            elif mode == "concurrent":
+                live = [p for p in plans if p.n_inst]
+                t1 = time.perf_counter()
+                for p in live:
+                    p.launch()
+                for p in live:
+                    p.wait()
+                tm["solve_ms"] = (time.perf_counter() - t1) * 1e3 if live else 0.0
+            elif mode == "serial":
                 tm["solve_ms"] = sum(p.run() for p in plans if p.n_inst)
+            else:
+                raise ValueError(f"unknown mode {mode!r}")
             t0 = time.perf_counter()
+            tm["steps"] = 0
+            tm["flops"] = 0
             for f, p in enumerate(plans):
                 if not p.n_inst:
                     continue
                 pk = p.pk().reshape(len(where[f]), nk)
-                rc = p.fetch()["retcode"].reshape(len(where[f]), nk)
+                r = p.fetch()
+                rc = r["retcode"].reshape(len(where[f]), nk)
+                steps = int(r["stats"][:, 0].sum() + r["stats"][:, 1].sum())
+                tm["steps"] += steps
+                tm["flops"] += steps * self.gms[f].em.flops_step   # the emitter's count per Rodas5P step
                 for j, i in enumerate(where[f]):
                     logp[i] = np.log(pk[j])
                     ret[i] = int(np.max(rc[j]))

@@ -30,11 +30,14 @@ def test_mixed_families_one_launch_match_per_family_and_oracle():
     st = MixedStream(fams, ks)
     try:
         lp1, ans1, ret1, tm1 = st.solve(reqs)
-        lp2, ans2, ret2, tm2 = st.solve(reqs, one_launch=False)
-        say(f"one launch {tm1['solve_ms']:.1f} ms, per family {tm2['solve_ms']:.1f} ms")
+        lp2, ans2, ret2, tm2 = st.solve(reqs, mode="serial")
+        lp5, _, ret5, tm5 = st.solve(reqs, mode="concurrent")
+        say(f"one launch {tm1['solve_ms']:.1f} ms, per family serial {tm2['solve_ms']:.1f} ms, "
+            f"concurrent {tm5['solve_ms']:.1f} ms")
         assert not ans1.any() and not ans2.any()
-        assert np.all(ret1 == 0) and np.all(ret2 == 0)
+        assert np.all(ret1 == 0) and np.all(ret2 == 0) and np.all(ret5 == 0)
         np.testing.assert_array_equal(lp1, lp2)
+        np.testing.assert_array_equal(lp1, lp5)
         # the oracle on the first request of every family
         for f in range(len(fams)):
             i = next(j for j, r in enumerate(reqs) if r[0] == f)
