@@ -199,6 +199,58 @@ def bench_mixed(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_mlp_train(args):
+    """Surrogate training step: Adam steps of the 7 -> 512^4 -> 256 GELU network (BF16 MFMA products,
+    FP32 accumulation and master weights) on batches of 1024 rows drawn from 32,768 resident
+    synthetic rows (uniform inputs, normal targets).  A step is one epoch of 32 Adam steps."""
+    from symboltz.mlp import Mlp
+    widths = [7, 512, 512, 512, 512, 256]
+    B, n_rows = 1024, 32768
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1, 1, (n_rows, widths[0])).astype(np.float32)
+    T = rng.normal(0, 1, (n_rows, widths[-1])).astype(np.float32)
+    net = Mlp(widths, max_batch=B, seed=0)
+    net.set_data(X, T)
+    for w in range(args.warmup):
+        net.train(rng.permutation(n_rows), B, 1e-3)
+    ms = 0.0
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        loss, t = net.train(rng.permutation(n_rows), B, 1e-3)
+        ms += t
+        log(f"step {s}: {t:.2f} ms for {n_rows // B} Adam steps, loss {float(loss.mean()):.4e}")
+    elapsed = (time.perf_counter() - t_start) / args.steps
+    adam_steps = n_rows // B
+    macs = sum(widths[l] * widths[l + 1] for l in range(len(widths) - 1))
+    flops = 6.0 * B * macs * adam_steps        # forward, backward data and backward weight products
+    achieved = flops / (ms / args.steps * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle import mlp_oracle as mo
+        p = net.get_params()
+        t_c = time.perf_counter()
+        mo.loss_and_grad(p, widths, X[:B], T[:B])
+        tc = time.perf_counter() - t_c
+        cores, cpu_model = host_cpu()
+        cpu = {"value": 1.0 / tc, "unit": "Adam steps/s (gradient only)", "cores": cores, "kind": "port",
+               "sample": f"one gradient of a 1024-row batch with the numpy oracle (FP64 BLAS on up to {cores} "
+                         f"threads, BF16 rounding points; {cpu_model}), {tc:.2f} s"}
+    net.close()
+    out = {
+        "metric": "surrogate training Adam steps/s (batch 1024, 7 -> 512x4 -> 256)",
+        "value": adam_steps / elapsed, "unit": "Adam steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": "mlp_train", "description": f"{n_rows} resident rows, batch {B}, {adam_steps} Adam "
+                   f"steps per step, widths {widths}", "gpu_ms_per_adam_step": ms / args.steps / adam_steps},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": achieved / 2500.0, "traffic": None, "kernel": "training step (all kernels)",
+                     "flops_per_adam_step": 6.0 * B * macs},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_cmb(args):
     """C_l workload: spectrum_cmb([TT, EE, ψψ]) of the reference's CLASS-comparison model (ΛCDM
     lmax 5; test/runtests.jl:14-16,879-886) for a batch of cosmologies: set 0 is Planck18 (checked
@@ -329,7 +381,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massless10", "cmb", "bg", "mixed"])
+    ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massless10", "cmb", "bg", "mixed", "mlp_train"])
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--algo", choices=["auto", "wave", "wave2", "wave4"], default="auto",
                     help="auto: timed choice of the kernel variant (default); wave: one wavefront per "
@@ -378,6 +430,8 @@ def main():
         return bench_bg(args)
     if args.workload == "mixed":
         return bench_mixed(args)
+    if args.workload == "mlp_train":
+        return bench_mlp_train(args)
     M, sets, ks_mpc, desc = workloads.workload(args.workload)
     gm = GpuModel(M)
     bgs, inst_set, ks, t0, t_bg = prepare(args, M, sets, ks_mpc, gm)

@@ -5,7 +5,7 @@ build's own design.
 
 Forward:  x_0 = bf16(x);  z_l = x_l W_l + b_l;  x_{l+1} = bf16(GELU(z_l));  y = z_L (FP32).
 Loss:     mean((y - t)²);  dy = 2 (y - t) / n.
-Backward: db_l = Σ_rows d_l (FP32 copy of d_l);  dW_l = x_lᵀ bf16(d_l);
+Backward: db_l = Σ_rows bf16(d_l);  dW_l = x_lᵀ bf16(d_l)  (both from the BF16 operand tiles);
           d_{l-1} = bf16((bf16(d_l) W_lᵀ) ⊙ GELU'(z_{l-1})).
 GELU(x) = x Φ(x) with the normal CDF (math.erf), GELU'(x) = Φ(x) + x φ(x)."""
 import math
@@ -69,12 +69,11 @@ def loss_and_grad(params, widths, x, t):
     d = 2.0 * (y - t) / n
     L = len(Wh)
     gW, gb = [None] * L, [None] * L
-    dh, df = bf16(d), d
+    dh = bf16(d)
     for l in range(L - 1, -1, -1):
-        gb[l] = df.sum(axis=0)
+        gb[l] = dh.sum(axis=0)
         gW[l] = xs[l].T @ dh
         if l == 0:
             break
         dh = bf16((dh @ Wh[l].T) * gelu_d(zs[l - 1]))
-        df = dh
     return loss, np.concatenate([g.ravel() for g in gW] + [g.ravel() for g in gb])

@@ -5,13 +5,15 @@
 // (v_mfma_f32_16x16x16_bf16), and the optimizer keeps FP32 master weights (Adam).  The loss is the
 // mean squared error of the (standardized) outputs.
 //
-// One GEMM kernel serves all three products of a layer through strides:
+// The three products of a layer are one GEMM description with strides:
 //   forward          Z  = X  W  + b     A = X [M][K],     B = W [K][N]
 //   backward data    dX = dZ Wᵀ         A = dZ [M][N],    B(p, j) = W[j][p]
-//   backward weight  dW = Xᵀ dZ         A(i, p) = X[p][i], B = dZ [M][N]
-// with fused epilogues (bias, GELU and its derivative, BF16/FP32 stores).  Tiles: a 256-thread
-// workgroup computes a 64 x 64 block of C, each of its four wavefronts a 32 x 32 quarter as 2 x 2
-// MFMA 16 x 16 blocks, the K dimension staged through LDS 32 at a time.
+//   backward weight  dW = Xᵀ dZ         A(i, p) = X[p][i], B = dZ [M][N]   (and db = column sums of dZ)
+// with fused epilogues (bias, GELU and its derivative, BF16/FP32 stores, atomic accumulation).
+// Kernels by shape: sbz_mlp_gemm_t (training sizes: 16-byte loads along each operand's contiguous
+// index, register prefetch, split contraction for the weight gradients), sbz_mlp_gemm_kk (large
+// inference batches, 128 x 128 tiles), sbz_mlp_gemm_smallk / _smallm (the 7-wide input layer), and
+// sbz_mlp_gemm (any other shape, element-wise staging).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -59,6 +61,8 @@ struct SbzGemm {
     const float* zin;     // GELU_BWD: pre-activation [M][N] of the layer below
     float* out_f;         // [M][N] FP32 result (before the activation) or null
     bf16_t* out_h;        // [M][N] BF16 result (after the activation) or null
+    int accum;            // 1: add the product atomically into out_f (zeroed by the host), linear only
+    float* bsum;          // [N] or null: column sums of B over the contraction, added atomically
 };
 
 #define TM 64
@@ -207,6 +211,188 @@ extern "C" __global__ void __launch_bounds__(256) sbz_mlp_gemm_kk(SbzGemm g) {
             }
 }
 
+// Training-size products (batches of ~10³ rows, widths of ~10²): the three products of a layer
+// with whole 16-byte loads whatever the operand layout.  A 256-thread workgroup computes a 64 x BN
+// tile (BN = 64 or 32) with v_mfma_f32_16x16x32_bf16; K advances BK = 128 at a time through
+// LDS tiles that are contiguous along k (the MFMA operand order), and the next step's operands are
+// loaded into registers before the current step's products, so their latency overlaps the matrix
+// work.  An operand contiguous along k is copied chunk for chunk; one contiguous along its row (i)
+// or column (j) index is loaded 8 rows at a time and scattered into the k-contiguous tile.  The
+// weight gradients accumulate atomically into a zeroed out_f (accum), with the contraction cut into
+// grid.z ranges when the tiles alone do not fill the chip, and the first row of workgroups adds the
+// column sums of B (the bias gradient, bsum) from the same tiles.  Requires the contiguous extent of
+// each operand and the other stride to be multiples of 8 and 16-byte aligned operands (the host
+// checks, otherwise a general kernel runs).
+#define GBM 64
+#define GBK 128
+#define GPAD 8
+template <int BN, int BK>
+__global__ void __launch_bounds__(256) sbz_mlp_gemm_t(SbzGemm g, int kps) {
+    constexpr int WM = (BN == 64) ? 2 : 4;                  // wavefronts along M
+    constexpr int WN = 4 / WM;
+    constexpr int FM = GBM / WM / 16, FN = BN / WN / 16;    // 16 x 16 blocks per wavefront
+    constexpr int ACH = GBM * BK / 8 / 256;                 // chunks of 8 per thread: A
+    constexpr int BCH = BN * BK / 8 / 256;                  //                         B
+    constexpr int KC = BK / 8;                              // chunks per k-contiguous row
+    __shared__ __attribute__((aligned(16))) bf16_t sA[GBM][BK + GPAD];
+    __shared__ __attribute__((aligned(16))) bf16_t sB[BN][BK + GPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = blockIdx.y * GBM, j0 = blockIdx.x * BN;
+    const int wm = (wave / WN) * (GBM / WM), wn = (wave % WN) * (BN / WN);
+    const int pb = blockIdx.z * kps, pe = min(g.K, pb + kps);
+    const bool a_k = (g.a_p == 1), b_k = (g.b_p == 1);
+    const bool do_bsum = g.bsum && blockIdx.y == 0;
+    sbz_float4 acc[FM][FN];
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = sbz_float4{0.f, 0.f, 0.f, 0.f};
+    float bs = 0.f;   // this thread's part of the column sum of B (column tid % BN)
+    uint4 ra[ACH], rb[BCH];
+    // chunk c of the A tile: k-contiguous: row c/KC, k 8(c%KC); i-contiguous: k c/8, rows 8(c%8)..+7
+    auto load = [&](int p0) {
+#pragma unroll
+        for (int h = 0; h < ACH; ++h) {
+            const int c = tid + 256 * h;
+            int i, p;
+            if (a_k) { i = i0 + c / KC; p = p0 + 8 * (c % KC); }
+            else     { p = p0 + c / 8; i = i0 + 8 * (c % 8); }
+            const bool ok = (i < g.M) && (p < pe);
+            ra[h] = ok ? *(const uint4*)(g.A + (long long)i * g.a_i + (long long)p * g.a_p) : uint4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int h = 0; h < BCH; ++h) {
+            const int c = tid + 256 * h;
+            int j, p;
+            if (b_k) { j = j0 + c / KC; p = p0 + 8 * (c % KC); }
+            else     { p = p0 + c / (BN / 8); j = j0 + 8 * (c % (BN / 8)); }
+            const bool ok = (j < g.N) && (p < pe);
+            rb[h] = ok ? *(const uint4*)(g.B + (long long)p * g.b_p + (long long)j * g.b_j) : uint4{0u, 0u, 0u, 0u};
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int h = 0; h < ACH; ++h) {
+            const int c = tid + 256 * h;
+            if (a_k) {
+                *(uint4*)&sA[c / KC][8 * (c % KC)] = ra[h];
+            } else {
+                const bf16_t* e = (const bf16_t*)&ra[h];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sA[8 * (c % 8) + q][c / 8] = e[q];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < BCH; ++h) {
+            const int c = tid + 256 * h;
+            if (b_k) {
+                *(uint4*)&sB[c / KC][8 * (c % KC)] = rb[h];
+            } else {
+                const bf16_t* e = (const bf16_t*)&rb[h];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sB[8 * (c % (BN / 8)) + q][c / (BN / 8)] = e[q];
+            }
+        }
+    };
+    if (pb < pe) load(pb);
+    for (int p0 = pb; p0 < pe; p0 += BK) {
+        store();
+        __syncthreads();
+        if (p0 + BK < pe) load(p0 + BK);   // next step's operands in flight during the products
+        if (do_bsum) {   // column tid % BN, k slice tid / BN of the tile (zero past the matrices)
+            constexpr int SL = 256 / BN;
+#pragma unroll
+            for (int q = 0; q < BK / SL; ++q) bs += sbz_bf2f(sB[tid % BN][(tid / BN) * (BK / SL) + q]);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 32) {
+            // lane l supplies row (l % 16) of A and column (l % 16) of B at k = kk + 8 (l / 16) .. + 7
+            const int kq = kk + 8 * (lane >> 4);
+            sbz_bf16x8 af[FM], bfr[FN];
+#pragma unroll
+            for (int a = 0; a < FM; ++a) af[a] = *(const sbz_bf16x8*)&sA[wm + 16 * a + (lane & 15)][kq];
+#pragma unroll
+            for (int b = 0; b < FN; ++b) bfr[b] = *(const sbz_bf16x8*)&sB[wn + 16 * b + (lane & 15)][kq];
+#pragma unroll
+            for (int a = 0; a < FM; ++a)
+#pragma unroll
+                for (int b = 0; b < FN; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    if (do_bsum && j0 + tid % BN < g.N) unsafeAtomicAdd(&g.bsum[j0 + tid % BN], bs);
+    // result layout: lane l holds rows 4 (l / 16) .. + 3 of column (l % 16)
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + wm + 16 * a + 4 * (lane >> 4) + q;
+                const int j = j0 + wn + 16 * b + (lane & 15);
+                if (i >= g.M || j >= g.N) continue;
+                float v = acc[a][b][q];
+                const long long o = (long long)i * g.N + j;
+                if (g.accum) {
+                    unsafeAtomicAdd(&g.out_f[o], v);
+                    continue;
+                }
+                if (g.bias) v += g.bias[j];
+                if (g.out_f) g.out_f[o] = v;
+                if (g.out_h) {
+                    float hv = v;
+                    if (g.epi == SBZ_EPI_GELU) hv = sbz_gelu(v);
+                    else if (g.epi == SBZ_EPI_GELU_BWD) hv = v * sbz_gelu_d(g.zin[o]);
+                    g.out_h[o] = sbz_f2bf(hv);
+                }
+            }
+}
+
+// Products with a contraction of at most 16 (the input layer's forward pass, K = 7): one thread per
+// output, the K products in FP32, the general epilogue.
+extern "C" __global__ void __launch_bounds__(256) sbz_mlp_gemm_smallk(SbzGemm g) {
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (e >= (long long)g.M * g.N) return;
+    const int i = (int)(e / g.N), j = (int)(e % g.N);
+    float v = 0.f;
+    for (int p = 0; p < g.K; ++p)
+        v += sbz_bf2f(g.A[(long long)i * g.a_i + (long long)p * g.a_p]) * sbz_bf2f(g.B[(long long)p * g.b_p + (long long)j * g.b_j]);
+    if (g.bias) v += g.bias[j];
+    if (g.out_f) g.out_f[e] = v;
+    if (g.out_h) {
+        float hv = v;
+        if (g.epi == SBZ_EPI_GELU) hv = sbz_gelu(v);
+        else if (g.epi == SBZ_EPI_GELU_BWD) hv = v * sbz_gelu_d(g.zin[e]);
+        g.out_h[e] = sbz_f2bf(hv);
+    }
+}
+
+// Accumulated products with at most 16 output rows and a long contraction (the input layer's weight
+// gradient, 7 x N over the batch): a block takes 64 columns and one of gridDim.y slices of the
+// contraction, its 4 wavefronts interleave over the slice, each lane keeps the M sums of its column
+// and adds them (and the column sum of B for bsum) atomically into the zeroed outputs.
+extern "C" __global__ void __launch_bounds__(256) sbz_mlp_gemm_smallm(SbzGemm g) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63), part = threadIdx.x >> 6;
+    if (j >= g.N) return;
+    const int per = (g.K + gridDim.y - 1) / gridDim.y;
+    const int pb = blockIdx.y * per, pe = min(g.K, pb + per);
+    float acc[16], bs = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+    for (int p = pb + part; p < pe; p += 4) {
+        const float b = sbz_bf2f(g.B[(long long)p * g.b_p + (long long)j * g.b_j]);
+        bs += b;
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < g.M) acc[i] += sbz_bf2f(g.A[(long long)i * g.a_i + (long long)p * g.a_p]) * b;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        if (i < g.M) unsafeAtomicAdd(&g.out_f[(long long)i * g.N + j], acc[i]);
+    if (g.bsum) unsafeAtomicAdd(&g.bsum[j], bs);
+}
+
 // BF16 transposed copy of one weight matrix: dst [N][K] = src [K][N]
 extern "C" __global__ void sbz_mlp_transpose(const bf16_t* __restrict__ src, int K, int N, bf16_t* __restrict__ dst) {
     __shared__ bf16_t t[32][33];
@@ -250,7 +436,7 @@ extern "C" __global__ void sbz_mlp_mse(const float* __restrict__ y, const float*
         const float r = y[e] - t[e];
         s += r * r;
         const float d = 2.0f * r * scale;
-        dy_f[e] = d;
+        if (dy_f) dy_f[e] = d;
         dy_h[e] = sbz_f2bf(d);
     }
     part[threadIdx.x] = s;
@@ -260,32 +446,6 @@ extern "C" __global__ void sbz_mlp_mse(const float* __restrict__ y, const float*
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
-}
-
-// bias gradient: column sums of the FP32 output gradient [M][N].  A 256-thread block sums a
-// 64-column x 64-row tile (64 columns x 4 row slices, coalesced along the columns), reduces its
-// slices in LDS and adds the tile's sums atomically (db is zeroed first).
-extern "C" __global__ void __launch_bounds__(256) sbz_mlp_colsum(const float* __restrict__ d, int M, int N,
-                                                                float* __restrict__ db) {
-    __shared__ float part[4][64];
-    const int c = threadIdx.x & 63, sl = threadIdx.x >> 6;
-    const int j = blockIdx.x * 64 + c, i0 = blockIdx.y * 64;
-    float s = 0.f;
-    if (j < N)
-        for (int i = i0 + sl; i < min(M, i0 + 64); i += 4) s += d[(long long)i * N + j];
-    part[sl][c] = s;
-    __syncthreads();
-    if (sl == 0 && j < N) atomicAdd(&db[j], (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]));
-}
-extern "C" __global__ void sbz_mlp_zero(float* __restrict__ x, int n) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) x[e] = 0.f;
-}
-
-// FP32 copy of a BF16 gradient (the hidden layers' dZ are produced in BF16 for the GEMMs)
-extern "C" __global__ void sbz_mlp_bf2f(const bf16_t* __restrict__ x, long long n, float* __restrict__ y) {
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
-        y[e] = sbz_bf2f(x[e]);
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters, then their BF16 working copy
@@ -340,7 +500,6 @@ struct sbz_mlp {
     std::vector<float*> z;     // per layer: pre-activation [max_batch][w[l+1]] FP32
     std::vector<bf16_t*> x;    // per layer input activation [max_batch][w[l]] BF16 (x[L] = output, unused)
     bf16_t *dh = nullptr;      // gradient scratch BF16 [max_batch][max width] x 2
-    float* df = nullptr;       // gradient scratch FP32 [max_batch][max width]
     float* y = nullptr;        // output FP32 [max_batch][w[L]]
     float* tb = nullptr;       // batch targets FP32
     int* idx = nullptr;        // batch indices
@@ -354,7 +513,7 @@ struct sbz_mlp {
 
 static void mlp_free(sbz_mlp* h) {
     for (void* p : {(void*)h->par, (void*)h->m, (void*)h->v, (void*)h->grad, (void*)h->par_h, (void*)h->par_t, (void*)h->dh,
-                    (void*)h->df, (void*)h->y, (void*)h->tb, (void*)h->idx, (void*)h->loss, (void*)h->data_x,
+                    (void*)h->y, (void*)h->tb, (void*)h->idx, (void*)h->loss, (void*)h->data_x,
                     (void*)h->data_t})
         if (p) (void)hipFree(p);
     for (auto p : h->z)
@@ -372,11 +531,51 @@ static bool kk_fits(const SbzGemm& g) {
            g.b_j % 8 == 0 && al16(g.A) && al16(g.B) && (long long)(g.M / FBM) * (g.N / FBN) >= 256;
 }
 
+// the training-size kernel: whole 16-byte chunks along each operand's contiguous index
+static bool t_fits(const SbzGemm& g) {
+    auto al16 = [](const void* p) { return ((uintptr_t)p & 15u) == 0; };
+    const bool a_ok = (g.a_p == 1) ? (g.K % 8 == 0 && g.a_i % 8 == 0) : (g.a_i == 1 && g.M % 8 == 0 && g.a_p % 8 == 0);
+    const bool b_ok = (g.b_p == 1) ? (g.K % 8 == 0 && g.b_j % 8 == 0) : (g.b_j == 1 && g.N % 8 == 0 && g.b_p % 8 == 0);
+    return a_ok && b_ok && al16(g.A) && al16(g.B) && g.M > 0 && g.N > 0 && g.K > 0;
+}
+
+// accumulating products (g.accum) need out_f (and bsum) zeroed by the caller; the other kernels
+// write their outputs
 static hipError_t gemm(sbz_mlp* h, const SbzGemm& g) {
-    if (kk_fits(g)) {
+    if (!g.accum && !g.bsum && kk_fits(g)) {
         hipLaunchKernelGGL(sbz_mlp_gemm_kk, dim3(g.N / FBN, g.M / FBM), dim3(256), 0, h->stream, g);
         return hipGetLastError();
     }
+    if (t_fits(g)) {
+        // 64 x 64 tiles when they fill the chip, else 64 x 32; an accumulating product with a long
+        // contraction (the weight gradients) is split along it until ~256 workgroups run
+        const int gm = (g.M + GBM - 1) / GBM;
+        const bool wide = (long long)gm * ((g.N + 63) / 64) >= 256;
+        const int bn = wide ? 64 : 32;
+        const long long tiles = (long long)gm * ((g.N + bn - 1) / bn);
+        // an accumulating product with a long contraction is split until ~256 workgroups run
+        // (staging a whole 512-deep range at once instead of 128-deep prefetched steps measured no
+        // faster: these products are bound by the launch and the per-tile latency, not by the loads)
+        int split = 1;
+        if (g.accum && tiles < 256) split = (int)std::min<long long>(std::max<long long>(1, 256 / tiles), std::max(1, g.K / 256));
+        const int kps = ((g.K + split - 1) / split + GBK - 1) / GBK * GBK;
+        split = (g.K + kps - 1) / kps;
+        const dim3 grid((g.N + bn - 1) / bn, gm, split);
+        if (wide) hipLaunchKernelGGL((sbz_mlp_gemm_t<64, GBK>), grid, dim3(256), 0, h->stream, g, kps);
+        else hipLaunchKernelGGL((sbz_mlp_gemm_t<32, GBK>), grid, dim3(256), 0, h->stream, g, kps);
+        return hipGetLastError();
+    }
+    if (g.K <= 16 && !g.accum && !g.bsum) {
+        const long long n = (long long)g.M * g.N;
+        hipLaunchKernelGGL(sbz_mlp_gemm_smallk, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, g);
+        return hipGetLastError();
+    }
+    if (g.M <= 16 && g.accum && g.epi == SBZ_EPI_LINEAR && !g.bias && !g.out_h) {
+        const int slices = std::max(1, std::min(32, g.K / 64));
+        hipLaunchKernelGGL(sbz_mlp_gemm_smallm, dim3((g.N + 63) / 64, slices), dim3(256), 0, h->stream, g);
+        return hipGetLastError();
+    }
+    if (g.accum || g.bsum) return hipErrorInvalidValue;   // no other kernel accumulates
     dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
     hipLaunchKernelGGL(sbz_mlp_gemm, grid, dim3(256), 0, h->stream, g);
     return hipGetLastError();
@@ -459,7 +658,6 @@ extern "C" int sbz_mlp_create(int n_layers, const int* widths, int max_batch, un
         if ((e = hipMalloc((void**)&h->x[l], sizeof(bf16_t) * B * widths[l])) != hipSuccess) return fail(e, "x");
     }
     if ((e = hipMalloc((void**)&h->dh, sizeof(bf16_t) * B * h->wmax * 2)) != hipSuccess) return fail(e, "dh");
-    if ((e = hipMalloc((void**)&h->df, sizeof(float) * B * h->wmax)) != hipSuccess) return fail(e, "df");
     if ((e = hipMalloc((void**)&h->y, sizeof(float) * B * widths[n_layers])) != hipSuccess) return fail(e, "y");
     if ((e = hipMalloc((void**)&h->tb, sizeof(float) * B * widths[n_layers])) != hipSuccess) return fail(e, "tb");
     if ((e = hipMalloc((void**)&h->idx, sizeof(int) * B)) != hipSuccess) return fail(e, "idx");
@@ -549,25 +747,22 @@ static hipError_t forward(sbz_mlp* h, int B, float* y, bool keep_z) {
     return hipSuccess;
 }
 
-// loss and gradient (into h->grad) of the batch in h->x[0] / h->tb; returns the loss on the host
-static int loss_and_grad(sbz_mlp* h, int B, float* loss_out) {
+// loss (added into the device float *d_loss, zeroed by the caller) and gradient (into h->grad) of
+// the batch in h->x[0] / h->tb; nothing waits for the device
+static int loss_and_grad(sbz_mlp* h, int B, float* d_loss) {
     const int L = h->L, nout = h->w[L];
     MCHK(forward(h, B, h->y, true));
-    MCHK(hipMemsetAsync(h->loss, 0, sizeof(float), h->stream));
+    // every weight and bias gradient accumulates into the zeroed gradient vector
+    MCHK(hipMemsetAsync(h->grad, 0, sizeof(float) * h->n_par, h->stream));
     bf16_t* d_cur = h->dh;                           // BF16 gradient w.r.t. the current layer's output
     bf16_t* d_nxt = h->dh + (size_t)h->max_batch * h->wmax;
     const long long n = (long long)B * nout;
-    hipLaunchKernelGGL(sbz_mlp_mse, dim3(64), dim3(256), 0, h->stream, h->y, h->tb, n, 1.0f / (float)n, d_cur, h->df,
-                       h->loss);
+    hipLaunchKernelGGL(sbz_mlp_mse, dim3(64), dim3(256), 0, h->stream, h->y, h->tb, n, 1.0f / (float)n, d_cur,
+                       (float*)nullptr, d_loss);
     MCHK(hipGetLastError());
     for (int l = L - 1; l >= 0; --l) {
         const int K = h->w[l], N = h->w[l + 1];
-        // db_l = column sums of dZ_l (FP32 copy in h->df)
-        hipLaunchKernelGGL(sbz_mlp_zero, dim3((N + 255) / 256), dim3(256), 0, h->stream, h->grad + h->off_b[l], N);
-        hipLaunchKernelGGL(sbz_mlp_colsum, dim3((N + 63) / 64, (B + 63) / 64), dim3(256), 0, h->stream, h->df, B, N,
-                           h->grad + h->off_b[l]);
-        MCHK(hipGetLastError());
-        // dW_l = X_lᵀ dZ_l
+        // dW_l = X_lᵀ dZ_l and db_l = column sums of dZ_l, from the same tiles of dZ_l
         SbzGemm gw{};
         gw.A = h->x[l];
         gw.a_i = 1;
@@ -580,6 +775,8 @@ static int loss_and_grad(sbz_mlp* h, int B, float* loss_out) {
         gw.K = B;
         gw.epi = SBZ_EPI_LINEAR;
         gw.out_f = h->grad + h->off_w[l];
+        gw.accum = 1;
+        gw.bsum = h->grad + h->off_b[l];
         MCHK(gemm(h, gw));
         if (l == 0) break;
         // dZ_{l-1} = (dZ_l W_lᵀ) ⊙ GELU'(Z_{l-1})
@@ -597,13 +794,7 @@ static int loss_and_grad(sbz_mlp* h, int B, float* loss_out) {
         gd.zin = h->z[l - 1];
         gd.out_h = d_nxt;
         MCHK(gemm(h, gd));
-        hipLaunchKernelGGL(sbz_mlp_bf2f, dim3(256), dim3(256), 0, h->stream, d_nxt, (long long)B * K, h->df);
-        MCHK(hipGetLastError());
         std::swap(d_cur, d_nxt);
-    }
-    if (loss_out) {
-        MCHK(hipMemcpyAsync(loss_out, h->loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
-        MCHK(hipStreamSynchronize(h->stream));
     }
     return SBZ_OK;
 }
@@ -621,9 +812,12 @@ extern "C" int sbz_mlp_grad(sbz_mlp* h, const float* x, const float* t, int B, f
     for (size_t e = 0; e < xb.size(); ++e) xb[e] = sbz_f2bf(x[e]);
     MCHK(hipMemcpy(h->x[0], xb.data(), sizeof(bf16_t) * xb.size(), hipMemcpyHostToDevice));
     MCHK(hipMemcpy(h->tb, t, sizeof(float) * (size_t)B * nout, hipMemcpyHostToDevice));
-    int st = loss_and_grad(h, B, loss);
+    MCHK(hipMemsetAsync(h->loss, 0, sizeof(float), h->stream));
+    int st = loss_and_grad(h, B, h->loss);
     if (st != SBZ_OK) return st;
-    MCHK(hipMemcpy(grad, h->grad, sizeof(float) * h->n_par, hipMemcpyDeviceToHost));
+    MCHK(hipMemcpyAsync(grad, h->grad, sizeof(float) * h->n_par, hipMemcpyDeviceToHost, h->stream));
+    if (loss) MCHK(hipMemcpyAsync(loss, h->loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    MCHK(hipStreamSynchronize(h->stream));
     return SBZ_OK;
 }
 
@@ -665,14 +859,23 @@ extern "C" int sbz_mlp_train(sbz_mlp* h, const int* idx, int n_batches, int batc
     sc.evs.push_back(e0);
     MCHK(hipEventCreate(&e1));
     sc.evs.push_back(e1);
+    // every step's batch indices and loss slot on the device: the host enqueues all steps without
+    // waiting for any
+    int* d_idx = nullptr;
+    float* d_loss = nullptr;
+    MCHK(hipMalloc((void**)&d_idx, sizeof(int) * (size_t)n_batches * batch));
+    sc.bufs.push_back(d_idx);
+    MCHK(hipMalloc((void**)&d_loss, sizeof(float) * (size_t)n_batches));
+    sc.bufs.push_back(d_loss);
+    MCHK(hipMemcpyAsync(d_idx, idx, sizeof(int) * (size_t)n_batches * batch, hipMemcpyHostToDevice, h->stream));
+    MCHK(hipMemsetAsync(d_loss, 0, sizeof(float) * (size_t)n_batches, h->stream));
     MCHK(hipEventRecord(e0, h->stream));
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int s = 0; s < n_batches; ++s) {
-        MCHK(hipMemcpyAsync(h->idx, idx + (size_t)s * batch, sizeof(int) * batch, hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(sbz_mlp_gather, dim3(batch), dim3(256), 0, h->stream, h->data_x, h->data_t, h->idx, batch,
-                           h->w[0], h->w[h->L], h->x[0], h->tb);
+        hipLaunchKernelGGL(sbz_mlp_gather, dim3(batch), dim3(256), 0, h->stream, h->data_x, h->data_t,
+                           d_idx + (size_t)s * batch, batch, h->w[0], h->w[h->L], h->x[0], h->tb);
         MCHK(hipGetLastError());
-        int st = loss_and_grad(h, batch, loss ? loss + s : nullptr);
+        int st = loss_and_grad(h, batch, d_loss + s);
         if (st != SBZ_OK) return st;
         ++h->step;
         const float c1 = 1.f - std::pow(b1, (float)h->step), c2 = 1.f - std::pow(b2, (float)h->step);
@@ -682,7 +885,9 @@ extern "C" int sbz_mlp_train(sbz_mlp* h, const int* idx, int n_batches, int batc
         h->par_t_ok = false;
     }
     MCHK(hipEventRecord(e1, h->stream));
+    if (loss) MCHK(hipMemcpyAsync(loss, d_loss, sizeof(float) * (size_t)n_batches, hipMemcpyDeviceToHost, h->stream));
     MCHK(hipEventSynchronize(e1));
+    MCHK(hipStreamSynchronize(h->stream));
     float ms = 0.f;
     MCHK(hipEventElapsedTime(&ms, e0, e1));
     if (kernel_ms) *kernel_ms = ms;

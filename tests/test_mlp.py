@@ -60,6 +60,28 @@ def test_gpu_forward_and_gradient_match_oracle():
 
 
 @pytest.mark.gpu
+def test_gpu_gradient_at_training_size_matches_oracle():
+    # batch 1024 through 512-wide layers: the training products take the 16-byte-load kernel with
+    # 128-deep contraction steps, the weight gradients split along the batch and accumulated
+    # atomically with the bias gradients from the same tiles; the 7-wide input layer takes the
+    # small-contraction and small-output kernels
+    from symboltz.mlp import Mlp
+    print("case: MLP gradient 1024 x [7, 512, 512, 256]", flush=True)
+    widths = [7, 512, 512, 256]
+    net = Mlp(widths, max_batch=1024, seed=6)
+    p = net.get_params()
+    _, x, t = _problem(widths, 1024, 3)
+    for _ in range(2):   # the accumulating products start from zero on every call
+        g, loss = net.grad(x, t)
+        loss_ref, g_ref = mo.loss_and_grad(p, widths, x, t)
+        assert abs(loss - loss_ref) <= 1e-3 * loss_ref
+        assert np.linalg.norm(g - g_ref) <= 2e-2 * np.linalg.norm(g_ref)
+        nb = sum(widths[1:])
+        assert np.linalg.norm(g[-nb:] - g_ref[-nb:]) <= 2e-2 * np.linalg.norm(g_ref[-nb:])   # biases
+    net.close()
+
+
+@pytest.mark.gpu
 def test_gpu_large_batch_forward_matches_oracle():
     # 16384 rows: the hidden layers take the 128 x 128 MFMA kernel on the transposed weights
     # (sbz_mlp_gemm_kk); the input layer (K = 7) keeps the general kernel
