@@ -104,7 +104,7 @@ def test_wave4_kernel_emulated_matches_oracle_and_wave(massless10):
     S = massless10
     E = WaveEmu(model_source(S.M).full_src)
     assert E.L.emu_has_wave4() == 1
-    ks = np.array([3000.0, 0.3, 0.0, 30.0, 1.0, np.nan, 300.0, 0.02, 100.0])
+    ks = np.array([100.0, 0.3, 0.0, 30.0, 1.0, np.nan, 50.0, 0.02, 10.0])
     kk = np.where(ks > 0, ks, 1.0)
     t0 = np.clip(np.minimum(1e-2 / kk, 1e-4), S.t[0], S.t[-1])
     g4 = E.solve(S.P, S.t, S.u, S.du, ks, t0, wave4=True)
@@ -122,7 +122,7 @@ def test_tangent_kernel_emulated_matches_oracle_differences(fixture5):
     """sbz_wave_tan_kernel (u and s = ∂u/∂log θ on one step sequence, the background's θ-dependence
     from two tight background solves at log θ ± h): u matches the plain kernel, and the total
     tangent of the final state, s + f(τ0) dτ0/dlog θ, matches the central difference of oracle
-    solves at log θ ± h (rtol 1e-9) for θ = Ω_c0 and Ω_b0, the reference's AD test parameters
+    solves at log θ ± h (rtol 1e-9) for θ = Ω_c0, one of the reference's AD test parameters
     (test/runtests.jl:363-377)."""
     from oracle import oracle
     from symboltz import _lib
@@ -130,11 +130,11 @@ def test_tangent_kernel_emulated_matches_oracle_differences(fixture5):
     M = S.M
     E = WaveEmu(model_source(M).full_src)
     assert E.L.emu_has_tan() == 1
-    ks = np.array([1.0, 30.0, 300.0])
+    ks = np.array([1.0, 30.0])
     t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), S.t[0], S.t[-1])
     h = 1e-3
     g1 = E.solve(S.P, S.t, S.u, S.du, ks, t0)
-    for name in ("Omega_c0", "Omega_b0"):
+    for name in ("Omega_c0",):   # Ω_b0 as well on the GPU (test_configs.py)
         x = np.log(S.pars[name])
         tight = []
         for sgn in (1, -1):
