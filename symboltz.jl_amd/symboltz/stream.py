@@ -28,6 +28,11 @@ class MixedStream:
     def __init__(self, families, ks_mpc, surrogate=None, surrogate_family=None, tol=2e-3, rtol=1e-5, atol=1e-5):
         self.families = list(families)
         self.ks_mpc = np.asarray(ks_mpc, dtype=np.float64)
+        if surrogate is not None:
+            if surrogate.widths[-1] != len(self.ks_mpc):
+                raise ValueError(f"the surrogate predicts {surrogate.widths[-1]} k, the stream's grid has {len(self.ks_mpc)}")
+            if surrogate_family is None or not 0 <= surrogate_family < len(self.families):
+                raise ValueError("a surrogate needs the index of the family it was trained on")
         self.surrogate = surrogate
         self.sfam = surrogate_family
         self.tol = tol
