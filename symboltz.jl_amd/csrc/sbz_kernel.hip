@@ -1187,7 +1187,9 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                 }
                 __syncthreads();
                 sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
-                if (live) st_f++;
+                // pass 9 runs for a row that rejected when another row accepted: not counted, so
+                // a mode's statistics do not depend on the modes that share its wavefront
+                if (live && (s <= 8 || EEst <= 1.0)) st_f++;
                 __syncthreads();
                 if (s == 9) break;
                 if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_rhs_f, 16, c.lane, h, DT, Ks, ck)

@@ -63,7 +63,8 @@ def wave4_source(src):
         ("        if (live) st_j++;\n        __syncthreads();\n", "        if (live) st_j++;\n        __syncthreads();\n        " + T(7) + "\n"),
         ("                __syncthreads();\n                sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);\n",
          "                __syncthreads();\n                " + T(8) + "\n                sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);\n"),
-        ("                if (live) st_f++;\n                __syncthreads();\n", "                if (live) st_f++;\n                __syncthreads();\n                " + T(9) + "\n"),
+        ("                if (live && (s <= 8 || EEst <= 1.0)) st_f++;\n                __syncthreads();\n",
+         "                if (live && (s <= 8 || EEst <= 1.0)) st_f++;\n                __syncthreads();\n                " + T(9) + "\n"),
         ("            __syncthreads();\n            // solve W x = Ks:", "            __syncthreads();\n            " + T(10) + "\n            // solve W x = Ks:"),
         ("                L.C[c.lane] = y;\n            }\n            __syncthreads();\n", "                L.C[c.lane] = y;\n            }\n            __syncthreads();\n            " + T(11) + "\n"),
         ("                    L.X[c.lane] = x;\n                }\n            }\n            __syncthreads();\n",
