@@ -65,11 +65,17 @@ extern "C" int sbz_multi_create(const char* hip_src, const char* cache_path, sbz
         return SBZ_ERR_ARG;
     }
     *out = nullptr;
-    for (int i = 0; i < n_models; ++i)
+    for (int i = 0; i < n_models; ++i) {
         if (!models[i] || models[i]->device != models[0]->device) {
             sbz_set_error("sbz_multi_create: model %d is missing or on another device", i);
             return SBZ_ERR_ARG;
         }
+        for (int j = 0; j < i; ++j)
+            if (models[j] == models[i]) {   // one plan per family: a repeated model would run its plan twice
+                sbz_set_error("sbz_multi_create: models %d and %d are the same handle", j, i);
+                return SBZ_ERR_ARG;
+            }
+    }
     std::vector<char> code;
     bool have = cache_path && sbz_read_file(cache_path, code);
     if (!have) {
@@ -186,6 +192,9 @@ extern "C" int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans,
         return SBZ_OK;
     }
     MCHK(hipSetDevice(mm->device));
+    // work a plan still has on its model's stream (an sbz_plan_launch not yet waited for, a pending
+    // fetch) finishes before the launch writes the plan's outputs
+    for (int f = 0; f < n_plans; ++f) MCHK(hipStreamSynchronize(plans[f]->m->stream));
     if ((size_t)n > mm->cap) {
         if (mm->d_fam) MCHK(hipFree(mm->d_fam));
         if (mm->d_inst) MCHK(hipFree(mm->d_inst));
