@@ -90,6 +90,10 @@ typedef struct {
 #define SBZ_ALGO_WAVE2 3 /* SBZ_ALGO_WAVE compiled for two resident wavefronts per SIMD (throughput) */
 #define SBZ_ALGO_WAVE4 4 /* four modes per wavefront, 16 lanes each (throughput); models whose border and
                             chains fit 16 lanes (sbz_model_capacity reports 0 otherwise), no saves */
+#define SBZ_ALGO_WAVE_ALTLU 5 /* SBZ_ALGO_WAVE with the other layout of the border factors in LDS: the
+                                 model's default keeps strictly-lower and strictly-upper rows apart
+                                 (unmasked solves) unless that costs resident workgroups, in which case
+                                 it keeps one combined row per lane (masked solves, half the LDS) */
 
 /* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number of
  * resident wavefronts of the persistent grid (0: as many as fit the device).  save: SBZ_SAVE_* flags

@@ -24,6 +24,7 @@ struct sbz_model {
     hipFunction_t k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
     hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
     hipFunction_t k_wave4 = nullptr;   // null when the model's border or chains exceed 16 lanes
+    hipFunction_t k_wave_alt = nullptr;   // one mode per wavefront, the other border-factor layout
     hipFunction_t k_tan = nullptr, k_pk_tan = nullptr;   // null when the tangent kernel does not fit LDS
     hipStream_t stream = nullptr;
     int device = 0;

@@ -288,7 +288,8 @@ __constant__ int sbz_stage_bg[10] = {0, 0, 1, 2, 3, 4, 5, 5, 5, 5};  // stage ->
 // are dead while W is assembled at the start of a step attempt, and the staging is dead once the
 // rows are loaded (its loads mask every entry the assembly does not write).
 #define SBZ_M0_IN_K (SBZ_NBORD * SBZ_WSTRIDE <= 8 * SBZ_N)
-struct WaveLDS {
+template <bool SPLIT>
+struct WaveLDSt {
     double v[14][SBZ_N];           // 0,1 U (ping-pong), 2,3 F (ping-pong), 4 DT, 5 US, 6..13 K1..K8
     double B[7][SBZ_NB];           // background at the distinct stage times (0..5) and a spare slot (6)
     double dB[2][SBZ_NB];          // d/dτ of the background at t_n (0) and at the spare time (1)
@@ -296,13 +297,15 @@ struct WaveLDS {
 #if !SBZ_M0_IN_K
     double M0[SBZ_NBORD * SBZ_WSTRIDE];   // border staging
 #endif
-    double LU[(1 + SBZ_LU_SPLIT) * SBZ_NBORD * SBZ_WSTRIDE];   // factored border block (sbz_wb_store)
+    double LU[(1 + SPLIT) * SBZ_NBORD * SBZ_WSTRIDE];   // factored border block (sbz_wb_store<SPLIT>)
     double X[SBZ_NBORD];           // border solution broadcast
     double C[SBZ_NCHAIN];          // chain → border exchange
     double ddkd;
     int mode;
 };
-SBZ_DEV double* sbz_wave_m0(WaveLDS& L) {
+using WaveLDS = WaveLDSt<SBZ_LU_SPLIT>;   // the model's default factor layout
+template <bool SPLIT>
+SBZ_DEV double* sbz_wave_m0(WaveLDSt<SPLIT>& L) {
 #if SBZ_M0_IN_K
     return &L.v[6][0];
 #else
@@ -544,8 +547,8 @@ SBZ_DEV WaveCtx sbz_wave_ctx(int lane) {
 
 // Integrate instance `inst` from its start to τ0 (saves and sources on the way) and write its
 // final state, return code and statistics; every lane of the wavefront calls it.
-template <bool FUSED>
-SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, int inst) {
+template <bool FUSED, bool SPLIT = SBZ_LU_SPLIT>
+SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDSt<SPLIT>& L, const WaveCtx& c, int inst) {
     double* const DT = L.v[4];
     double* const US = L.v[5];
     double* const TMP = L.v[8];   // source scratch: K3 is dead between step attempts (rewritten at stage 3)
@@ -732,9 +735,9 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, 
             sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, sbz_wave_m0(L), row, cpl, al, be, wb);
             double rpo;   // reciprocal pivot of this lane's row
             bool ok = sbz_wb_lu<SbzBcWave>(c.lane, row, rpo);
-            sbz_wb_store(c.lane, row, rpo, L.LU);
+            sbz_wb_store<SPLIT>(c.lane, row, rpo, L.LU);
             // Woodbury data: z = A^-1 [α β] and C^-1 with C = I - [p q]^T z (2 x 2)
-            sbz_wb_tri2<SbzBcWave>(c.lane, L.LU, al, be);
+            sbz_wb_tri2<SPLIT, SbzBcWave>(c.lane, L.LU, al, be);
             wb.z1 = c.isB ? al : 0.0;
             wb.z2 = c.isB ? be : 0.0;
             {
@@ -789,7 +792,7 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, 
                     {
                         double b = c.isB ? Ks[c.bcomp] : 0.0;
                         if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                        sbz_wb_tri<SbzBcWave>(c.lane, L.LU, b);   // y = A^-1 b
+                        sbz_wb_tri<SPLIT, SbzBcWave>(c.lane, L.LU, b);   // y = A^-1 b
                         const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
                         const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                         if (c.isB) {
@@ -887,9 +890,9 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDS& L, const WaveCtx& c, 
     }
 }
 
-template <bool FUSED>
+template <bool FUSED, bool SPLIT = SBZ_LU_SPLIT>
 SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
-    __shared__ WaveLDS L;
+    __shared__ WaveLDSt<SPLIT> L;
     const WaveCtx c = sbz_wave_ctx(threadIdx.x);
     for (;;) {
         if (c.lane == 0) L.mode = sbz_pull(a, (int)(blockIdx.x % (unsigned)a.n_xcd));
@@ -897,7 +900,7 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
         const int qi = L.mode;
         __syncthreads();
         if (qi < 0) break;
-        sbz_wave_mode<FUSED>(a, L, c, a.order ? a.order[qi] : qi);
+        sbz_wave_mode<FUSED, SPLIT>(a, L, c, a.order ? a.order[qi] : qi);
     }
 }
 
@@ -1460,7 +1463,7 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
             {
                 double b = c.isB ? bv[c.bcomp] : 0.0;
                 if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                sbz_wb_tri<SbzBcWave>(c.lane, L.LU, b);
+                sbz_wb_tri<SBZ_LU_SPLIT, SbzBcWave>(c.lane, L.LU, b);
                 const double s1 = sbz_wave_sum(wb.p * b), s2 = sbz_wave_sum(wb.q * b);
                 const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                 if (c.isB) {
@@ -1525,8 +1528,8 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
             sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M0, row, cpl, al, be, wb);
             double rpo;
             bool ok = sbz_wb_lu<SbzBcWave>(c.lane, row, rpo);
-            sbz_wb_store(c.lane, row, rpo, L.LU);
-            sbz_wb_tri2<SbzBcWave>(c.lane, L.LU, al, be);
+            sbz_wb_store<SBZ_LU_SPLIT>(c.lane, row, rpo, L.LU);
+            sbz_wb_tri2<SBZ_LU_SPLIT, SbzBcWave>(c.lane, L.LU, al, be);
             wb.z1 = c.isB ? al : 0.0;
             wb.z2 = c.isB ? be : 0.0;
             {
@@ -1669,6 +1672,11 @@ SBZ_DEV void sbz_wave_tan_body(const SbzSolveArgs& a) {
 // SIMDs).  SBZ_ALGO_WAVE / SBZ_ALGO_WAVE2 select them.
 extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel(SbzSolveArgs a) { sbz_wave_body<true>(a); }
 extern "C" __global__ void __launch_bounds__(64, 2) sbz_wave_kernel_o2(SbzSolveArgs a) { sbz_wave_body<false>(a); }
+// The one-wavefront kernel with the other border-factor layout (split where the model's default is
+// combined and vice versa): fewer instructions per solve against less LDS (SBZ_ALGO_WAVE_ALTLU).
+extern "C" __global__ void __launch_bounds__(64) sbz_wave_kernel_altlu(SbzSolveArgs a) {
+    sbz_wave_body<true, !SBZ_LU_SPLIT>(a);
+}
 
 #if SBZ_TAN_OK
 // Tangent solve, one mode and its parameter derivative per wavefront.

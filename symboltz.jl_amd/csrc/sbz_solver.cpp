@@ -126,6 +126,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         return SBZ_ERR_COMPILE;
     }
     if (hipModuleGetFunction(&m->k_wave4, m->module, "sbz_wave4_kernel") != hipSuccess) m->k_wave4 = nullptr;
+    if (hipModuleGetFunction(&m->k_wave_alt, m->module, "sbz_wave_kernel_altlu") != hipSuccess) m->k_wave_alt = nullptr;
     if (hipModuleGetFunction(&m->k_tan, m->module, "sbz_wave_tan_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk_tan, m->module, "sbz_pk_tan_kernel") != hipSuccess)
         m->k_tan = m->k_pk_tan = nullptr;
@@ -197,6 +198,7 @@ hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsi
     void* kargs[] = {args};
     hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)             ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
+                       : (algo == SBZ_ALGO_WAVE_ALTLU)      ? m->k_wave_alt
                        : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
                                                             : m->k_wave;
     return hipModuleLaunchKernel(kf, units, 1, 1, 64, 1, 1, 0, m->stream, kargs, nullptr);
@@ -206,6 +208,7 @@ int sbz_resident_waves(sbz_model* m, int algo) {
     int per_cu = 0, ncu = 0;
     hipFunction_t kw = (algo == SBZ_ALGO_WAVE2)             ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
+                       : (algo == SBZ_ALGO_WAVE_ALTLU)      ? m->k_wave_alt
                        : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
                                                             : m->k_wave;
     if (!kw) return 0;
@@ -219,13 +222,14 @@ int sbz_resident_waves(sbz_model* m, int algo) {
 extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     sbz_clear_error();
     if (!m || !capacity ||
-        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_WAVE4)) {
+        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_WAVE4 && algo != SBZ_ALGO_WAVE_ALTLU)) {
         sbz_set_error("sbz_model_capacity: invalid argument");
         return SBZ_ERR_ARG;
     }
-    hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)   ? m->k_wave2
-                       : (algo == SBZ_ALGO_WAVE4) ? m->k_wave4
-                                                  : m->k_wave;
+    hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)        ? m->k_wave2
+                       : (algo == SBZ_ALGO_WAVE4)      ? m->k_wave4
+                       : (algo == SBZ_ALGO_WAVE_ALTLU) ? m->k_wave_alt
+                                                       : m->k_wave;
     if (!kf) {
         *capacity = 0;
         return SBZ_OK;
@@ -278,8 +282,14 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     p->save_flags = n_save > 0 ? opts->save : 0;
     if (n_save > 0 && !(p->save_flags & (SBZ_SAVE_STATES | SBZ_SAVE_SOURCES))) p->save_flags |= SBZ_SAVE_STATES;
     p->saveat_stride = per_set ? n_save : 0;
-    if (opts->algo != 0 && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2 && opts->algo != SBZ_ALGO_WAVE4) {
+    if (opts->algo != 0 && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2 && opts->algo != SBZ_ALGO_WAVE4 &&
+        opts->algo != SBZ_ALGO_WAVE_ALTLU) {
         sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
+        delete p;
+        return SBZ_ERR_ARG;
+    }
+    if (opts->algo == SBZ_ALGO_WAVE_ALTLU && !m->k_wave_alt) {
+        sbz_set_error("sbz_plan_create: the model's code object has no SBZ_ALGO_WAVE_ALTLU kernel");
         delete p;
         return SBZ_ERR_ARG;
     }

@@ -3,18 +3,22 @@
 For a plan it picks among the wavefront kernels:
 - SBZ_ALGO_WAVE: one mode per wavefront, one resident wavefront per SIMD, every register
   available: the lowest latency per mode;
-- SBZ_ALGO_WAVE2: the same with two resident per SIMD: the second hides the first's latency once
-  there are more instances than SIMDs;
+- SBZ_ALGO_WAVE_ALTLU: the same with the other LDS layout of the border factors (split rows with
+  unmasked solves, or combined rows with half the LDS; the model's default is split unless that
+  costs resident workgroups): fewer instructions per step against more modes in flight;
+- SBZ_ALGO_WAVE2: one mode per wavefront, two resident per SIMD: the second hides the first's
+  latency once there are more instances than SIMDs;
 - SBZ_ALGO_WAVE4: four modes per wavefront (16 lanes each), for models whose border and chains
   fit 16 lanes and plans that save nothing: the per-step work that the lanes of a mode share
   (border RHS and Jacobian assembly, background) serves four modes.
 
-When the instances fit the one-per-SIMD capacity the choice is WAVE without measuring: the
-launch is then the latency of its longest mode.  Otherwise each candidate is timed on a
-cost-stratified sample of the plan's own instances (every m-th of the instances sorted by k)
-large enough (8 × the largest capacity, or all) that its time is set by throughput rather than
-by its single longest mode, and the fastest is kept.  Choices are cached per model,
-instance-count bucket and output kind.
+When the instances fit the one-per-SIMD capacity the launch is the latency of its longest mode:
+the two factor layouts are timed on the 64 longest instances (when the other layout holds them
+all at once) and the faster kept.  Otherwise every variant is timed on a cost-stratified sample
+of the plan's own instances (every m-th of the instances sorted by k) large enough (8 × the
+largest capacity, or all) that its time is set by throughput rather than by its single longest
+mode, and the fastest is kept.  Choices are cached per model, instance-count bucket and output
+kind.
 """
 import ctypes
 import math
@@ -23,8 +27,8 @@ import numpy as np
 
 from . import _lib
 
-ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4 = 1, 3, 4
-NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4"}
+ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_WAVE_ALTLU = 1, 3, 4, 5
+NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4", ALGO_WAVE_ALTLU: "WAVE_ALTLU"}
 _cache = {}
 last_times = {}
 
@@ -46,18 +50,27 @@ def choose(gm, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=N
     if key in _cache:
         return _cache[key]
     caps = {ALGO_WAVE: capacity(gm, ALGO_WAVE), ALGO_WAVE2: capacity(gm, ALGO_WAVE2)}
+    c_alt = capacity(gm, ALGO_WAVE_ALTLU)
+    if c_alt > 0:
+        caps[ALGO_WAVE_ALTLU] = c_alt
     if n_save == 0:
         c4 = capacity(gm, ALGO_WAVE4)
         if c4 > 0:
             caps[ALGO_WAVE4] = c4
-    if n <= caps[ALGO_WAVE] or max(caps.values()) <= caps[ALGO_WAVE]:
-        _cache[key] = ALGO_WAVE
-        return ALGO_WAVE
     order = np.argsort(-np.asarray(ks), kind="stable")
-    m = max(1, n // (8 * max(caps.values())))
-    sel = np.sort(order[::m])
+    if n <= caps[ALGO_WAVE] or max(caps.values()) <= caps[ALGO_WAVE]:
+        # latency-bound: the longest modes set the launch; only the factor layout can matter
+        cands = [ALGO_WAVE] + ([ALGO_WAVE_ALTLU] if caps.get(ALGO_WAVE_ALTLU, 0) >= n else [])
+        if len(cands) == 1:
+            _cache[key] = ALGO_WAVE
+            return ALGO_WAVE
+        sel = np.sort(order[:min(n, 64)])
+    else:
+        cands = list(caps)
+        m = max(1, n // (8 * max(caps.values())))
+        sel = np.sort(order[::m])
     times = {}
-    for algo in caps:
+    for algo in cands:
         plan = gm.plan(np.asarray(inst_set)[sel], np.asarray(ks)[sel], np.asarray(t0)[sel], rtol=rtol, atol=atol,
                        maxiters=maxiters, saveat=saveat, save=save, algo=algo)
         times[algo] = plan.run()

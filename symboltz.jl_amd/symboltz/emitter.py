@@ -899,45 +899,45 @@ class Emitted:
         lu.append("    return ok;")
         lu.append("}")
         out += lu
-        # The factors go to LDS in one of two forms, whichever keeps the one-mode kernel's LDS at
-        # the most resident workgroups per CU (4, one per SIMD, needs <= 40 KB):
-        #   split (SBZ_LU_SPLIT 1): strictly-lower rows (sL) and strictly-upper rows plus the
-        #     reciprocal pivot in column nb (sU = sL + nb·ws), masked once when stored, so the
-        #     solves (nine per step) load them unmasked;
-        #   combined (0): one row per lane, L left of the diagonal, U right of it, 1/pivot in column
-        #     nb: half the LDS, the solves mask each entry as they load it.
+        # The factors go to LDS in one of two layouts; both are emitted (template argument SPLIT):
+        #   split: strictly-lower rows (sL) and strictly-upper rows plus the reciprocal pivot in
+        #     column nb (sU = sL + nb·ws), masked once when stored, so the solves (nine per step)
+        #     load them unmasked;
+        #   combined: one row per lane, L left of the diagonal, U right of it, 1/pivot in column nb:
+        #     half the LDS, the solves mask each entry as they load it.
+        # SBZ_LU_SPLIT is the default: split unless it would lower the one-mode kernel's resident
+        # workgroups per CU (4, one per SIMD, needs <= 40 KB); the other layout is the kernel
+        # variant SBZ_ALGO_WAVE_ALTLU, which the autotuner may choose.
         wl_doubles = 14 * n + 9 * NB + len(C_NAMES) + nb + len(chains) + 1 + (0 if nb * ws <= 8 * n else nb * ws)
         occ = lambda b: min(4, 163840 // b)
         split = occ(8 * (wl_doubles + 2 * nb * ws) + 16) >= occ(8 * (wl_doubles + nb * ws) + 16)
         self.wave["lu_split"] = split
         out.append(f"#define SBZ_LU_SPLIT {1 if split else 0}")
-        st = ["SBZ_DEV void sbz_wb_store(int lane, const double* row, double rpo, double* sLU) {",
-              "    if (lane < SBZ_NBORD) {"]
-        if split:
-            st += ["        double* lr = sLU + lane * SBZ_WSTRIDE;",
-                   "        double* ur = sLU + (SBZ_NBORD + lane) * SBZ_WSTRIDE;"]
-            for j in range(nb):
-                st.append(f"        lr[{j}] = ({j} < lane) ? row[{j}] : 0.0; ur[{j}] = ({j} > lane) ? row[{j}] : 0.0;")
-            st.append("        ur[SBZ_NBORD] = rpo;")
-        else:
-            st.append("        double* mr = sLU + lane * SBZ_WSTRIDE;")
-            for j in range(nb):
-                st.append(f"        mr[{j}] = row[{j}];")
-            st.append("        mr[SBZ_NBORD] = rpo;")
-        st.append("    }")
-        st.append("    __syncthreads();")
-        st.append("}")
+        st = ["template <bool SPLIT>",
+              "SBZ_DEV void sbz_wb_store(int lane, const double* row, double rpo, double* sLU) {",
+              "    if (lane < SBZ_NBORD) {",
+              "        if constexpr (SPLIT) {",
+              "            double* lr = sLU + lane * SBZ_WSTRIDE;",
+              "            double* ur = sLU + (SBZ_NBORD + lane) * SBZ_WSTRIDE;"]
+        for j in range(nb):
+            st.append(f"            lr[{j}] = ({j} < lane) ? row[{j}] : 0.0; ur[{j}] = ({j} > lane) ? row[{j}] : 0.0;")
+        st += ["            ur[SBZ_NBORD] = rpo;",
+               "        } else {",
+               "            double* mr = sLU + lane * SBZ_WSTRIDE;"]
+        for j in range(nb):
+            st.append(f"            mr[{j}] = row[{j}];")
+        st += ["            mr[SBZ_NBORD] = rpo;",
+               "        }",
+               "    }",
+               "    __syncthreads();",
+               "}"]
         out += st
 
         # A^-1 applied to NR right-hand sides at once (lane r holds entry r of each), level
         # scheduled: x_j is broadcast with readlane and every lane subtracts its (possibly zero)
         # factor entry times x_j
-        def tri(name, nr):
-            args = ", ".join(f"double& b{q}" for q in range(nr))
-            so = ["template <class BC>",
-                  f"SBZ_DEV void {name}(int lane, const double* __restrict__ sLU, {args}) {{",
-                  "    const bool own = lane < SBZ_NBORD;   // lanes past the border use zero factors",
-                  "    const int r = own ? lane : SBZ_NBORD - 1;"]
+        def tri_body(split, nr):
+            so = []
             if split:
                 so += ["    const double* lr = sLU + r * SBZ_WSTRIDE;",
                        "    const double* ur = sLU + (SBZ_NBORD + r) * SBZ_WSTRIDE;",
@@ -985,7 +985,19 @@ class Emitted:
                 so.append("    }")
             for q in range(nr):
                 so.append(f"    b{q} *= rpr;")
-            so.append("}")
+            return so
+
+        def tri(name, nr):
+            args = ", ".join(f"double& b{q}" for q in range(nr))
+            so = ["template <bool SPLIT, class BC>",
+                  f"SBZ_DEV void {name}(int lane, const double* __restrict__ sLU, {args}) {{",
+                  "    const bool own = lane < SBZ_NBORD;   // lanes past the border use zero factors",
+                  "    const int r = own ? lane : SBZ_NBORD - 1;",
+                  "    if constexpr (SPLIT) {"]
+            so += ["    " + ln for ln in tri_body(True, nr)]
+            so.append("    } else {")
+            so += ["    " + ln for ln in tri_body(False, nr)]
+            so += ["    }", "}"]
             return so
         out += tri("sbz_wb_tri", 1)
         out += tri("sbz_wb_tri2", 2)

@@ -65,6 +65,7 @@ class _Opts(ctypes.Structure):
 ALGO_WAVE = 1
 ALGO_WAVE2 = 3
 ALGO_WAVE4 = 4   # four modes per wavefront (models whose border and chains fit 16 lanes; no saves)
+ALGO_WAVE_ALTLU = 5   # ALGO_WAVE with the other layout of the border factors in LDS
 ALGO_AUTO = -1   # host-side: symboltz.autotune picks among the wavefront variants
 
 

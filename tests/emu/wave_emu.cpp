@@ -173,7 +173,8 @@ static inline double __longlong_as_double(long long x) {
 #include "emu_kernel.inc"
 
 // 0: sbz_wave_kernel (fused stage combinations), 1: sbz_wave_kernel_o2 (two-pass combinations),
-// 2: sbz_wave4_kernel (four modes per wavefront; models whose border and chains fit 16 lanes)
+// 2: sbz_wave4_kernel (four modes per wavefront; models whose border and chains fit 16 lanes),
+// 3: sbz_wave_kernel_altlu (the other LDS layout of the border factors)
 static int g_o2 = 0;
 extern "C" void emu_set_o2(int on) { g_o2 = on; }
 extern "C" int emu_has_wave4(void) { return SBZ_WAVE4_OK; }
@@ -184,7 +185,8 @@ static void wave_body(void* a) {
         return;
     }
 #endif
-    if (g_o2) sbz_wave_kernel_o2(*(SbzSolveArgs*)a);
+    if (g_o2 == 3) sbz_wave_kernel_altlu(*(SbzSolveArgs*)a);
+    else if (g_o2) sbz_wave_kernel_o2(*(SbzSolveArgs*)a);
     else sbz_wave_kernel(*(SbzSolveArgs*)a);
 }
 

@@ -19,8 +19,8 @@ import pytest
 from conftest import Setup
 from symboltz import _lib, models, workloads
 from symboltz.constants import k_dimensionless
-from symboltz.problem import CosmologyProblem, spectrum_matter
-from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_WAVE2, ALGO_AUTO, SAVE_SOURCES, SAVE_STATES
+from symboltz.problem import CosmologyProblem, solvebg, spectrum_matter
+from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE_ALTLU, ALGO_AUTO, SAVE_SOURCES, SAVE_STATES
 
 pytestmark = pytest.mark.gpu
 
@@ -353,3 +353,36 @@ def test_forward_mode_sensitivities_match_differences():
         Jo = (lp[0] - lp[1]) / (2 * h)
         say(f"{name}: max |J_tangent - J_oracle| = {np.abs(Jt[:, j] - Jo).max():.2e} (|J| up to {np.abs(Jo).max():.2f})")
         assert np.max(np.abs(Jt[:, j] - Jo)) < 1e-3
+
+
+def test_factor_layouts_agree_and_autotuner_times_them():
+    """The one-wavefront kernel in both LDS layouts of the border factors (ALGO_WAVE: the model's
+    default; ALGO_WAVE_ALTLU: the other) gives bit-identical states and statistics: the same factor
+    values meet the same operations.  n = 236 (default combined rows, 4 workgroups per CU) and
+    n = 38 (default split rows); below the one-per-SIMD capacity the autotuner times the two
+    layouts on the longest modes and keeps one of them."""
+    from symboltz import autotune
+    for M in (models.LCDM(lmax=32), models.LCDM(lmax=10, h=False)):
+        say(f"case: n = {M.n_unknowns}, 24 k, ALGO_WAVE vs ALGO_WAVE_ALTLU")
+        pars = models.parameters_Planck18(M)
+        bg = solvebg(CosmologyProblem(M, pars))
+        gm = GpuModel(M)
+        gm.upload_background(0, bg.t, bg.u, bg.du, bg.P)
+        ks = np.logspace(-1, 3.5, 24)
+        t0 = np.clip(np.minimum(1e-2 / ks, 1e-4), bg.t[0], bg.t[-1])
+        iset = np.zeros(len(ks), np.int32)
+        out = []
+        for algo in (ALGO_WAVE, ALGO_WAVE_ALTLU):
+            plan = gm.plan(iset, ks, t0, algo=algo)
+            plan.run()
+            out.append(plan.fetch())
+            plan.close()
+        assert np.all(out[0]["retcode"] == 0)
+        np.testing.assert_array_equal(out[1]["u"], out[0]["u"])
+        np.testing.assert_array_equal(out[1]["stats"], out[0]["stats"])
+        autotune._cache.clear()
+        plan = gm.plan(iset, ks, t0, algo=ALGO_AUTO)
+        say(f"autotuner: {autotune.last_times} -> {plan.algo}")
+        assert plan.algo in (ALGO_WAVE, ALGO_WAVE_ALTLU)
+        plan.close()
+        gm.close()
