@@ -395,6 +395,60 @@ def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=
     return pk, res, bgs
 
 
+_multi_models = {}
+
+
+def spectrum_matter_problems(probs, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=1e-4, timing=None):
+    """P(k, τ0) [n_probs][n_k] of problems of one or several models in ONE persistent launch: the
+    problems of each model batched as `solve_sets` does, and the modes of all models in one queue,
+    longest first (symboltz.multi).  The batched form of calling `spectrum_matter` for every problem
+    of a mixed list.  Per-mode failures warn as `solve` does and give NaN."""
+    from .multi import MultiModel
+    ks = _check_ks(ks)
+    bgopts = dict(reltol=1e-7, abstol=1e-7) | (bgopts or {})
+    ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
+    models = []
+    for q in probs:
+        if q.M not in models:
+            models.append(q.M)
+    if len(models) > 8:
+        raise ValueError("one launch holds at most 8 models")
+    gms = [_gpu_model(M) for M in models]
+    key = tuple(M.key() for M in models)
+    if key not in _multi_models:
+        _multi_models[key] = MultiModel(gms)
+    mm = _multi_models[key]
+    plans, where = [], []
+    for M, gm in zip(models, gms):
+        idx = [i for i, q in enumerate(probs) if q.M == M]
+        bgs = solvebg_sets([probs[i] for i in idx], bgopts["reltol"], bgopts["abstol"])
+        for s, bg in enumerate(bgs):
+            gm.upload_background(s, bg.t, bg.u, bg.du, bg.P)
+        inst_set = np.repeat(np.arange(len(idx), dtype=np.int32), len(ks))
+        inst_k = np.tile(ks, len(idx))
+        inst_t0 = np.concatenate([np.clip(np.minimum(ktini / ks, tauinimax), bg.t[0], bg.t[-1]) for bg in bgs])
+        plans.append(gm.plan(inst_set, inst_k, inst_t0, rtol=ptopts["reltol"], atol=ptopts["abstol"],
+                             maxiters=ptopts["maxiters"]))
+        where.append(idx)
+    out = np.full((len(probs), len(ks)), np.nan)
+    try:
+        ms = mm.run(plans)
+        if timing is not None:
+            timing["kernel_ms"] = ms
+        for plan, idx in zip(plans, where):
+            pk = plan.pk().reshape(len(idx), len(ks))
+            rc = plan.fetch()["retcode"].reshape(len(idx), len(ks))
+            for j, i in enumerate(idx):
+                for q in np.nonzero(rc[j])[0]:
+                    warnings.warn(f"Perturbation (mode k = {ks[q]}) solution failed with return code "
+                                  f"{RETCODES[int(rc[j][q])]}.")
+                out[i] = np.where(rc[j] == 0, pk[j], np.nan)
+    finally:
+        for plan in plans:
+            plan.close()
+    return out
+
+
 def spectrum_matter_jacobian(prob, ks, names, relstep=1e-3, ktini=1e-2, tauinimax=1e-4, bgopts=None, ptopts=None,
                              timing=None):
     """∂ log P(k) / ∂ log θ for the parameters `names` (e.g. ["Omega_c0", "Omega_b0"]) at the

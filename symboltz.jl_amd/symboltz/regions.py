@@ -85,7 +85,8 @@ def wave4_source(src):
 def wave_source(src):
     """The model source with sbz_wave_mode instrumented; totals per instance."""
     i0 = src.index("SBZ_DEV void sbz_wave_mode(")
-    i1 = src.index("\ntemplate <bool FUSED>\nSBZ_DEV void sbz_wave_body", i0)
+    i1 = src.index("SBZ_DEV void sbz_wave_body", i0)
+    i1 = src.rindex("\ntemplate <", i0, i1)   # the template line of sbz_wave_body
     T = _tick
     body = _patch(src[i0:i1], [
         ("        int bidx = kidx;   // knot cursor", f"        long long PR[{NR}] = {{}}; long long tq = clock64();\n        int bidx = kidx;   // knot cursor"),

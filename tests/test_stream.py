@@ -93,3 +93,23 @@ def test_stream_routes_requests_between_surrogate_and_solver():
         st.close()
         plain.close()
         S.close()
+
+
+def test_spectrum_matter_problems_one_launch_matches_per_problem():
+    """The problem-level entry: problems of three models in one launch give the P(k) of
+    `spectrum_matter` on each problem (the same arithmetic per mode)."""
+    from symboltz import models
+    from symboltz.problem import CosmologyProblem, remake, spectrum_matter, spectrum_matter_problems
+    fams = [models.LCDM(lmax=10, h=False), models.w0waCDM(lmax=6), models.LCDM(lmax=6)]
+    probs = []
+    for M in fams:
+        base = CosmologyProblem(M, models.parameters_Planck18(M))
+        probs += [base, remake(base, {"h": 0.7})]
+    ks = np.logspace(0, 3, 8)
+    say(f"case: {len(probs)} problems over {len(fams)} models, {len(ks)} k, one launch")
+    P = spectrum_matter_problems(probs, ks)
+    for i, q in enumerate(probs):
+        Pi = spectrum_matter(q, ks)
+        Pi = Pi[0] if isinstance(Pi, tuple) else Pi
+        err = float(np.max(np.abs(P[i] / np.asarray(Pi) - 1)))
+        assert err < 1e-12, (i, err)
