@@ -170,6 +170,10 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--generate-only", action="store_true")
     ap.add_argument("--tol", type=float, default=5e-3, help="fallback threshold on member disagreement in log P")
+    ap.add_argument("--stream", type=int, default=0,
+                    help="then run a mixed-model request stream of this many requests with the trained surrogate")
+    ap.add_argument("--stream-out-of-box", type=float, default=0.1,
+                    help="fraction of the surrogate family's stream requests placed outside its box")
     args = ap.parse_args()
     log = lambda *a: print(*a, file=sys.stderr, flush=True)   # noqa: E731
     if args.data:
@@ -224,8 +228,54 @@ def main():
         t_eff = args.infer_batch / res["surrogate_spectra_per_s"] + fb.mean() * args.infer_batch / res["plain_spectra_per_s"]
         res["with_fallback_spectra_per_s"] = args.infer_batch / t_eff
         res["with_fallback_speedup_vs_plain"] = res["with_fallback_spectra_per_s"] / res["plain_spectra_per_s"]
+    if args.stream:
+        res["stream"] = stream_run(S, args.stream, args.stream_out_of_box, args.tol, log)
     print(json.dumps(res), flush=True)
     S.close()
+
+
+def stream_run(S, n, out_of_box, tol, log):
+    """The mixed-model request stream with this surrogate: n requests over the five families of
+    `workloads.mixed_families` (256 k each), the surrogate answering the in-box, agreed requests of
+    its family and one persistent launch solving every other request; then the answered requests
+    solved by the plain solver, the judge of the surrogate's error."""
+    from . import workloads
+    from .stream import MixedStream
+    fams = workloads.mixed_families()
+    sf = workloads.MIXED_SURROGATE_FAMILY
+    if fams[sf] != surrogate_model():
+        raise RuntimeError("the stream's surrogate family is not the surrogate's model")
+    ks = k_grid(S.widths[-1])
+    reqs = workloads.mixed_requests(n, seed=0, out_of_box=out_of_box)
+    st = MixedStream(fams, ks, surrogate=S, surrogate_family=sf, tol=tol)
+    plain = MixedStream(fams, ks)
+    try:
+        st.solve(reqs[:2 * len(fams)])                          # module loads and first-call costs
+        t0 = time.perf_counter()
+        lp, ans, ret, tm = st.solve(reqs)
+        wall = time.perf_counter() - t0
+        log(f"stream: {n} requests in {wall:.2f} s, {int(ans.sum())} answered by the surrogate; {tm}")
+        idx = np.flatnonzero(ans)
+        err = 0.0
+        if len(idx):
+            lp0, _, ret0, _ = plain.solve([reqs[i] for i in idx])
+            if not np.all(ret0 == 0):
+                raise RuntimeError("a plain solve of an answered request failed")
+            err = float(np.abs(np.exp(lp[idx] - lp0) - 1.0).max())
+        fam = np.array([r[0] for r in reqs])
+        return {
+            "requests": n, "families": [M.n_unknowns for M in fams], "n_k": len(ks),
+            "surrogate_family": sf, "requests_of_surrogate_family": int((fam == sf).sum()),
+            "answered_by_surrogate": int(ans.sum()),
+            "out_of_box_requests": int(sum(1 for r in reqs if r[0] == sf and r[2] is not None
+                                           and not np.all((r[2] >= LO) & (r[2] <= HI)))),
+            "solved_modes": int((~ans).sum()) * len(ks), "all_success": bool(np.all(ret == 0)),
+            "wall_s": wall, "spectra_per_s": n / wall, "stage_ms": {k: float(v) for k, v in tm.items()},
+            "answered_max_rel_err_vs_solver": err, "tol_logP": tol,
+        }
+    finally:
+        plain.close()
+        st.close()
 
 
 if __name__ == "__main__":
