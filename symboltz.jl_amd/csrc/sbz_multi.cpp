@@ -36,6 +36,7 @@ struct sbz_multi {
     size_t cap = 0;
     unsigned int* d_queue = nullptr;
     int resident = 0;
+    std::vector<int> h_fam, h_inst;   // host side of the entry tables: alive while their copies are in flight
 };
 
 #define MCHK(x)                                                                                   \
@@ -195,6 +196,7 @@ extern "C" int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans,
     // work a plan still has on its model's stream (an sbz_plan_launch not yet waited for, a pending
     // fetch) finishes before the launch writes the plan's outputs
     for (int f = 0; f < n_plans; ++f) MCHK(hipStreamSynchronize(plans[f]->m->stream));
+    MCHK(hipStreamSynchronize(mm->stream));   // copies of an earlier call that stopped on an error
     if ((size_t)n > mm->cap) {
         if (mm->d_fam) MCHK(hipFree(mm->d_fam));
         if (mm->d_inst) MCHK(hipFree(mm->d_inst));
@@ -204,13 +206,14 @@ extern "C" int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans,
         MCHK(hipMalloc((void**)&mm->d_inst, sizeof(int) * (size_t)n));
         mm->cap = (size_t)n;
     }
-    std::vector<int> hf(n), hi(n);
+    mm->h_fam.resize(n);
+    mm->h_inst.resize(n);
     for (int g = 0; g < n; ++g) {
-        hf[g] = ent[g].fam;
-        hi[g] = ent[g].inst;
+        mm->h_fam[g] = ent[g].fam;
+        mm->h_inst[g] = ent[g].inst;
     }
-    MCHK(hipMemcpyAsync(mm->d_fam, hf.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, mm->stream));
-    MCHK(hipMemcpyAsync(mm->d_inst, hi.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, mm->stream));
+    MCHK(hipMemcpyAsync(mm->d_fam, mm->h_fam.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, mm->stream));
+    MCHK(hipMemcpyAsync(mm->d_inst, mm->h_inst.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, mm->stream));
     MCHK(hipMemsetAsync(mm->d_queue, 0, sizeof(unsigned int), mm->stream));
     SbzMultiArgsHost a;
     std::memset(&a, 0, sizeof(a));
