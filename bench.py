@@ -383,10 +383,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massless10", "cmb", "bg", "mixed", "mlp_train"])
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--algo", choices=["auto", "wave", "wave2", "wave4", "wave_altlu"], default="auto",
+    ap.add_argument("--algo", choices=["auto", "wave", "wave2", "wave4", "wave2h", "wave_altlu"], default="auto",
                     help="auto: timed choice of the kernel variant (default); wave: one wavefront per "
                          "k-mode, one resident per SIMD; wave2: two resident per SIMD; wave4: four k-modes per "
-                         "wavefront")
+                         "wavefront; wave2h: two k-modes per wavefront; wave_altlu: wave with the other border-factor "
+                         "layout")
     ap.add_argument("--graph", action="store_true", help="replay each step from a captured HIP graph")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="oracle baseline: every N-th instance (0: all for one set, every 4th for many sets)")
@@ -420,7 +421,8 @@ def main():
         raise SystemExit(2)
 
     from symboltz import _lib, workloads
-    from symboltz.solver import GpuModel, ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_WAVE_ALTLU, ALGO_AUTO, SAVE_SOURCES
+    from symboltz.solver import (GpuModel, ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_WAVE2H, ALGO_WAVE_ALTLU, ALGO_AUTO,
+                                 SAVE_SOURCES)
 
     if _lib.device_count() < 1:
         raise SystemExit("bench.py: no GPU visible")
@@ -436,16 +438,17 @@ def main():
     gm = GpuModel(M)
     bgs, inst_set, ks, t0, t_bg = prepare(args, M, sets, ks_mpc, gm)
     bg = bgs[0]
-    algo = {"auto": ALGO_AUTO, "wave": ALGO_WAVE, "wave2": ALGO_WAVE2, "wave4": ALGO_WAVE4,
+    algo = {"auto": ALGO_AUTO, "wave": ALGO_WAVE, "wave2": ALGO_WAVE2, "wave4": ALGO_WAVE4, "wave2h": ALGO_WAVE2H,
             "wave_altlu": ALGO_WAVE_ALTLU}[args.algo]
     taus = None
     if not args.no_sources and args.workload == "massive_nu100":
         taus = workloads.source_taus(bg.t[0], bg.t[-1], args.n_tau)
     plan = gm.plan(inst_set, ks, t0, lanes=args.lanes, algo=algo, saveat=taus,
                    save=SAVE_SOURCES if taus is not None else 0)
-    chosen = {0: "wave", ALGO_WAVE: "wave", ALGO_WAVE2: "wave2", ALGO_WAVE4: "wave4", ALGO_WAVE_ALTLU: "wave_altlu"}[plan.algo]
+    chosen = {0: "wave", ALGO_WAVE: "wave", ALGO_WAVE2: "wave2", ALGO_WAVE4: "wave4", ALGO_WAVE2H: "wave2h",
+              ALGO_WAVE_ALTLU: "wave_altlu"}[plan.algo]
     kname = {"wave": "sbz_wave_kernel", "wave2": "sbz_wave_kernel_o2", "wave4": "sbz_wave4_kernel",
-             "wave_altlu": "sbz_wave_kernel_altlu"}[chosen]
+             "wave2h": "sbz_wave2h_kernel", "wave_altlu": "sbz_wave_kernel_altlu"}[chosen]
     from symboltz import autotune
     if args.algo == "auto" and autotune.last_times:
         log("autotune sample times (ms): " + ", ".join(f"{k} {v:.1f}" for k, v in autotune.last_times.items()))

@@ -94,6 +94,9 @@ typedef struct {
                                  model's default keeps strictly-lower and strictly-upper rows apart
                                  (unmasked solves) unless that costs resident workgroups, in which case
                                  it keeps one combined row per lane (masked solves, half the LDS) */
+#define SBZ_ALGO_WAVE2H 6 /* two modes per wavefront, 32 lanes each (throughput); models without the
+                             four-mode kernel whose border and chains fit 32 lanes
+                             (sbz_model_capacity reports 0 otherwise), no saves */
 
 /* Solver options (solvept keywords: reltol, abstol, maxiters; src/solve.jl:543).  lanes: number of
  * resident wavefronts of the persistent grid (0: as many as fit the device).  save: SBZ_SAVE_* flags
@@ -163,8 +166,8 @@ int sbz_plan_pk(sbz_plan* p, double* pk);
 int sbz_plan_pk_modes(sbz_plan* p, const int* masks, int n_masks, int at_saves, double* out);
 int sbz_plan_destroy(sbz_plan* p);
 /* Resident capacity of a kernel variant on the model's device: modes that run at once (one per
- * wavefront for SBZ_ALGO_WAVE and SBZ_ALGO_WAVE2, four per wavefront for SBZ_ALGO_WAVE4), and 0 for
- * SBZ_ALGO_WAVE4 when the model has no such kernel.  No reference counterpart: the
+ * wavefront for SBZ_ALGO_WAVE and SBZ_ALGO_WAVE2, four per wavefront for SBZ_ALGO_WAVE4, two for
+ * SBZ_ALGO_WAVE2H), and 0 for SBZ_ALGO_WAVE4 / SBZ_ALGO_WAVE2H when the model has no such kernel.  No reference counterpart: the
  * autotuner (symboltz/autotune.py) uses it to decide whether higher occupancy can pay off. */
 int sbz_model_capacity(sbz_model* m, int algo, int* capacity);
 

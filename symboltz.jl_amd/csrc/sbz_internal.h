@@ -24,6 +24,7 @@ struct sbz_model {
     hipFunction_t k_pk = nullptr, k_wave = nullptr, k_wave2 = nullptr, k_pk_modes = nullptr;
     hipFunction_t k_srcbg = nullptr, k_fill = nullptr, k_qreset = nullptr;
     hipFunction_t k_wave4 = nullptr;   // null when the model's border or chains exceed 16 lanes
+    hipFunction_t k_wave2h = nullptr;  // two modes per wavefront; null unless the model fits 32 lanes only
     hipFunction_t k_wave_alt = nullptr;   // one mode per wavefront, the other border-factor layout
     hipFunction_t k_tan = nullptr, k_pk_tan = nullptr;   // null when the tangent kernel does not fit LDS
     hipStream_t stream = nullptr;
@@ -74,7 +75,7 @@ constexpr int SBZ_ALGO_TANGENT_INTERNAL = 100;
 constexpr int SBZ_N_XCD = 8;
 
 // Launch the persistent solve kernel of `algo` with `args` on the model's stream: grid `units`
-// wavefronts (four modes each for SBZ_ALGO_WAVE4).
+// wavefronts (four modes each for SBZ_ALGO_WAVE4, two for SBZ_ALGO_WAVE2H).
 hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsigned units);
 // Resident wavefronts of a wave variant on the model's device (the persistent grid size).
 int sbz_resident_waves(sbz_model* m, int algo);

@@ -7,6 +7,7 @@
 // emitted for this model and spliced in at the @@SBZ_MODEL@@ marker by symboltz.emitter:
 //   sbz_wave_kernel, sbz_wave_kernel_o2   one mode per 64-lane wavefront (one / two per SIMD)
 //   sbz_wave4_kernel                      four modes per wavefront, 16 lanes each
+//   sbz_wave2h_kernel                     two modes per wavefront, 32 lanes each
 //   sbz_wave_tan_kernel                   one mode and its parameter tangent per wavefront
 // plus the source background table, P(k) and output-fill kernels.  Background spline tables (one
 // per parameter set, shared by all its modes) are read through the cache.
@@ -49,6 +50,15 @@ struct SbzBcWave {
 struct SbzBcRow16 {
     template <int J>
     static SBZ_DEV double at(double v) { return sbz_dpp<0x150 + J>(v); }
+};
+// Within each half of 32 lanes (two modes per wavefront): lane J and lane 32 + J read out, each
+// half keeping its own.
+struct SbzBcHalf32 {
+    template <int J>
+    static SBZ_DEV double at(double v) {
+        const double lo = sbz_readlane(v, J), hi = sbz_readlane(v, 32 + J);
+        return (threadIdx.x & 32) ? hi : lo;
+    }
 };
 
 // @@SBZ_MODEL@@
@@ -904,17 +914,19 @@ SBZ_DEV void sbz_wave_body(const SbzSolveArgs& a) {
     }
 }
 
-#if SBZ_WAVE4_OK
+#if SBZ_WAVE4_OK || SBZ_WAVE2H_OK
 // ============================================================================================
-// Four modes per wavefront (SBZ_ALGO_WAVE4): the 16 lanes of row g = lane / 16 integrate one mode,
-// for models whose border and chains fit 16 lanes.  Per mode the work is sbz_wave_body's, with
-// the wavefront broadcasts and reductions in their 16-lane row forms (DPP row_newbcast and a
-// row butterfly).  The four rows share one uniform control flow: every iteration each row takes
-// one step attempt; accept/reject, step size and time are per row (uniform within the row); a
-// row whose mode ends writes it and pulls the next instance.  Rows without a mode (the queue is
+// Several modes per wavefront: the LN lanes of row g = lane / LN integrate one mode, for models
+// whose border and chains fit LN lanes.  LN = 16 (SBZ_ALGO_WAVE4, four modes per wavefront) or
+// LN = 32 (SBZ_ALGO_WAVE2H, two modes per wavefront).  Per mode the work is sbz_wave_body's, with
+// the wavefront broadcasts and reductions in their row forms (16 lanes: DPP row_newbcast and a
+// row butterfly; 32 lanes: readlane per half and the row butterfly plus a swap of the half's two
+// 16-lane rows).  The rows share one uniform control flow: every iteration each row takes one
+// step attempt; accept/reject, step size and time are per row (uniform within the row); a row
+// whose mode ends writes it and pulls the next instance.  Rows without a mode (the queue is
 // empty) step on harmless values and discard them.  States only: no saves, no sources.
 // ============================================================================================
-struct Wave4Row {
+struct WaveRow {
     double v[14][SBZ_N];           // 0,1 U (ping-pong), 2,3 F (ping-pong), 4 DT, 5 US, 6..13 K1..K8
     double B[6][SBZ_NB];           // background at the six distinct stage times
     double dB[SBZ_NB];             // d/dτ of the background at t_n
@@ -933,19 +945,41 @@ SBZ_DEV double sbz_row_sum(double v) {
     v += sbz_dpp<0x140>(v);   // row_mirror
     return v;
 }
-// p holds in all 16 lanes of row g (all 64 lanes active).
-SBZ_DEV bool sbz_row_all(bool p, int g) {
-    const unsigned long long m = __ballot(p);
-    return ((m >> (16 * g)) & 0xFFFFull) == 0xFFFFull;
-}
+// The row operations of LN-lane rows (all 64 lanes active): the broadcast class of the border
+// solves, the sum over the row (identical in every lane of the row) and whether p holds in every
+// lane of row g.
+template <int LN>
+struct SbzRows;
+template <>
+struct SbzRows<16> {
+    using BC = SbzBcRow16;
+    static SBZ_DEV double sum(double v) { return sbz_row_sum(v); }
+    static SBZ_DEV bool all(bool p, int g) {
+        const unsigned long long m = __ballot(p);
+        return ((m >> (16 * g)) & 0xFFFFull) == 0xFFFFull;
+    }
+};
+template <>
+struct SbzRows<32> {
+    using BC = SbzBcHalf32;
+    static SBZ_DEV double sum(double v) {
+        v = sbz_row_sum(v);
+        return v + __shfl_xor(v, 16, 32);   // the other 16-lane row of the half
+    }
+    static SBZ_DEV bool all(bool p, int g) {
+        const unsigned long long m = __ballot(p);
+        return ((m >> (32 * g)) & 0xFFFFFFFFull) == 0xFFFFFFFFull;
+    }
+};
 
-template <bool FUSED>
-SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
-    constexpr int LN = 16;
+template <bool FUSED, int LN>
+SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
+    using RO = SbzRows<LN>;
+    using BC = typename RO::BC;
     constexpr int OWN = (SBZ_N + LN - 1) / LN;
-    __shared__ Wave4Row R4[4];
-    const int g = threadIdx.x >> 4;
-    Wave4Row& L = R4[g];
+    __shared__ WaveRow R4[64 / LN];
+    const int g = threadIdx.x / LN;
+    WaveRow& L = R4[g];
     WaveCtx c;
     c.lane = threadIdx.x & (LN - 1);
     c.isB = c.lane < SBZ_NBORD;
@@ -1054,7 +1088,7 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                         const double v = L.v[0][i];
                         if (!(v == v) || fabs(v) > 1e300) fin = false;
                     }
-                    const bool fin_row = sbz_row_all(fin, g);
+                    const bool fin_row = RO::all(fin, g);
                     if (fresh && !fin_row) ret = SBZ_RET_INITIALFAILURE;
                 }
                 if (fresh)
@@ -1073,8 +1107,8 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                         d0 += (L.v[0][i] / sk) * (L.v[0][i] / sk);
                         d1 += (L.v[2][i] / sk) * (L.v[2][i] / sk);
                     }
-                d0 = sqrt(sbz_row_sum(d0) / SBZ_N);
-                d1 = sqrt(sbz_row_sum(d1) / SBZ_N);
+                d0 = sqrt(RO::sum(d0) / SBZ_N);
+                d1 = sqrt(RO::sum(d1) / SBZ_N);
                 double dt0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
                 dt0 = fmin(dt0, t1 - t);
                 if (init) {
@@ -1097,7 +1131,7 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                         const double r = (L.v[3][i] - L.v[2][i]) / sk;
                         d2 += r * r;
                     }
-                d2 = sqrt(sbz_row_sum(d2) / SBZ_N) / dt0;
+                d2 = sqrt(RO::sum(d2) / SBZ_N) / dt0;
                 if (init) {
                     const double mx = fmax(d1, d2);
                     const double dt1 = (mx <= 1e-15) ? fmax(1e-6, dt0 * 1e-3) : pow(0.01 / mx, 1.0 / 6.0);
@@ -1155,14 +1189,14 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
         const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;
         double al, be;
         sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M, row, cpl, al, be, wb);
-        bool ok = sbz_wb_lu<SbzBcRow16>(c.lane, row, rpo);
-        sbz_wb_unpack<SbzBcRow16>(c.lane, row, rpo, lrow, urow, rpb);
-        sbz_wb_tri2_r<SbzBcRow16>(lrow, urow, rpb, rpo, al, be);
+        bool ok = sbz_wb_lu<BC>(c.lane, row, rpo);
+        sbz_wb_unpack<BC>(c.lane, row, rpo, lrow, urow, rpb);
+        sbz_wb_tri2_r<BC>(lrow, urow, rpb, rpo, al, be);
         wb.z1 = c.isB ? al : 0.0;
         wb.z2 = c.isB ? be : 0.0;
         {
-            const double c11 = 1.0 - sbz_row_sum(wb.p * wb.z1), c12 = -sbz_row_sum(wb.p * wb.z2);
-            const double c21 = -sbz_row_sum(wb.q * wb.z1), c22 = 1.0 - sbz_row_sum(wb.q * wb.z2);
+            const double c11 = 1.0 - RO::sum(wb.p * wb.z1), c12 = -RO::sum(wb.p * wb.z2);
+            const double c21 = -RO::sum(wb.q * wb.z1), c22 = 1.0 - RO::sum(wb.q * wb.z2);
             const double det = c11 * c22 - c12 * c21;
             ok = ok && (det != 0.0) && (det == det) && (fabs(det) < 1e300);
             const double idet = 1.0 / det;
@@ -1171,7 +1205,7 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
             wb.c21 = -c21 * idet;
             wb.c22 = c11 * idet;
         }
-        ok = sbz_row_all(ok || !c.isB, g);
+        ok = RO::all(ok || !c.isB, g);
         sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
         if (live) st_j++;
         __syncthreads();
@@ -1185,8 +1219,8 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                 double* in = (s <= 8) ? US : U1;
                 double ck[OWN];
                 if (s <= 8) {
-                    if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_in_f, 16, c.lane, L.v, Un, US, idt, ck)
-                    else SBZ_STAGE_SWITCH_LN(sbz_stage_in, 16, c.lane, L.v, Un, US)
+                    if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_in_f, LN, c.lane, L.v, Un, US, idt, ck)
+                    else SBZ_STAGE_SWITCH_LN(sbz_stage_in, LN, c.lane, L.v, Un, US)
                 }
                 __syncthreads();
                 sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
@@ -1195,8 +1229,8 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                 if (live && (s <= 8 || EEst <= 1.0)) st_f++;
                 __syncthreads();
                 if (s == 9) break;
-                if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_rhs_f, 16, c.lane, h, DT, Ks, ck)
-                else SBZ_STAGE_SWITCH_LN(sbz_stage_rhs, 16, c.lane, L.v, h, idt, DT, Ks)
+                if constexpr (FUSED) SBZ_STAGE_SWITCH_LN(sbz_stage_rhs_f, LN, c.lane, h, DT, Ks, ck)
+                else SBZ_STAGE_SWITCH_LN(sbz_stage_rhs, LN, c.lane, L.v, h, idt, DT, Ks)
             }
             __syncthreads();
             // solve W x = Ks: chain forward sweeps, border solve, chain back-substitution
@@ -1212,8 +1246,8 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
             {
                 double b = c.isB ? Ks[c.bcomp] : 0.0;
                 if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                sbz_wb_tri_r<SbzBcRow16>(lrow, urow, rpb, rpo, b);   // y = A^-1 b
-                const double s1 = sbz_row_sum(wb.p * b), s2 = sbz_row_sum(wb.q * b);
+                sbz_wb_tri_r<BC>(lrow, urow, rpb, rpo, b);   // y = A^-1 b
+                const double s1 = RO::sum(wb.p * b), s2 = RO::sum(wb.q * b);
                 const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                 if (c.isB) {
                     Ks[c.bcomp] = x;
@@ -1241,8 +1275,8 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
                     ss += r * r;
                     if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
                 }
-                ss = sbz_row_sum(ss);
-                const bool fin_row = sbz_row_all(fin, g);   // a ballot: evaluated by every lane
+                ss = RO::sum(ss);
+                const bool fin_row = RO::all(fin, g);   // a ballot: evaluated by every lane
                 EEst = (ok && fin_row) ? sqrt(ss / SBZ_N) : __builtin_inf();
                 if (!__any(live && EEst <= 1.0)) break;   // no row accepts: no f(t + h) needed
             }
@@ -1251,7 +1285,7 @@ SBZ_DEV void sbz_wave4_body(const SbzSolveArgs& a) {
         // the controller's two powers EEst^β1 and qold^β2, on lanes 0 and 1 of every row
         const double pw = pow((c.lane == 1) ? qold : ((EEst < __builtin_inf()) ? EEst : 1.0),
                               (c.lane == 1) ? SBZ_CTL_BETA2 : SBZ_CTL_BETA1);
-        const double q11 = sbz_dpp<0x150>(pw), qob = sbz_dpp<0x151>(pw);
+        const double q11 = BC::template at<0>(pw), qob = BC::template at<1>(pw);
         if (live) {
             if (!(EEst <= 1.0)) {
                 const double q11r = (EEst < __builtin_inf()) ? q11 : 1.0 / SBZ_CTL_QMIN;
@@ -1685,7 +1719,11 @@ extern "C" __global__ void __launch_bounds__(64) sbz_wave_tan_kernel(SbzSolveArg
 
 #if SBZ_WAVE4_OK
 // Four modes per wavefront, one resident wavefront per SIMD (all 512 registers per lane).
-extern "C" __global__ void __launch_bounds__(64) sbz_wave4_kernel(SbzSolveArgs a) { sbz_wave4_body<true>(a); }
+extern "C" __global__ void __launch_bounds__(64) sbz_wave4_kernel(SbzSolveArgs a) { sbz_waverow_body<true, 16>(a); }
+#endif
+#if SBZ_WAVE2H_OK
+// Two modes per wavefront, for models whose border or chains exceed 16 lanes but fit 32.
+extern "C" __global__ void __launch_bounds__(64) sbz_wave2h_kernel(SbzSolveArgs a) { sbz_waverow_body<true, 32>(a); }
 #endif
 
 // Source background table for the wave kernels: one thread per (set, save time), rows as

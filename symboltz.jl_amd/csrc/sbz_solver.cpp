@@ -126,6 +126,7 @@ extern "C" int sbz_model_create(const char* hip_src, const sbz_model_desc* desc,
         return SBZ_ERR_COMPILE;
     }
     if (hipModuleGetFunction(&m->k_wave4, m->module, "sbz_wave4_kernel") != hipSuccess) m->k_wave4 = nullptr;
+    if (hipModuleGetFunction(&m->k_wave2h, m->module, "sbz_wave2h_kernel") != hipSuccess) m->k_wave2h = nullptr;
     if (hipModuleGetFunction(&m->k_wave_alt, m->module, "sbz_wave_kernel_altlu") != hipSuccess) m->k_wave_alt = nullptr;
     if (hipModuleGetFunction(&m->k_tan, m->module, "sbz_wave_tan_kernel") != hipSuccess ||
         hipModuleGetFunction(&m->k_pk_tan, m->module, "sbz_pk_tan_kernel") != hipSuccess)
@@ -198,6 +199,7 @@ hipError_t sbz_launch_solve(sbz_model* m, int algo, SbzSolveArgsHost* args, unsi
     void* kargs[] = {args};
     hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)             ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
+                       : (algo == SBZ_ALGO_WAVE2H)          ? m->k_wave2h
                        : (algo == SBZ_ALGO_WAVE_ALTLU)      ? m->k_wave_alt
                        : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
                                                             : m->k_wave;
@@ -208,12 +210,13 @@ int sbz_resident_waves(sbz_model* m, int algo) {
     int per_cu = 0, ncu = 0;
     hipFunction_t kw = (algo == SBZ_ALGO_WAVE2)             ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4)           ? m->k_wave4
+                       : (algo == SBZ_ALGO_WAVE2H)          ? m->k_wave2h
                        : (algo == SBZ_ALGO_WAVE_ALTLU)      ? m->k_wave_alt
                        : (algo == SBZ_ALGO_TANGENT_INTERNAL) ? m->k_tan
                                                             : m->k_wave;
     if (!kw) return 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kw, 64, 0) != hipSuccess || per_cu < 1)
-        per_cu = (algo == SBZ_ALGO_WAVE2) ? 8 : (algo == SBZ_ALGO_WAVE4) ? 2 : 4;
+        per_cu = (algo == SBZ_ALGO_WAVE2) ? 8 : (algo == SBZ_ALGO_WAVE4 || algo == SBZ_ALGO_WAVE2H) ? 2 : 4;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess || ncu < 1)
         ncu = 256;
     return per_cu * ncu;
@@ -222,12 +225,14 @@ int sbz_resident_waves(sbz_model* m, int algo) {
 extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     sbz_clear_error();
     if (!m || !capacity ||
-        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_WAVE4 && algo != SBZ_ALGO_WAVE_ALTLU)) {
+        (algo != SBZ_ALGO_WAVE && algo != SBZ_ALGO_WAVE2 && algo != SBZ_ALGO_WAVE4 && algo != SBZ_ALGO_WAVE_ALTLU &&
+         algo != SBZ_ALGO_WAVE2H)) {
         sbz_set_error("sbz_model_capacity: invalid argument");
         return SBZ_ERR_ARG;
     }
     hipFunction_t kf = (algo == SBZ_ALGO_WAVE2)        ? m->k_wave2
                        : (algo == SBZ_ALGO_WAVE4)      ? m->k_wave4
+                       : (algo == SBZ_ALGO_WAVE2H)     ? m->k_wave2h
                        : (algo == SBZ_ALGO_WAVE_ALTLU) ? m->k_wave_alt
                                                        : m->k_wave;
     if (!kf) {
@@ -237,7 +242,7 @@ extern "C" int sbz_model_capacity(sbz_model* m, int algo, int* capacity) {
     int per_cu = 0, ncu = 0;
     HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, 64, 0));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, m->device));
-    *capacity = per_cu * ncu * (algo == SBZ_ALGO_WAVE4 ? 4 : 1);
+    *capacity = per_cu * ncu * (algo == SBZ_ALGO_WAVE4 ? 4 : algo == SBZ_ALGO_WAVE2H ? 2 : 1);
     return SBZ_OK;
 }
 
@@ -283,7 +288,7 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
     if (n_save > 0 && !(p->save_flags & (SBZ_SAVE_STATES | SBZ_SAVE_SOURCES))) p->save_flags |= SBZ_SAVE_STATES;
     p->saveat_stride = per_set ? n_save : 0;
     if (opts->algo != 0 && opts->algo != SBZ_ALGO_WAVE && opts->algo != SBZ_ALGO_WAVE2 && opts->algo != SBZ_ALGO_WAVE4 &&
-        opts->algo != SBZ_ALGO_WAVE_ALTLU) {
+        opts->algo != SBZ_ALGO_WAVE_ALTLU && opts->algo != SBZ_ALGO_WAVE2H) {
         sbz_set_error("sbz_plan_create: unknown algo %d", opts->algo);
         delete p;
         return SBZ_ERR_ARG;
@@ -299,13 +304,21 @@ extern "C" int sbz_plan_create(sbz_model* m, const int* inst_set, const double* 
         delete p;
         return SBZ_ERR_ARG;
     }
+    if (opts->algo == SBZ_ALGO_WAVE2H && (!m->k_wave2h || p->save_flags != 0)) {
+        sbz_set_error(!m->k_wave2h
+                          ? "sbz_plan_create: SBZ_ALGO_WAVE2H needs a model without the four-mode kernel whose border "
+                            "and chains fit 32 lanes"
+                          : "sbz_plan_create: SBZ_ALGO_WAVE2H saves no states or sources (final states only)");
+        delete p;
+        return SBZ_ERR_ARG;
+    }
     p->algo = opts->algo ? opts->algo : SBZ_ALGO_WAVE;
     // `lanes`: concurrently resident wavefronts (the persistent grid)
     int lanes = opts->lanes > 0 ? opts->lanes : n_inst;
     if (opts->lanes <= 0) {
         // persistent grid: as many wavefronts as can be resident at once, each pulling instances
-        // (four at a time for SBZ_ALGO_WAVE4)
-        const int need = (p->algo == SBZ_ALGO_WAVE4) ? (n_inst + 3) / 4 : n_inst;
+        // (four at a time for SBZ_ALGO_WAVE4, two for SBZ_ALGO_WAVE2H)
+        const int need = (p->algo == SBZ_ALGO_WAVE4) ? (n_inst + 3) / 4 : (p->algo == SBZ_ALGO_WAVE2H) ? (n_inst + 1) / 2 : n_inst;
         lanes = std::min(need, sbz_resident_waves(m, p->algo));
     }
     lanes = std::max(1, std::min(lanes, std::max(n_inst, 1)));

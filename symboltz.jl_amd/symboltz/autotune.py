@@ -10,7 +10,9 @@ For a plan it picks among the wavefront kernels:
   latency once there are more instances than SIMDs;
 - SBZ_ALGO_WAVE4: four modes per wavefront (16 lanes each), for models whose border and chains
   fit 16 lanes and plans that save nothing: the per-step work that the lanes of a mode share
-  (border RHS and Jacobian assembly, background) serves four modes.
+  (border RHS and Jacobian assembly, background) serves four modes;
+- SBZ_ALGO_WAVE2H: two modes per wavefront (32 lanes each), the same for the models whose border
+  or chains exceed 16 lanes but fit 32.
 
 When the instances fit the one-per-SIMD capacity the launch is the latency of its longest mode:
 the two factor layouts are timed on the 64 longest instances (when the other layout holds them
@@ -27,8 +29,9 @@ import numpy as np
 
 from . import _lib
 
-ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_WAVE_ALTLU = 1, 3, 4, 5
-NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4", ALGO_WAVE_ALTLU: "WAVE_ALTLU"}
+ALGO_WAVE, ALGO_WAVE2, ALGO_WAVE4, ALGO_WAVE_ALTLU, ALGO_WAVE2H = 1, 3, 4, 5, 6
+NAMES = {ALGO_WAVE: "WAVE", ALGO_WAVE2: "WAVE2", ALGO_WAVE4: "WAVE4", ALGO_WAVE_ALTLU: "WAVE_ALTLU",
+         ALGO_WAVE2H: "WAVE2H"}
 _cache = {}
 last_times = {}
 
@@ -54,9 +57,10 @@ def choose(gm, inst_set, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=N
     if c_alt > 0:
         caps[ALGO_WAVE_ALTLU] = c_alt
     if n_save == 0:
-        c4 = capacity(gm, ALGO_WAVE4)
-        if c4 > 0:
-            caps[ALGO_WAVE4] = c4
+        for al in (ALGO_WAVE4, ALGO_WAVE2H):
+            c = capacity(gm, al)
+            if c > 0:
+                caps[al] = c
     order = np.argsort(-np.asarray(ks), kind="stable")
     if n <= caps[ALGO_WAVE] or max(caps.values()) <= caps[ALGO_WAVE]:
         # latency-bound: the longest modes set the launch; only the factor layout can matter

@@ -835,6 +835,12 @@ class Emitted:
         items = [(lhs, with_reciprocals(e, invm)) for lhs, e in items]
         lines, _ = _emit_block(items, "tw", interleave=True)
         out.append(f"#define SBZ_WSTRIDE {ws}")
+        # two modes per wavefront (32 lanes each) for the models that have no four-mode kernel but
+        # whose border and chains fit 32 lanes, when two rows' LDS fits 64 KB
+        row_bytes = 8 * (14 * n + 7 * NB + len(C_NAMES) + nb * ws + nb + len(chains) + 1)
+        w4 = nb <= 16 and len(chains) <= 16
+        w2h = (not w4) and nb <= 32 and len(chains) <= 32 and 2 * row_bytes <= 65536
+        out.append(f"#define SBZ_WAVE2H_OK {1 if w2h else 0}")
         out.append("struct SbzWb {   // per-lane low-rank data of the border solve")
         out.append("    double p, q, z1, z2;   // p_r, q_r; (A^-1 α)_r, (A^-1 β)_r")
         out.append("    double c11, c12, c21, c22;   // (I - [p q]^T A^-1 [α β])^-1, uniform")

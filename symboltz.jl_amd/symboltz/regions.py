@@ -42,9 +42,9 @@ WAVE_REGIONS = ["saves", "step checks", "background", "chain factor", "W assembl
 
 
 def wave4_source(src):
-    """The model source with sbz_wave4_body instrumented; totals per wavefront in row 0's last
-    instance."""
-    i0 = src.index("SBZ_DEV void sbz_wave4_body")
+    """The model source with sbz_waverow_body (the four-mode kernel's body) instrumented; totals per
+    wavefront in row 0's last instance."""
+    i0 = src.index("SBZ_DEV void sbz_waverow_body")
     i1 = src.index("#endif", i0)
     T = _tick
     body = _patch(src[i0:i1], [
@@ -56,8 +56,8 @@ def wave4_source(src):
         ("        __syncthreads();\n        const double* B0 = L.B[0];", "        __syncthreads();\n        " + T(2) + "\n        const double* B0 = L.B[0];"),
         ("        const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;\n",
          "        " + T(3) + "\n        const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;\n"),
-        ("        bool ok = sbz_wb_lu<SbzBcRow16>(c.lane, row, rpo);\n",
-         "        " + T(4) + "\n        bool ok = sbz_wb_lu<SbzBcRow16>(c.lane, row, rpo);\n        " + T(5) + "\n"),
+        ("        bool ok = sbz_wb_lu<BC>(c.lane, row, rpo);\n",
+         "        " + T(4) + "\n        bool ok = sbz_wb_lu<BC>(c.lane, row, rpo);\n        " + T(5) + "\n"),
         ("        sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)\n",
          "        " + T(6) + "\n        sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)\n"),
         ("        if (live) st_j++;\n        __syncthreads();\n", "        if (live) st_j++;\n        __syncthreads();\n        " + T(7) + "\n"),

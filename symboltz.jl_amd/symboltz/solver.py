@@ -66,6 +66,7 @@ ALGO_WAVE = 1
 ALGO_WAVE2 = 3
 ALGO_WAVE4 = 4   # four modes per wavefront (models whose border and chains fit 16 lanes; no saves)
 ALGO_WAVE_ALTLU = 5   # ALGO_WAVE with the other layout of the border factors in LDS
+ALGO_WAVE2H = 6   # two modes per wavefront (models without ALGO_WAVE4 whose border and chains fit 32 lanes; no saves)
 ALGO_AUTO = -1   # host-side: symboltz.autotune picks among the wavefront variants
 
 

@@ -78,6 +78,10 @@ def solve_spectra(X, ks_mpc=None, M=None, chunk=1024, algo=None, log=None):
             tt.append(np.clip(np.minimum(1e-2 / k, 1e-4), bg.t[0], bg.t[-1]))
         plan = gm.plan(np.concatenate(iset), np.concatenate(kk), np.concatenate(tt),
                        algo=ALGO_AUTO if algo is None else algo)
+        if log and c0 == 0:
+            from . import autotune
+            log(f"kernel variant {autotune.NAMES.get(plan.algo, plan.algo)}"
+                + (f" (sample ms: {autotune.last_times})" if algo is None and autotune.last_times else ""))
         kern += plan.run() * 1e-3
         r = plan.fetch()
         if not np.all(r["retcode"] == 0):

@@ -49,12 +49,13 @@ class WaveEmu:
         self.n = self.L.emu_n()
 
     def solve(self, P, t, u, du, ks, t0, rtol=1e-5, atol=1e-5, maxiters=100000, saveat=None, sources=False,
-              srcbg=False, o2=False, wave4=False, altlu=False):
+              srcbg=False, o2=False, wave4=False, altlu=False, wave2h=False):
         """srcbg: give the kernel the per-set source background table (as the runtime does for the wave
         kernels) instead of letting it evaluate the background at every save of every mode.
         o2: run sbz_wave_kernel_o2 (the two-wavefronts-per-SIMD build) instead of sbz_wave_kernel.
         wave4: run sbz_wave4_kernel (four modes per wavefront, one emulated wavefront for all).
-        altlu: run sbz_wave_kernel_altlu (the other LDS layout of the border factors)."""
+        altlu: run sbz_wave_kernel_altlu (the other LDS layout of the border factors).
+        wave2h: run sbz_wave2h_kernel (two modes per wavefront, one emulated wavefront for all)."""
         p = lambda a, ty=ctypes.c_double: a.ctypes.data_as(ctypes.POINTER(ty))
         nk = len(t)
         knots = np.concatenate([np.asarray(t, float), np.asarray(u, float).ravel(), np.asarray(du, float).ravel()])
@@ -73,7 +74,7 @@ class WaveEmu:
         ret = np.zeros(m, dtype=np.int32)
         st = np.zeros((m, 6), dtype=np.int64)
         self.L.emu_set_srcbg(1 if srcbg else 0)
-        self.L.emu_set_o2(3 if altlu else 2 if wave4 else (1 if o2 else 0))
+        self.L.emu_set_o2(4 if wave2h else 3 if altlu else 2 if wave4 else (1 if o2 else 0))
         rc = self.L.emu_wave_solve(p(P), p(knots), p(koff, ctypes.c_longlong), p(kn, ctypes.c_int),
                                    p(iset, ctypes.c_int), p(ks), p(t0), m, rtol, atol, maxiters,
                                    p(saveat) if ns else None, ns, p(out_u), p(out_save) if ns else None,

@@ -174,14 +174,22 @@ static inline double __longlong_as_double(long long x) {
 
 // 0: sbz_wave_kernel (fused stage combinations), 1: sbz_wave_kernel_o2 (two-pass combinations),
 // 2: sbz_wave4_kernel (four modes per wavefront; models whose border and chains fit 16 lanes),
-// 3: sbz_wave_kernel_altlu (the other LDS layout of the border factors)
+// 3: sbz_wave_kernel_altlu (the other LDS layout of the border factors),
+// 4: sbz_wave2h_kernel (two modes per wavefront; models whose border and chains fit 32 lanes only)
 static int g_o2 = 0;
 extern "C" void emu_set_o2(int on) { g_o2 = on; }
 extern "C" int emu_has_wave4(void) { return SBZ_WAVE4_OK; }
+extern "C" int emu_has_wave2h(void) { return SBZ_WAVE2H_OK; }
 static void wave_body(void* a) {
 #if SBZ_WAVE4_OK
     if (g_o2 == 2) {
         sbz_wave4_kernel(*(SbzSolveArgs*)a);
+        return;
+    }
+#endif
+#if SBZ_WAVE2H_OK
+    if (g_o2 == 4) {
+        sbz_wave2h_kernel(*(SbzSolveArgs*)a);
         return;
     }
 #endif

@@ -112,6 +112,42 @@ def test_lhs_sets_wave4_forced_match_oracle():
         gm5.plan(np.zeros(2, np.int32), np.array([1.0, 10.0]), np.full(2, 1e-4), algo=ALGO_WAVE4)
 
 
+def test_wave2h_forced_matches_oracle_and_one_mode_kernel():
+    # two modes per wavefront (32-lane rows) on the models too wide for four: the surrogate's
+    # three-massive-ν model (n = 54, 4 Latin-hypercube sets) and the n = 100 model, k to 10/Mpc
+    from symboltz import surrogate as sg
+    from symboltz.solver import ALGO_WAVE2H, ALGO_WAVE4
+    from symboltz.autotune import capacity
+    M100, p100 = workloads.massive_nu100()
+    cases = [(sg.surrogate_model(), [sg.pars_of(sg.surrogate_model(), x) for x in sg.lhs(4, seed=5)]),
+             (M100, [p100])]
+    for M, sets in cases:
+        say(f"case: n = {M.n_unknowns}, {len(sets)} set(s) x 48 k in [1e-5, 10]/Mpc, WAVE2H forced and WAVE")
+        setups = setups_of(M, sets)
+        gm = GpuModel(M)
+        assert capacity(gm, ALGO_WAVE4) == 0 and capacity(gm, ALGO_WAVE2H) > 0
+        out = {}
+        for algo in (ALGO_WAVE2H, ALGO_WAVE):
+            plan, iset, kk, tt = plan_for(gm, setups, np.logspace(-5, 1, 48), algo=algo)
+            ms = plan.run()
+            out[algo] = (plan.fetch(), plan.pk())
+            plan.close()
+            say(f"algo {algo}: {len(kk)} instances in {ms:.1f} ms")
+        r2, pk2 = out[ALGO_WAVE2H]
+        r1, _ = out[ALGO_WAVE]
+        assert np.all(r2["retcode"] == 0) and np.all(r1["retcode"] == 0)
+        # the same arithmetic but for the summation order of the error norms, so a step on the
+        # acceptance border may go the other way: agreement far below the 1e-4 parity bar
+        err = rel_state_err(r2["u"], r1["u"])
+        say(f"WAVE2H against WAVE: states {err:.2e}")
+        assert err < 1e-6
+        assert np.all(np.abs(r2["stats"][:, 0] - r1["stats"][:, 0]) <= 2)
+        oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), 4), r2["u"], pk2)
+    # final states only, like the four-mode kernel
+    with pytest.raises(_lib.SbzError, match="WAVE2H"):
+        gm.plan(iset[:4], kk[:4], tt[:4], algo=ALGO_WAVE2H, saveat=np.array([1.0]))
+
+
 def test_lhs_sets_above_capacity_both_variants_match_oracle():
     # twice the one-per-SIMD capacity: both occupancy builds, forced, against the oracle
     from symboltz.autotune import capacity

@@ -133,6 +133,30 @@ def test_wave4_kernel_emulated_matches_oracle_and_wave(massless10):
     assert np.all(np.abs(g4["stats"][good, 0] - g1["stats"][good, 0]) <= 2)
 
 
+def test_wave2h_kernel_emulated_matches_oracle_and_wave():
+    """sbz_wave2h_kernel (two modes per wavefront, 32 lanes each: readlane broadcasts per half,
+    row reductions across the half's two 16-lane rows, per-row step control) on the three-massive-ν
+    lmax-6 model (border 26, chains 7: too wide for four modes per wavefront): 5 instances through
+    the 2 rows, k = 0 and k = NaN fail with InitialFailure; states against the oracle (1e-4) and the
+    one-mode-per-wavefront kernel (1e-8: the same arithmetic but for the summation order)."""
+    M = models.LCDM(lmax=6)
+    S = Setup(M, models.parameters_Planck18(M))
+    E = WaveEmu(model_source(M).full_src)
+    assert E.L.emu_has_wave4() == 0 and E.L.emu_has_wave2h() == 1
+    ks = np.array([20.0, 0.0, 0.3, np.nan, 3.0])
+    kk = np.where(ks > 0, ks, 1.0)
+    t0 = np.clip(np.minimum(1e-2 / kk, 1e-4), S.t[0], S.t[-1])
+    g2 = E.solve(S.P, S.t, S.u, S.du, ks, t0, wave2h=True)
+    g1 = E.solve(S.P, S.t, S.u, S.du, ks, t0)
+    good = np.isfinite(ks) & (ks > 0)
+    assert list(g2["retcode"][~good]) == [4, 4]
+    assert np.all(g2["retcode"][good] == 0)
+    o = S.O.solve(ks[good], t0[good])
+    assert rel(g2["u"][good], o["u"]) < 1e-4
+    assert rel(g2["u"][good], g1["u"][good]) < 1e-8
+    assert np.all(np.abs(g2["stats"][good, 0] - g1["stats"][good, 0]) <= 2)
+
+
 def test_tangent_kernel_emulated_matches_oracle_differences(fixture5):
     """sbz_wave_tan_kernel (u and s = ∂u/∂log θ on one step sequence, the background's θ-dependence
     from two tight background solves at log θ ± h): u matches the plain kernel, and the total
