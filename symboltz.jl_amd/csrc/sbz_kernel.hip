@@ -1012,8 +1012,13 @@ SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
     SbzWb wb;
     double cinv[SBZ_MCHAIN];
     double csk = 0.0;
-    double lrow[SBZ_NBORD], urow[SBZ_NBORD], rpb[SBZ_NBORD];   // this lane's factored border row
-    double rpo = 0.0;                                           // its reciprocal pivot
+    // the backward border solves scale by the reciprocal pivots after their broadcast, from
+    // registers (16-lane rows), or before it, in the pivot's own lane (32-lane rows, which are
+    // short of registers)
+    constexpr bool OWNSCALE = (LN == 32);
+    double lrow[SBZ_NBORD], urow[SBZ_NBORD];   // this lane's factored border row
+    double rpb[OWNSCALE ? 1 : SBZ_NBORD];      // the broadcast reciprocal pivots
+    double rpo = 0.0;                           // its reciprocal pivot
     for (int i = c.lane; i < SBZ_N; i += LN)
 #pragma unroll
         for (int j = 0; j < 14; ++j) L.v[j][i] = 0.0;
@@ -1190,8 +1195,9 @@ SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
         double al, be;
         sbz_wb_w(c.lane, B0, L.Cs, k, ig, xc, L.M, row, cpl, al, be, wb);
         bool ok = sbz_wb_lu<BC>(c.lane, row, rpo);
-        sbz_wb_unpack<BC>(c.lane, row, rpo, lrow, urow, rpb);
-        sbz_wb_tri2_r<BC>(lrow, urow, rpb, rpo, al, be);
+        sbz_wb_unpack(c.lane, row, lrow, urow);
+        if constexpr (!OWNSCALE) sbz_wb_rpb<BC>(rpo, rpb);
+        sbz_wb_tri2_r<BC, OWNSCALE>(lrow, urow, rpb, rpo, al, be);
         wb.z1 = c.isB ? al : 0.0;
         wb.z2 = c.isB ? be : 0.0;
         {
@@ -1246,7 +1252,7 @@ SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
             {
                 double b = c.isB ? Ks[c.bcomp] : 0.0;
                 if (c.isB && c.bchain >= 0) b -= cpl * L.C[c.bchain];
-                sbz_wb_tri_r<BC>(lrow, urow, rpb, rpo, b);   // y = A^-1 b
+                sbz_wb_tri_r<BC, OWNSCALE>(lrow, urow, rpb, rpo, b);   // y = A^-1 b
                 const double s1 = RO::sum(wb.p * b), s2 = RO::sum(wb.q * b);
                 const double x = b + wb.z1 * (wb.c11 * s1 + wb.c12 * s2) + wb.z2 * (wb.c21 * s1 + wb.c22 * s2);
                 if (c.isB) {

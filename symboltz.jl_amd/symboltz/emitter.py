@@ -1010,21 +1010,23 @@ class Emitted:
 
         # The same solves with the factors kept in registers (four-modes-per-wavefront kernel, where
         # LDS bounds the resident wavefronts): each lane unpacks its factored row once per step into
-        # its strictly-lower and strictly-upper parts, plus the broadcast reciprocal pivots
+        # its strictly-lower and strictly-upper parts (its reciprocal pivot stays in rpo); the
+        # reciprocal pivots of the backward pivots, broadcast once per step, for the solves that
+        # scale after the broadcast
         bwd_piv = [j for j in range(nb) if ucol[j]]
-        up = ["template <class BC>",
-              "SBZ_DEV void sbz_wb_unpack(int lane, const double* row, double rpo, double* lrow, double* urow, double* rpb) {"]
+        up = ["SBZ_DEV void sbz_wb_unpack(int lane, const double* row, double* lrow, double* urow) {"]
         up.append("    const bool own = lane < SBZ_NBORD;   // lanes past the border carry zeros")
         for j in range(nb):
             up.append(f"    lrow[{j}] = (own && {j} < lane) ? row[{j}] : 0.0; urow[{j}] = (own && {j} > lane) ? row[{j}] : 0.0;")
-        for j in bwd_piv:
-            up.append(f"    rpb[{j}] = BC::template at<{j}>(rpo);")
+        up.append("}")
+        up += ["template <class BC>", "SBZ_DEV void sbz_wb_rpb(double rpo, double* rpb) {"]
+        up += [f"    rpb[{j}] = BC::template at<{j}>(rpo);" for j in bwd_piv]
         up.append("}")
         out += up
 
         def tri_r(name, nr):
             args = ", ".join(f"double& b{q}" for q in range(nr))
-            so = ["template <class BC>",
+            so = ["template <class BC, bool OWNSCALE>",
                   f"SBZ_DEV void {name}(const double* lrow, const double* urow, const double* rpb, double rpo, {args}) {{"]
             for L_ in range(nlev):
                 piv = [j for j in range(nb) if lev[j] == L_ and lower[j]]
@@ -1041,10 +1043,22 @@ class Emitted:
                 piv = [j for j in range(nb) if levb[j] == L_ and ucol[j]]
                 if not piv:
                     continue
+                # x_j = b_j / p_j: OWNSCALE scales every lane's own entry by its reciprocal pivot
+                # and broadcasts lane j's product (no registers for the broadcast pivots); else the
+                # broadcast b_j is scaled by rpb[j].  Both form the same product.
                 so.append("    {")
+                so.append("        double " + ", ".join(f"x{q}_{j}" for q in range(nr) for j in piv) + ";")
+                so.append("        if constexpr (OWNSCALE) {")
+                for q in range(nr):
+                    so.append(f"            const double s{q} = b{q} * rpo;")
                 for q in range(nr):
                     for j in piv:
-                        so.append(f"        const double x{q}_{j} = BC::template at<{j}>(b{q}) * rpb[{j}];")
+                        so.append(f"            x{q}_{j} = BC::template at<{j}>(s{q});")
+                so.append("        } else {")
+                for q in range(nr):
+                    for j in piv:
+                        so.append(f"            x{q}_{j} = BC::template at<{j}>(b{q}) * rpb[{j}];")
+                so.append("        }")
                 for q in range(nr):
                     so.append(f"        b{q} -= " + " + ".join(f"urow[{j}] * x{q}_{j}" for j in piv) + ";")
                 so.append("    }")
