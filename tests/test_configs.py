@@ -114,13 +114,16 @@ def test_lhs_sets_wave4_forced_match_oracle():
 
 def test_wave2h_forced_matches_oracle_and_one_mode_kernel():
     # two modes per wavefront (32-lane rows) on the models too wide for four: the surrogate's
-    # three-massive-ν model (n = 54, 4 Latin-hypercube sets) and the n = 100 model, k to 10/Mpc
+    # three-massive-ν model (n = 54, 4 Latin-hypercube sets), the n = 100 model and w0waCDM with
+    # massive ν (n = 56, 3 sets of the mixed stream), k to 10/Mpc
     from symboltz import surrogate as sg
     from symboltz.solver import ALGO_WAVE2H, ALGO_WAVE4
     from symboltz.autotune import capacity
     M100, p100 = workloads.massive_nu100()
+    fams = workloads.mixed_families()
+    de = [p for f, p, _ in workloads.mixed_requests(15, seed=5) if fams[f].de == "w0wa"]
     cases = [(sg.surrogate_model(), [sg.pars_of(sg.surrogate_model(), x) for x in sg.lhs(4, seed=5)]),
-             (M100, [p100])]
+             (M100, [p100]), (fams[2], de)]
     for M, sets in cases:
         say(f"case: n = {M.n_unknowns}, {len(sets)} set(s) x 48 k in [1e-5, 10]/Mpc, WAVE2H forced and WAVE")
         setups = setups_of(M, sets)
@@ -141,7 +144,10 @@ def test_wave2h_forced_matches_oracle_and_one_mode_kernel():
         err = rel_state_err(r2["u"], r1["u"])
         say(f"WAVE2H against WAVE: states {err:.2e}")
         assert err < 1e-6
-        assert np.all(np.abs(r2["stats"][:, 0] - r1["stats"][:, 0]) <= 2)
+        # step counts: a flipped acceptance shifts the rest of the sequence (by up to 10 of
+        # 3,000 to 15,000 steps for the w0wa modes above k = 2,000 H0/c, with hundreds of rejections)
+        acc1, acc2 = r1["stats"][:, 0], r2["stats"][:, 0]
+        assert np.all(np.abs(acc2 - acc1) <= 2 + acc1 // 100)
         oracle_check(setups, iset, kk, tt, np.arange(0, len(kk), 4), r2["u"], pk2)
     # final states only, like the four-mode kernel
     with pytest.raises(_lib.SbzError, match="WAVE2H"):
