@@ -228,10 +228,22 @@ int sbz_multi_create(const char* hip_src, const char* cache_path, sbz_model* con
 /* Run plans[f] of models[f] for every family f together (one plan per model, in order; a plan may
  * hold no instances).  Plans save no states or sources (final states only) and are not tangent
  * plans.  Results land in each plan's buffers: sbz_plan_fetch and sbz_plan_pk read them as after
- * sbz_plan_run.  *kernel_ms receives the launch's duration. */
+ * sbz_plan_run.  *kernel_ms receives the launch's duration.  With sbz_multi_attach_rows, a family
+ * whose body integrates several modes per wavefront (sbz_multi_rows) and whose plan holds at least
+ * twice the resident wavefronts of the attached kernel runs in that body (its wavefronts drain the
+ * family's instances longest first), in a launch of the attached kernel; every other instance runs
+ * in its model's one-mode body, and a run without such a family launches the one-mode kernel. */
 int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans, float* kernel_ms);
-/* Resident wavefronts of the mixed-model kernel (modes that run at once). */
+/* Resident wavefronts of the one-mode mixed-model kernel (modes that run at once). */
 int sbz_multi_capacity(sbz_multi* mm, int* capacity);
+/* Attach the kernel with the several-modes bodies, sbz_multi_kernel_rows, from its own code object
+ * (source and cache as sbz_multi_create; the same models in the same order).  It is a code object
+ * of its own so that the one-mode kernel, which serves the latency-bound launches, keeps the
+ * registers of its bodies alone.  No reference counterpart. */
+int sbz_multi_attach_rows(sbz_multi* mm, const char* hip_src, const char* cache_path);
+/* Modes per wavefront of each family's several-modes body: rows[f] = 4 (16-lane rows), 2 (32-lane
+ * rows) or 1 (none, or no such kernel attached), f < n_models.  No reference counterpart. */
+int sbz_multi_rows(sbz_multi* mm, int* rows);
 int sbz_multi_destroy(sbz_multi* mm);
 
 /* ---- line-of-sight integration ----------------------------------------------------------- */

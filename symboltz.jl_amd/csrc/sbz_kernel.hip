@@ -972,12 +972,13 @@ struct SbzRows<32> {
     }
 };
 
+// The rows of one wavefront integrate instances pulled from a's queue until it is empty; R4 holds
+// the 64 / LN rows' LDS.
 template <bool FUSED, int LN>
-SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
+SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
     using RO = SbzRows<LN>;
     using BC = typename RO::BC;
     constexpr int OWN = (SBZ_N + LN - 1) / LN;
-    __shared__ WaveRow R4[64 / LN];
     const int g = threadIdx.x / LN;
     WaveRow& L = R4[g];
     WaveCtx c;
@@ -1315,6 +1316,12 @@ SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
         }
         __syncthreads();
     }
+}
+
+template <bool FUSED, int LN>
+SBZ_DEV void sbz_waverow_body(const SbzSolveArgs& a) {
+    __shared__ WaveRow R4[64 / LN];
+    sbz_waverow_run<FUSED, LN>(a, R4);
 }
 #endif
 

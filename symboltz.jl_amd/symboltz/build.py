@@ -101,15 +101,15 @@ def build_models(models=None, jobs=4, verbose=False, with_multi=None):
         if not path.exists():
             work.append((em.full_src, path))
     multis = []
-    if with_multi:   # the mixed-model code objects of the standard model lists
-        from .multi import multi_source
-        import hashlib
+    if with_multi:   # the mixed-model code objects of the standard model lists (both kernels)
+        from .multi import multi_cache_path, multi_source
         for fams in standard_multis():
-            src = multi_source(fams)
-            path = KCACHE / f"multi_{hashlib.sha1(src.encode()).hexdigest()[:16]}.hsaco"
-            multis.append(path.name)
-            if not path.exists():
-                work.append((src, path))
+            for rk in (False, True):
+                src = multi_source(fams, rows_kernel=rk)
+                path = multi_cache_path(src)
+                multis.append(path.name)
+                if not path.exists():
+                    work.append((src, path))
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(work) or 1))) as ex:
         for p in ex.map(lambda w: compile_code_object(*w), work):
             if verbose:

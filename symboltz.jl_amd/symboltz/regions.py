@@ -42,10 +42,10 @@ WAVE_REGIONS = ["saves", "step checks", "background", "chain factor", "W assembl
 
 
 def wave4_source(src):
-    """The model source with sbz_waverow_body (the four-mode kernel's body) instrumented; totals per
+    """The model source with sbz_waverow_run (the four-mode kernel's body) instrumented; totals per
     wavefront in row 0's last instance."""
-    i0 = src.index("SBZ_DEV void sbz_waverow_body")
-    i1 = src.index("#endif", i0)
+    i0 = src.index("SBZ_DEV void sbz_waverow_run")
+    i1 = src.index("\ntemplate <bool FUSED, int LN>\nSBZ_DEV void sbz_waverow_body", i0)
     T = _tick
     body = _patch(src[i0:i1], [
         ("    auto finish = [&]() {", f"    long long PR[{NR}] = {{}}; long long PI = 0, tq = 0;\n    auto finish = [&]() {{"),

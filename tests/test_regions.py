@@ -13,5 +13,5 @@ def test_region_sources_build_for_both_kernels():
     out1 = wave_source(src1)
     assert out1.count("clock64()") >= NR - 1
     # the copies differ from the model's own source only inside the instrumented body
-    assert out4.startswith(src4[:src4.index("SBZ_DEV void sbz_waverow_body")])
+    assert out4.startswith(src4[:src4.index("SBZ_DEV void sbz_waverow_run")])
     assert out1.startswith(src1[:src1.index("SBZ_DEV void sbz_wave_mode(")])

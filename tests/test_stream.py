@@ -57,6 +57,41 @@ def test_mixed_families_one_launch_match_per_family_and_oracle():
         st.close()
 
 
+def test_mixed_launch_several_modes_bodies_match_per_family_and_oracle():
+    """Family plans of at least twice the capacity run in their several-modes bodies inside the one
+    launch (four modes per wavefront at n = 38, two at n = 54): against one launch per family with
+    the one-mode kernel (P within 1e-6: the error norms' summation order differs) and the oracle."""
+    from symboltz import surrogate as sg
+    from symboltz.stream import MixedStream
+    fams = workloads.mixed_families()
+    ks = sg.k_grid()
+    st = MixedStream(fams, ks)
+    try:
+        rows, cap = st.multi.rows(), st.multi.capacity()
+        assert rows == [4, 2, 2, 2, 1], rows
+        per = -(-2 * cap // len(ks))   # requests per family: at least twice the capacity in modes
+        reqs = [r for r in workloads.mixed_requests(5 * per, seed=7) if r[0] in (0, 1)]
+        say(f"case: {len(reqs)} requests of the n = 38 and n = 54 families x {len(ks)} k (capacity {cap}), "
+            f"one launch (several-modes bodies) vs one launch per family")
+        lp1, _, ret1, tm1 = st.solve(reqs)
+        lp2, _, ret2, tm2 = st.solve(reqs, mode="serial")
+        say(f"one launch {tm1['solve_ms']:.1f} ms, per family serial {tm2['solve_ms']:.1f} ms")
+        assert np.all(ret1 == 0) and np.all(ret2 == 0)
+        err = float(np.max(np.abs(np.expm1(lp1 - lp2))))
+        say(f"one launch against per family: max |P1/P2 - 1| = {err:.2e}")
+        assert err < 1e-6
+        for f in (0, 1):
+            i = next(j for j, r in enumerate(reqs) if r[0] == f)
+            S = Setup(fams[f], reqs[i][1])
+            P, r = S.O.spectrum_matter(k_dimensionless(ks[::32], reqs[i][1]["h"]))
+            assert np.all(r["retcode"] == 0)
+            e = np.max(np.abs(np.exp(lp1[i, ::32]) / P - 1))
+            say(f"family {f}: max |P/P_oracle - 1| = {e:.2e}")
+            assert e < 1e-4
+    finally:
+        st.close()
+
+
 def test_stream_routes_requests_between_surrogate_and_solver():
     from symboltz import surrogate as sg
     from symboltz.stream import MixedStream
