@@ -368,10 +368,13 @@ def spectrum_matter_modes(modes, prob, ks, taus=None, ktini=1e-2, tauinimax=1e-4
     return P if saveat is not None else P[:, 0, :]
 
 
-def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=1e-4, lanes=0, timing=None):
+def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=1e-4, lanes=0, timing=None,
+               algo=None):
     """Many parameter sets × the same ks in ONE persistent launch (the batched form of
     parameter_updater + solve, src/solve.jl:272-307 driven in a loop at test/runtests.jl:508-532).
-    Returns (P[n_sets][n_k], results, backgrounds)."""
+    algo: a kernel variant (symboltz.solver.ALGO_*); None picks the autotuner when the batch holds
+    at least twice the one-mode kernel's resident modes (throughput-bound: the variants differ by
+    up to 4x there), else the one-mode kernel.  Returns (P[n_sets][n_k], results, backgrounds)."""
     ks = _check_ks(ks)
     bgopts = dict(reltol=1e-7, abstol=1e-7) | (bgopts or {})
     ptopts = dict(reltol=1e-5, abstol=1e-5, maxiters=100000) | (ptopts or {})
@@ -384,11 +387,16 @@ def solve_sets(M, par_sets, ks, bgopts=None, ptopts=None, ktini=1e-2, tauinimax=
         inst_set.append(np.full(len(ks), s, dtype=np.int32))
         inst_k.append(ks)
         inst_t0.append(np.clip(np.minimum(ktini / ks, tauinimax), bg.t[0], bg.t[-1]))
+    if algo is None:
+        from .autotune import capacity
+        from .solver import ALGO_AUTO, ALGO_WAVE
+        algo = ALGO_AUTO if (lanes == 0 and len(par_sets) * len(ks) >= 2 * capacity(gm, ALGO_WAVE)) else 0
     plan = gm.plan(np.concatenate(inst_set), np.concatenate(inst_k), np.concatenate(inst_t0),
-                   rtol=ptopts["reltol"], atol=ptopts["abstol"], maxiters=ptopts["maxiters"], lanes=lanes)
+                   rtol=ptopts["reltol"], atol=ptopts["abstol"], maxiters=ptopts["maxiters"], lanes=lanes, algo=algo)
     ms = plan.run()
     if timing is not None:
         timing["kernel_ms"] = ms
+        timing["algo"] = plan.algo
     res = plan.fetch()
     pk = plan.pk().reshape(len(par_sets), len(ks))
     plan.close()

@@ -154,6 +154,23 @@ def test_wave2h_forced_matches_oracle_and_one_mode_kernel():
         gm.plan(iset[:4], kk[:4], tt[:4], algo=ALGO_WAVE2H, saveat=np.array([1.0]))
 
 
+def test_solve_sets_takes_the_autotuned_variant_for_large_batches():
+    # 8 sets x 512 k = 4,096 instances, above twice the one-mode capacity: solve_sets times the
+    # variants and keeps the four-modes kernel; its spectra match the one-mode kernel's
+    from symboltz.problem import solve_sets
+    from symboltz.solver import ALGO_WAVE4
+    M, sets = workloads.lhs_sets(128, seed=0, take=8)
+    ks = k_dimensionless(np.logspace(-5, 1, 512), sets[0]["h"])
+    say(f"case: solve_sets, {len(sets)} sets x {len(ks)} k, default variant choice against ALGO_WAVE")
+    tm, tm1 = {}, {}
+    pk, res, _ = solve_sets(M, sets, ks, timing=tm)
+    pk1, res1, _ = solve_sets(M, sets, ks, timing=tm1, algo=ALGO_WAVE)
+    say(f"chosen algo {tm['algo']}: {tm['kernel_ms']:.1f} ms; ALGO_WAVE {tm1['kernel_ms']:.1f} ms")
+    assert tm["algo"] == ALGO_WAVE4
+    assert np.all(res["retcode"] == 0) and np.all(res1["retcode"] == 0)
+    assert np.max(np.abs(pk / pk1 - 1)) < 1e-6
+
+
 def test_lhs_sets_above_capacity_both_variants_match_oracle():
     # twice the one-per-SIMD capacity: both occupancy builds, forced, against the oracle
     from symboltz.autotune import capacity
