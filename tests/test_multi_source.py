@@ -1,7 +1,7 @@
 """The mixed-model sources (symboltz.multi): the one-mode kernel and the kernel with the
 several-modes bodies, each family in its own namespace, with the family tables the host checks
-(sbz_multi_create / sbz_multi_attach_rows).  CPU only: the sources are inspected and, for the
-one-mode kernel, compiled for gfx950."""
+(sbz_multi_create / sbz_multi_attach_rows).  CPU only: the sources are inspected (the build
+compiles both for gfx950)."""
 import re
 
 from symboltz import workloads
