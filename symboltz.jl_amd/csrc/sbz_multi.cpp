@@ -104,7 +104,7 @@ static int load_multi_module(const char* who, const char* hip_src, const char* c
         st = SBZ_ERR_COMPILE;
     } else if (hipModuleGetGlobal(&dn, &bytes, *mod, "sbz_multi_n") != hipSuccess || bytes != sizeof(nf) ||
                hipMemcpyDtoH(nf, dn, sizeof(nf)) != hipSuccess ||
-               hipModuleGetGlobal(&dn, &bytes, *mod, "sbz_multi_rows") != hipSuccess ||
+               hipModuleGetGlobal(&dn, &bytes, *mod, "sbz_multi_modes") != hipSuccess ||
                bytes != sizeof(int) * kMultiMaxF || hipMemcpyDtoH(rows, dn, sizeof(int) * kMultiMaxF) != hipSuccess) {
         sbz_set_error("%s: code object lacks its family tables", who);
         st = SBZ_ERR_COMPILE;

@@ -84,7 +84,7 @@ constexpr unsigned long sbz_cmax(unsigned long a, unsigned long b) {{ return a >
 // state size of each family (0: unused slot); the host checks it against the models it was given
 __constant__ int sbz_multi_n[{MAX_FAMILIES}] = {{{ns}}};
 // modes per wavefront of each family's several-modes body (1: none; 2, 4: tickets allowed)
-__constant__ int sbz_multi_rows[{MAX_FAMILIES}] = {{{rw}}};""")
+__constant__ int sbz_multi_modes[{MAX_FAMILIES}] = {{{rw}}};""")
 
     def kernel(name, with_rows):
         k = [f"""

@@ -44,8 +44,11 @@ def model_source(M):
 
 
 def _deps_text():
+    """The other files the emitted source is made of (their changes must change the tag)."""
     base = Path(__file__).resolve().parent.parent / "csrc"
-    return "".join((base / f).read_text() for f in ("sbz_thermo.h", "sbz_dual.h", "sbz_rodas5p.h"))
+    inc = Path(__file__).resolve().parent.parent.parent / "include"
+    return ("".join((base / f).read_text() for f in ("sbz_thermo.h", "sbz_dual.h", "sbz_rodas5p.h")) +
+            "".join((inc / f).read_text() for f in ("sbz.h", "sbz_params.h")))
 
 
 class _Stored:

@@ -30,7 +30,7 @@ def test_two_kernels_two_sources():
         assert f"sbzf{f}::sbz_waverow_run<true, 32>" in body
     assert "sbzf4::sbz_waverow_run" not in body
     for src in (one, rows):
-        assert re.search(r"__constant__ int sbz_multi_rows\[8\] = \{4, 2, 2, 2, 1\};", src)
+        assert re.search(r"__constant__ int sbz_multi_modes\[8\] = \{4, 2, 2, 2, 1\};", src)
         assert "__constant__ int sbz_multi_n[8]" in src
         for f in range(len(fams)):
             assert f"namespace sbzf{f} {{" in src
