@@ -158,6 +158,22 @@ def bench_mixed(args):
     log(f"one launch {kern_ms:.1f} ms vs one launch per family {tm2['solve_ms']:.1f} ms (serial), "
         f"{tm3['solve_ms']:.1f} ms (concurrent streams); all success {ok}; "
         f"{achieved:.3f} TFLOP/s")
+    # error against the same solver at rtol = atol = 1e-8, on the first two requests of every family
+    ref_err = None
+    if not args.no_ref:
+        sub = [i for f in range(len(fams)) for i in [j for j, r in enumerate(reqs) if r[0] == f][:2]]
+        ref = MixedStream(fams, ks, rtol=1e-8, atol=1e-8)
+        try:
+            lpr, _, retr, _ = ref.solve([reqs[i] for i in sub])
+        finally:
+            ref.close()
+        err = np.abs(np.expm1(lp[sub] - lpr)).max(axis=1)
+        ref_err = {f"n={fams[reqs[i][0]].n_unknowns}": 0.0 for i in sub}
+        for e, i in zip(err, sub):
+            key = f"n={fams[reqs[i][0]].n_unknowns}"
+            ref_err[key] = max(ref_err[key], float(e))
+        log(f"max |P(rtol 1e-5) / P(rtol 1e-8) - 1| by family (first two requests each): {ref_err} "
+            f"(reference run success: {bool(np.all(retr == 0))})")
     cpu = None
     if not args.no_cpu:
         from oracle import oracle
@@ -193,6 +209,8 @@ def bench_mixed(args):
                                   f"(n = {[M.n_unknowns for M in fams]}), Latin-hypercube parameters (seed 0), "
                                   f"256 k log-spaced in [1e-4, 5]/Mpc: {n_modes} k-modes in one launch; rtol=atol=1e-5",
                    "k_mode_solves_per_s": n_modes / elapsed, "all_success": ok,
+                   "pk_max_rel_err_vs_rtol_1e-8": None if ref_err is None else max(ref_err.values()),
+                   "pk_rel_err_vs_rtol_1e-8_by_family": ref_err,
                    "one_launch_kernel_ms": kern_ms, "per_family_launches_kernel_ms": tm2["solve_ms"],
                    "per_family_concurrent_ms": tm3["solve_ms"],
                    "stage_ms_per_step": {k: float(np.mean([t[k] for t in tms])) for k in
