@@ -406,7 +406,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massless10", "cmb", "bg", "mixed", "mlp_train"])
+    ap.add_argument("--workload", default="lhs128", choices=["lhs128", "massive_nu100", "massive_nu100_lhs64", "massless10", "cmb", "bg", "mixed",
+                                                      "mlp_train"])
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--algo", choices=["auto", "wave", "wave2", "wave4", "wave2h", "wave_altlu"], default="auto",
                     help="auto: timed choice of the kernel variant (default); wave: one wavefront per "

@@ -17,6 +17,8 @@ A_s = 2.1e-9, n_s = 0.966, hydrogen reionization at z = 7.7.
 The reference has no τ_reion parameter (reionization is fixed at z = 7.6711 in baryons.jl:202-203);
 z_re is this build's parameter, and τ_reion is converted by `problem.z_re_for_optical_depth`.
 
+- "massive_nu100_lhs64": the n = 100 model over the first 64 "lhs128" parameter sets (their z_re)
+  × the 1024 k of "massive_nu100", P(k) only: the same model at a throughput-bound size.
 - "mixed": the mixed-model stream (`mixed_families`, `mixed_requests`): requests spread over five
   model families with 38 to 236 unknowns (ΛCDM with massless ν; three degenerate massive ν, the
   surrogate's family; the w0wa fluid; one massive ν at lmax 16; three massive ν at lmax 32), each
@@ -85,6 +87,15 @@ def workload(name):
                 "1024 k log-spaced in [1e-5, 10]/Mpc; rtol=atol=1e-5; P(k) at z=0 and CMB sources "
                 "(S_T, S_E, S_psi) on a 2000-point cosgrid of conformal time")
         return M, [pars], ks, desc
+    if name == "massive_nu100_lhs64":
+        M, base = massive_nu100()
+        _, lhs = lhs_sets(128, seed=0, take=64)
+        sets = [dict(base, **{q: p[q] for q in ("h", "Omega_b0", "Omega_c0", "ln_As1e10", "ns", "z_re")}) for p in lhs]
+        ks = np.logspace(-5, 1, 1024)
+        desc = ("the n=100 massive-nu model (one massive nu 0.06 eV, 4 bins, lmax 10; photon/massless-nu lmax 16) "
+                "over the first 64 Latin-hypercube sets of lhs128 (their h, omega_b, omega_c, ln 1e10 As, ns, z_re) "
+                "x 1024 k log-spaced in [1e-5, 10]/Mpc = 65536 instances in one launch; rtol=atol=1e-5; P(k) at z=0")
+        return M, sets, ks, desc
     if name == "massless10":
         M = models.LCDM(lmax=10, h=False)
         ks = np.logspace(-4, 0, 100)
