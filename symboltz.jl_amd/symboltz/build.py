@@ -81,9 +81,11 @@ def standard_models():
 
 
 def standard_multis():
-    """Model lists compiled ahead of time into one mixed-model code object each (symboltz.multi)."""
+    """Model lists compiled ahead of time into mixed-model code objects (symboltz.multi): the mixed
+    stream's five families, and the three models of the problem-level test."""
+    from .models import LCDM, w0waCDM
     from .workloads import mixed_families
-    return [mixed_families()]
+    return [mixed_families(), [LCDM(lmax=10, h=False), w0waCDM(lmax=6), LCDM(lmax=6)]]
 
 
 def build_models(models=None, jobs=4, verbose=False, with_multi=None):
