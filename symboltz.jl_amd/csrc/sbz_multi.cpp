@@ -273,6 +273,12 @@ extern "C" int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans,
         use_rows[f] = mm->k_rows && mm->rows[f] > 1 && plans[f]->n_inst >= 2 * mm->resident_rows;
         rows_run = rows_run || use_rows[f];
     }
+    MCHK(hipSetDevice(mm->device));
+    // work a plan still has on its model's stream (an sbz_plan_launch not yet waited for, a pending
+    // fetch) finishes before the launch writes the plan's outputs; copies of an earlier call that
+    // stopped on an error finish before the host tables they read are rewritten
+    for (int f = 0; f < n_plans; ++f) MCHK(hipStreamSynchronize(plans[f]->m->stream));
+    MCHK(hipStreamSynchronize(mm->stream));
     mm->h_ford.clear();
     for (int f = 0; f < n_plans; ++f) {
         const sbz_plan* p = plans[f];
@@ -298,11 +304,6 @@ extern "C" int sbz_multi_run(sbz_multi* mm, sbz_plan* const* plans, int n_plans,
         if (kernel_ms) *kernel_ms = 0.0f;
         return SBZ_OK;
     }
-    MCHK(hipSetDevice(mm->device));
-    // work a plan still has on its model's stream (an sbz_plan_launch not yet waited for, a pending
-    // fetch) finishes before the launch writes the plan's outputs
-    for (int f = 0; f < n_plans; ++f) MCHK(hipStreamSynchronize(plans[f]->m->stream));
-    MCHK(hipStreamSynchronize(mm->stream));   // copies of an earlier call that stopped on an error
     if ((size_t)n > mm->cap) {
         if (mm->d_fam) MCHK(hipFree(mm->d_fam));
         if (mm->d_inst) MCHK(hipFree(mm->d_inst));
