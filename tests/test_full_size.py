@@ -2,8 +2,9 @@
 oracle run of every instance: every mode succeeds; the results of an instance do not depend on
 where it sits in the batch (the batch reversed gives bit-identical states, step counts and P(k));
 and a random sample of instances matches the oracle at 1e-4 (P(k), the parity bar of the smaller
-tests).  Workloads: the 128-set Latin-hypercube batch (65,536 instances, n = 38) and the n = 100
-massive-ν model at 1024 k with CMB sources on 2000 τ."""
+tests).  Workloads: the 128-set Latin-hypercube batch (65,536 instances, n = 38), the n = 100
+massive-ν model over 64 of those sets × 1024 k (65,536 instances), and the n = 100 model at 1024 k
+with CMB sources on 2000 τ."""
 import sys
 
 import numpy as np
@@ -58,6 +59,43 @@ def test_latin_hypercube_batch_full_size():
     # a random sample against the oracle
     rng = np.random.default_rng(7)
     pick = rng.choice(len(ks), 48, replace=False)
+    worst = 0.0
+    for si in np.unique(iset[pick]):
+        m = pick[iset[pick] == si]
+        S = Setup(M, sets[si])
+        P, ro = S.O.spectrum_matter(ks[m])
+        assert np.all(ro["retcode"] == 0)
+        worst = max(worst, float(np.max(np.abs(pk[m] / P - 1))))
+    say(f"sample of {len(pick)} instances: max |P/P_oracle - 1| = {worst:.2e}")
+    assert worst < 1e-4
+    plan.close()
+    plan2.close()
+    gm.close()
+
+
+@pytest.mark.timeout(600)
+def test_massive_nu_batched_full_size():
+    # the n = 100 model over 64 sets x 1024 k: the autotuner's variant (two modes per wavefront)
+    M, sets, ks_mpc, _ = workloads.workload("massive_nu100_lhs64")
+    say(f"case: {len(sets)} sets x {len(ks_mpc)} k = {len(sets) * len(ks_mpc)} instances, n = {M.n_unknowns}")
+    bgs = solvebg_sets([CosmologyProblem(M, p) for p in sets])
+    gm = GpuModel(M)
+    iset, ks, t0 = _instances(M, sets, bgs, ks_mpc, gm)
+    plan = gm.plan(iset, ks, t0, algo=ALGO_AUTO)
+    say(f"kernel variant {plan.algo}: {plan.run():.1f} ms")
+    r = plan.fetch()
+    pk = plan.pk()
+    assert np.all(r["retcode"] == 0)
+    assert np.all(np.isfinite(pk)) and np.all(pk > 0)
+    rev = np.arange(len(ks))[::-1].copy()
+    plan2 = gm.plan(iset[rev], ks[rev], t0[rev], algo=plan.algo)
+    plan2.run()
+    r2 = plan2.fetch()
+    np.testing.assert_array_equal(r2["u"][::-1], r["u"])
+    np.testing.assert_array_equal(r2["stats"][::-1], r["stats"])
+    np.testing.assert_array_equal(plan2.pk()[::-1], pk)
+    rng = np.random.default_rng(11)
+    pick = rng.choice(len(ks), 12, replace=False)
     worst = 0.0
     for si in np.unique(iset[pick]):
         m = pick[iset[pick] == si]
