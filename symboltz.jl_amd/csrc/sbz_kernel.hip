@@ -488,30 +488,78 @@ SBZ_DEV void sbz_stage_rhs(int lane, const double (*v)[SBZ_N], double dt, double
 // The same two combinations with one read of each stage vector: the input pass also forms this
 // lane's  ck = Σ_{j<S} (c_Sj / h) K_j  and keeps it in registers across the RHS application (for
 // the one-wavefront-per-SIMD kernels; at two per SIMD the extra live registers cost more spills)
+// With SBZ_STAGE_PRELOAD every load of a pass is unconditional (a lane past the state reads
+// component N - 1 and discards it), so all of them issue together ahead of the arithmetic and only
+// the stores are masked; without it each component's loads sit behind its bounds check, holding
+// fewer registers (the mixed-model kernel with the several-modes bodies, short of them, takes that
+// form).  The arithmetic is the same either way.
+#ifndef SBZ_STAGE_PRELOAD
+#define SBZ_STAGE_PRELOAD 1
+#endif
 template <int S, int LN>
 SBZ_DEV void sbz_stage_in_f(int lane, const double (*v)[SBZ_N], const double* Un, double* US, double idt, double* ck) {
+    if constexpr (!SBZ_STAGE_PRELOAD) {
 #pragma unroll
-    for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
-        const int i = lane + LN * r;
-        if (i < SBZ_N) {
-            double acc = Un[i], cc = 0.0;
+        for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+            const int i = lane + LN * r;
+            if (i < SBZ_N) {
+                double acc = Un[i], cc = 0.0;
 #pragma unroll
-            for (int j = 1; j < S; ++j) {
-                const double kj = v[5 + j][i];
-                acc += SbzR5::A[S][j - 1] * kj;
-                cc += (SbzR5::C[S][j - 1] * idt) * kj;
+                for (int j = 1; j < S; ++j) {
+                    const double kj = v[5 + j][i];
+                    acc += SbzR5::A[S][j - 1] * kj;
+                    cc += (SbzR5::C[S][j - 1] * idt) * kj;
+                }
+                US[i] = acc;
+                ck[r] = cc;
             }
-            US[i] = acc;
-            ck[r] = cc;
         }
+        return;
+    }
+    constexpr int OWNL = (SBZ_N + LN - 1) / LN;
+    double un[OWNL], kv[OWNL][S > 1 ? S - 1 : 1];
+#pragma unroll
+    for (int r = 0; r < OWNL; ++r) {
+        const int i = (lane + LN * r < SBZ_N) ? lane + LN * r : SBZ_N - 1;
+        un[r] = Un[i];
+#pragma unroll
+        for (int j = 1; j < S; ++j) kv[r][j - 1] = v[5 + j][i];
+    }
+#pragma unroll
+    for (int r = 0; r < OWNL; ++r) {
+        const int i = lane + LN * r;
+        double acc = un[r], cc = 0.0;
+#pragma unroll
+        for (int j = 1; j < S; ++j) {
+            acc += SbzR5::A[S][j - 1] * kv[r][j - 1];
+            cc += (SbzR5::C[S][j - 1] * idt) * kv[r][j - 1];
+        }
+        if (i < SBZ_N) US[i] = acc;
+        ck[r] = cc;
     }
 }
 template <int S, int LN>
 SBZ_DEV void sbz_stage_rhs_f(int lane, double dt, const double* DT, double* Ks, const double* ck) {
+    if constexpr (!SBZ_STAGE_PRELOAD) {
 #pragma unroll
-    for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+        for (int r = 0; r < (SBZ_N + LN - 1) / LN; ++r) {
+            const int i = lane + LN * r;
+            if (i < SBZ_N) Ks[i] = (Ks[i] + (dt * SbzR5::d[S]) * DT[i]) + ck[r];
+        }
+        return;
+    }
+    constexpr int OWNL = (SBZ_N + LN - 1) / LN;
+    double kk[OWNL], dd[OWNL];
+#pragma unroll
+    for (int r = 0; r < OWNL; ++r) {
+        const int i = (lane + LN * r < SBZ_N) ? lane + LN * r : SBZ_N - 1;
+        kk[r] = Ks[i];
+        dd[r] = DT[i];
+    }
+#pragma unroll
+    for (int r = 0; r < OWNL; ++r) {
         const int i = lane + LN * r;
-        if (i < SBZ_N) Ks[i] = (Ks[i] + (dt * SbzR5::d[S]) * DT[i]) + ck[r];
+        if (i < SBZ_N) Ks[i] = (kk[r] + (dt * SbzR5::d[S]) * dd[r]) + ck[r];
     }
 }
 #define SBZ_STAGE_SWITCH_LN(F, LN, ...)                                                           \

@@ -53,7 +53,8 @@ def multi_source(models, rows_kernel=False):
     abi = set(_macro_names((INCLUDE / "sbz.h").read_text()) + _macro_names((INCLUDE / "sbz_params.h").read_text()))
     for f, M in enumerate(models):
         block = model_source(M).full_src
-        out += [f"namespace sbzf{f} {{", "#define SBZ_MULTI_FAMILY 1", block,
+        pre = ["#define SBZ_STAGE_PRELOAD 0"] if rows_kernel else []   # the kernel short of registers
+        out += [f"namespace sbzf{f} {{", "#define SBZ_MULTI_FAMILY 1"] + pre + [block,
                 "constexpr int SBZ_MULTI_N = SBZ_N;", f"}}  // namespace sbzf{f}"]
         # every macro of the block is the block's own, so the next model defines its own values,
         # except the C ABI's constants: the ABI header (return codes, parameter slots, and the
