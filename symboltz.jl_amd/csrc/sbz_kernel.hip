@@ -141,17 +141,39 @@ SBZ_DEV void sbz_pt_background(const double* __restrict__ P, const double* __res
     const double* ku = kt + nk;
     const double* kdu = kt + 6 * (size_t)nk;
     double tv = dval(tau);
-    while (idx > 0 && tv < kt[idx]) --idx;
-    while (idx < nk - 2 && tv >= kt[idx + 1]) ++idx;
-    double t0 = kt[idx], h = kt[idx + 1] - t0;
+    // The interval of the cursor and its spline data are loaded together: τ usually still lies in
+    // it, and then one round trip to memory serves both the search and the spline.  Otherwise the
+    // cursor walks to τ's interval and the data are loaded again.  (idx stays in [0, nk - 2], and
+    // every set has nk >= 2 knots.)
+    double ta = kt[idx], tb = kt[idx + 1];
+    double ua[5], ub[5], da[5], db[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        ua[j] = ku[idx * 5 + j];
+        ub[j] = ku[(idx + 1) * 5 + j];
+        da[j] = kdu[idx * 5 + j];
+        db[j] = kdu[(idx + 1) * 5 + j];
+    }
+    if ((idx > 0 && tv < ta) || (idx < nk - 2 && tv >= tb)) {
+        while (idx > 0 && tv < kt[idx]) --idx;
+        while (idx < nk - 2 && tv >= kt[idx + 1]) ++idx;
+        ta = kt[idx];
+        tb = kt[idx + 1];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            ua[j] = ku[idx * 5 + j];
+            ub[j] = ku[(idx + 1) * 5 + j];
+            da[j] = kdu[idx * 5 + j];
+            db[j] = kdu[(idx + 1) * 5 + j];
+        }
+    }
+    double t0 = ta, h = tb - t0;
     T x = (tau - t0) / h;
     T x2 = x * x, x3 = x2 * x;
     T h00 = 2.0 * x3 - 3.0 * x2 + 1.0, h10 = x3 - 2.0 * x2 + x, h01 = -2.0 * x3 + 3.0 * x2, h11 = x3 - x2;
     T s[5];
 #pragma unroll
-    for (int j = 0; j < 5; ++j)
-        s[j] = h00 * ku[idx * 5 + j] + h10 * (h * kdu[idx * 5 + j]) + h01 * ku[(idx + 1) * 5 + j] +
-               h11 * (h * kdu[(idx + 1) * 5 + j]);
+    for (int j = 0; j < 5; ++j) s[j] = h00 * ua[j] + h10 * (h * da[j]) + h01 * ub[j] + h11 * (h * db[j]);
     SbzDens<T> d;
     sbz_densities<T>(P, s[0], d);
     SbzThermo<T> th;
