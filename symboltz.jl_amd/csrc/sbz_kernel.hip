@@ -1049,6 +1049,9 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
     using RO = SbzRows<LN>;
     using BC = typename RO::BC;
     constexpr int OWN = (SBZ_N + LN - 1) / LN;
+    // the per-attempt passes over a row's components issue all their loads first at 16-lane rows
+    // (registers to spare); 32-lane rows and the mixed kernel keep the bounds-checked loops
+    constexpr bool LOADS_FIRST = SBZ_STAGE_PRELOAD && LN == 16;
     const int g = threadIdx.x / LN;
     WaveRow& L = R4[g];
     WaveCtx c;
@@ -1291,7 +1294,20 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
         for (int s = 1; s <= 9; ++s) {
             double* Ks = (s <= 8) ? L.v[5 + s] : F1;
             if (s == 1) {
-                for (int i = c.lane; i < SBZ_N; i += LN) Ks[i] = F0[i] + h * SBZ_R5_D1 * DT[i];
+                if constexpr (LOADS_FIRST) {   // every load first, masked stores
+                    double f0[OWN], d0[OWN];
+#pragma unroll
+                    for (int r = 0; r < OWN; ++r) {
+                        const int i = (c.lane + LN * r < SBZ_N) ? c.lane + LN * r : SBZ_N - 1;
+                        f0[r] = F0[i];
+                        d0[r] = DT[i];
+                    }
+#pragma unroll
+                    for (int r = 0; r < OWN; ++r)
+                        if (c.lane + LN * r < SBZ_N) Ks[c.lane + LN * r] = f0[r] + h * SBZ_R5_D1 * d0[r];
+                } else {
+                    for (int i = c.lane; i < SBZ_N; i += LN) Ks[i] = F0[i] + h * SBZ_R5_D1 * DT[i];
+                }
             } else {
                 double* in = (s <= 8) ? US : U1;
                 double ck[OWN];
@@ -1344,13 +1360,36 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
                 // u_{n+1} = stage-8 input + k8; error estimate k8
                 double ss = 0.0;
                 bool fin = true;
-                for (int i = c.lane; i < SBZ_N; i += LN) {
-                    const double un1 = US[i] + Ks[i];
-                    U1[i] = un1;
-                    const double sc = a.atol + fmax(fabs(Un[i]), fabs(un1)) * a.rtol;
-                    const double r = Ks[i] / sc;
-                    ss += r * r;
-                    if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                if constexpr (LOADS_FIRST) {   // every load first, as for stage 1
+                    double us[OWN], kv[OWN], uo[OWN];
+#pragma unroll
+                    for (int r = 0; r < OWN; ++r) {
+                        const int i = (c.lane + LN * r < SBZ_N) ? c.lane + LN * r : SBZ_N - 1;
+                        us[r] = US[i];
+                        kv[r] = Ks[i];
+                        uo[r] = Un[i];
+                    }
+#pragma unroll
+                    for (int r = 0; r < OWN; ++r) {
+                        const int i = c.lane + LN * r;
+                        if (i < SBZ_N) {
+                            const double un1 = us[r] + kv[r];
+                            U1[i] = un1;
+                            const double sc = a.atol + fmax(fabs(uo[r]), fabs(un1)) * a.rtol;
+                            const double q = kv[r] / sc;
+                            ss += q * q;
+                            if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                        }
+                    }
+                } else {
+                    for (int i = c.lane; i < SBZ_N; i += LN) {
+                        const double un1 = US[i] + Ks[i];
+                        U1[i] = un1;
+                        const double sc = a.atol + fmax(fabs(Un[i]), fabs(un1)) * a.rtol;
+                        const double q = Ks[i] / sc;
+                        ss += q * q;
+                        if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                    }
                 }
                 ss = RO::sum(ss);
                 const bool fin_row = RO::all(fin, g);   // a ballot: evaluated by every lane
