@@ -629,6 +629,9 @@ SBZ_DEV WaveCtx sbz_wave_ctx(int lane) {
 // final state, return code and statistics; every lane of the wavefront calls it.
 template <bool FUSED, bool SPLIT = SBZ_LU_SPLIT>
 SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDSt<SPLIT>& L, const WaveCtx& c, int inst) {
+    // the per-attempt passes over the components issue all their loads first (the fused one-wavefront-
+    // per-SIMD kernels, with registers to spare)
+    constexpr bool LOADS_FIRST = FUSED && SBZ_STAGE_PRELOAD;
     double* const DT = L.v[4];
     double* const US = L.v[5];
     double* const TMP = L.v[8];   // source scratch: K3 is dead between step attempts (rewritten at stage 3)
@@ -841,7 +844,20 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDSt<SPLIT>& L, const Wave
                 for (int s = 1; s <= 9; ++s) {
                     double* Ks = (s <= 8) ? L.v[5 + s] : F1;
                     if (s == 1) {
-                        for (int i = c.lane; i < SBZ_N; i += 64) Ks[i] = F0[i] + dt * SBZ_R5_D1 * DT[i];
+                        if constexpr (LOADS_FIRST) {   // every load first, masked stores
+                            double f0[SBZ_OWN], d0[SBZ_OWN];
+#pragma unroll
+                            for (int r = 0; r < SBZ_OWN; ++r) {
+                                const int i = (c.lane + 64 * r < SBZ_N) ? c.lane + 64 * r : SBZ_N - 1;
+                                f0[r] = F0[i];
+                                d0[r] = DT[i];
+                            }
+#pragma unroll
+                            for (int r = 0; r < SBZ_OWN; ++r)
+                                if (c.lane + 64 * r < SBZ_N) Ks[c.lane + 64 * r] = f0[r] + dt * SBZ_R5_D1 * d0[r];
+                        } else {
+                            for (int i = c.lane; i < SBZ_N; i += 64) Ks[i] = F0[i] + dt * SBZ_R5_D1 * DT[i];
+                        }
                     } else {
                         double* in = (s <= 8) ? US : U1;
                         double ck[SBZ_OWN];   // fused: Σ_{j<s} (c_sj / h) K_j of this lane's components
@@ -894,13 +910,36 @@ SBZ_DEV void sbz_wave_mode(const SbzSolveArgs& a, WaveLDSt<SPLIT>& L, const Wave
                         // u_{n+1} = stage-8 input + k8; error estimate k8
                         double ss = 0.0;
                         bool fin = true;
-                        for (int i = c.lane; i < SBZ_N; i += 64) {
-                            const double un1 = US[i] + Ks[i];
-                            U1[i] = un1;
-                            const double sc = a.atol + fmax(fabs(Un[i]), fabs(un1)) * a.rtol;
-                            const double r = Ks[i] / sc;
-                            ss += r * r;
-                            if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                        if constexpr (LOADS_FIRST) {   // every load first, as for stage 1
+                            double us[SBZ_OWN], kv[SBZ_OWN], uo[SBZ_OWN];
+#pragma unroll
+                            for (int r = 0; r < SBZ_OWN; ++r) {
+                                const int i = (c.lane + 64 * r < SBZ_N) ? c.lane + 64 * r : SBZ_N - 1;
+                                us[r] = US[i];
+                                kv[r] = Ks[i];
+                                uo[r] = Un[i];
+                            }
+#pragma unroll
+                            for (int r = 0; r < SBZ_OWN; ++r) {
+                                const int i = c.lane + 64 * r;
+                                if (i < SBZ_N) {
+                                    const double un1 = us[r] + kv[r];
+                                    U1[i] = un1;
+                                    const double sc = a.atol + fmax(fabs(uo[r]), fabs(un1)) * a.rtol;
+                                    const double q = kv[r] / sc;
+                                    ss += q * q;
+                                    if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                                }
+                            }
+                        } else {
+                            for (int i = c.lane; i < SBZ_N; i += 64) {
+                                const double un1 = US[i] + Ks[i];
+                                U1[i] = un1;
+                                const double sc = a.atol + fmax(fabs(Un[i]), fabs(un1)) * a.rtol;
+                                const double q = Ks[i] / sc;
+                                ss += q * q;
+                                if (!(un1 == un1) || fabs(un1) > 1e300) fin = false;
+                            }
                         }
                         ss = sbz_wave_sum(ss);
                         EEst = __all(fin) ? sqrt(ss / SBZ_N) : __builtin_inf();
