@@ -442,10 +442,43 @@ SBZ_DEV void sbz_chain_bwd(double sk, const double* cinv, double xatt, double* k
 }
 #define SBZ_IN_GROUP(lo, hi) (c.lane >= (lo) && c.lane < (hi))
 
+// The chain rows with the chain's inputs already in registers (cv[e] = the member l = 3 + e).
+template <int LEN>
+SBZ_DEV void sbz_chain_rows_r(const SbzChainRow& r, double att, const double* cv, double* co) {
+    double v[LEN + 2];
+    v[0] = att;
+#pragma unroll
+    for (int e = 0; e < LEN; ++e) v[e + 1] = cv[e];
+    v[LEN + 1] = 0.0;
+#pragma unroll
+    for (int e = 0; e < LEN - 1; ++e) co[e] = r.sk * SBZ_RL(e) * v[e] + r.dd * v[e + 1] - r.sk * SBZ_RL1(e) * v[e + 2];
+    co[LEN - 1] = r.sk * v[LEN - 1] + r.top * v[LEN];
+}
+
 // out = J(τ) in, i.e. f(τ, in) (f is linear in u); with dB given, out = ∂f/∂τ at fixed `in`.
-// B, dB, Cs, in, out are LDS arrays; lane 0 writes the border rows, chain lanes their chains.
+// B, dB, Cs, in, out are LDS arrays (in and out distinct); lane 0 writes the border rows, chain
+// lanes their chains.  PREC: every lane loads its chain's inputs and coefficients before the border
+// block, so their latency overlaps it (a lane reads SBZ_MCHAIN members from its chain's base,
+// clamped to the state; the arithmetic is the same).
+template <bool PREC = false>
 SBZ_DEV void sbz_wave_apply(const WaveCtx& c, const double* B, const double* dB, const double* Cs, double k,
                             const double* in, double* out) {
+    if constexpr (PREC) {
+        double cv[SBZ_MCHAIN];
+#pragma unroll
+        for (int e = 0; e < SBZ_MCHAIN; ++e) cv[e] = in[(c.cbase + e < SBZ_N) ? c.cbase + e : SBZ_N - 1];
+        const double att = in[c.cattc];
+        const SbzChainRow r = dB ? sbz_chain_drow(c, B, dB, Cs, k) : sbz_chain_row(c, B, Cs, k);
+        if (dB) sbz_wb_dfdt(c.lane, B, dB, Cs, k, in, out);
+        else sbz_wb_f(c.lane, B, Cs, k, in, out);
+        if (c.isC) {
+#define SBZ_CG_ROWS_R(lo, hi, len) \
+            if (SBZ_IN_GROUP(lo, hi)) sbz_chain_rows_r<len>(r, att, cv, out + c.cbase);
+            SBZ_CHAIN_GROUPS(SBZ_CG_ROWS_R)
+#undef SBZ_CG_ROWS_R
+        }
+        return;
+    }
     if (dB) sbz_wb_dfdt(c.lane, B, dB, Cs, k, in, out);
     else sbz_wb_f(c.lane, B, Cs, k, in, out);
     if (c.isC) {
@@ -1217,7 +1250,7 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
                 // (dead) stage vectors and discard the result
                 const bool init = fresh && ret == SBZ_RET_SUCCESS && t1 > t;
                 double d0 = 0.0, d1 = 0.0;
-                sbz_wave_apply(c, L.B[0], nullptr, L.Cs, k, init ? L.v[0] : L.v[6], init ? L.v[2] : L.v[7]);
+                sbz_wave_apply<LOADS_FIRST>(c, L.B[0], nullptr, L.Cs, k, init ? L.v[0] : L.v[6], init ? L.v[2] : L.v[7]);
                 __syncthreads();
                 if (init)
                     for (int i = c.lane; i < SBZ_N; i += LN) {
@@ -1240,7 +1273,7 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
                     }
                 }
                 __syncthreads();
-                sbz_wave_apply(c, L.B[1], nullptr, L.Cs, k, init ? L.v[1] : L.v[6], init ? L.v[3] : L.v[7]);
+                sbz_wave_apply<LOADS_FIRST>(c, L.B[1], nullptr, L.Cs, k, init ? L.v[1] : L.v[6], init ? L.v[3] : L.v[7]);
                 __syncthreads();
                 double d2 = 0.0;
                 if (init)
@@ -1325,7 +1358,7 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
             wb.c22 = c11 * idet;
         }
         ok = RO::all(ok || !c.isB, g);
-        sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
+        sbz_wave_apply<LOADS_FIRST>(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)
         if (live) st_j++;
         __syncthreads();
         double EEst = __builtin_inf();
@@ -1355,7 +1388,7 @@ SBZ_DEV void sbz_waverow_run(const SbzSolveArgs& a, WaveRow* R4) {
                     else SBZ_STAGE_SWITCH_LN(sbz_stage_in, LN, c.lane, L.v, Un, US)
                 }
                 __syncthreads();
-                sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
+                sbz_wave_apply<LOADS_FIRST>(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);
                 // pass 9 runs for a row that rejected when another row accepted: not counted, so
                 // a mode's statistics do not depend on the modes that share its wavefront
                 if (live && (s <= 8 || EEst <= 1.0)) st_f++;

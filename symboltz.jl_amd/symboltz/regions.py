@@ -58,11 +58,11 @@ def wave4_source(src):
          "        " + T(3) + "\n        const double xc = (c.isB && c.bchain >= 0) ? L.C[c.bchain] : 0.0;\n"),
         ("        bool ok = sbz_wb_lu<BC>(c.lane, row, rpo);\n",
          "        " + T(4) + "\n        bool ok = sbz_wb_lu<BC>(c.lane, row, rpo);\n        " + T(5) + "\n"),
-        ("        sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)\n",
-         "        " + T(6) + "\n        sbz_wave_apply(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)\n"),
+        ("        sbz_wave_apply<LOADS_FIRST>(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)\n",
+         "        " + T(6) + "\n        sbz_wave_apply<LOADS_FIRST>(c, B0, L.dB, L.Cs, k, Un, DT);  // ∂f/∂τ at (t, u_n)\n"),
         ("        if (live) st_j++;\n        __syncthreads();\n", "        if (live) st_j++;\n        __syncthreads();\n        " + T(7) + "\n"),
-        ("                __syncthreads();\n                sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);\n",
-         "                __syncthreads();\n                " + T(8) + "\n                sbz_wave_apply(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);\n"),
+        ("                __syncthreads();\n                sbz_wave_apply<LOADS_FIRST>(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);\n",
+         "                __syncthreads();\n                " + T(8) + "\n                sbz_wave_apply<LOADS_FIRST>(c, L.B[sbz_stage_bg[s]], nullptr, L.Cs, k, in, Ks);\n"),
         ("                if (live && (s <= 8 || EEst <= 1.0)) st_f++;\n                __syncthreads();\n",
          "                if (live && (s <= 8 || EEst <= 1.0)) st_f++;\n                __syncthreads();\n                " + T(9) + "\n"),
         ("            __syncthreads();\n            // solve W x = Ks:", "            __syncthreads();\n            " + T(10) + "\n            // solve W x = Ks:"),
